@@ -1,0 +1,392 @@
+"""cpvs_amd -- precomputed voxelized shadows (TobiasRp/cpvs build + lookup path) on MI355X.
+
+Python mirror of the reference's C++ interface, over the C ABI of ``libcpvs_amd.so``
+(include/cpvs.h).  Names, argument meaning and ownership follow the reference:
+
+* ``MinMaxHierarchy(image)``                      -- src/MinMaxHierarchy.h:23-72
+* ``CompressedShadow.create(minMax, zTileIndex, zTileNum)`` and ``traverse``, ``getDAG``,
+  ``getNumLevels``, ``getTotalVisibility``         -- src/CompressedShadow.h:20-125
+* ``CompressedShadowContainer(length | shadow)`` with ``set``/``get``/``copyToGPU``/
+  ``moveToGPU``/``freeOnCPU``/``setFilterSize``/``evaluate`` -- src/CompressedShadowContainer.h:18-92
+
+Every computation runs as HIP kernels on the GPU; DAGs stay in device memory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import CpvsError, call
+
+__all__ = ["Context", "DeviceArray", "MinMaxHierarchy", "CompressedShadow", "CompressedShadowContainer",
+           "CpvsError", "default_context", "gen_depth", "gen_queries_uniform", "gen_queries_gbuffer"]
+
+
+class Context:
+    """One HIP device, its stream and build scratch (cpvs_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        call("cpvs_ctx_create", device, C.byref(h))
+        self.handle = h
+        self.device = device
+
+    def synchronize(self):
+        call("cpvs_ctx_synchronize", self.handle)
+
+    @property
+    def stream(self):
+        s = C.c_void_p()
+        call("cpvs_ctx_stream", self.handle, C.byref(s))
+        return s
+
+    def profile(self, mode: int):
+        """1: time every kernel with HIP events, 0: stop, -1: clear totals."""
+        call("cpvs_ctx_profile", self.handle, mode)
+
+    def profile_read(self) -> dict:
+        """{kernel name: (total ms, launches)} accumulated while profiling was on."""
+        buf = C.create_string_buffer(1 << 16)
+        call("cpvs_ctx_profile_read", self.handle, buf, len(buf))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, ms, cnt = line.split("\t")
+            out[name] = (float(ms), int(cnt))
+        return out
+
+    def event(self, stream=None):
+        e = C.c_void_p()
+        call("cpvs_event_record", self.handle, stream, C.byref(e))
+        return e
+
+    @staticmethod
+    def elapsed_ms(start, stop) -> float:
+        ms = C.c_float()
+        call("cpvs_event_elapsed_ms", start, stop, C.byref(ms))
+        return float(ms.value)
+
+    @staticmethod
+    def destroy_event(e):
+        call("cpvs_event_destroy", e)
+
+    def close(self):
+        if self.handle:
+            _lib.load().cpvs_ctx_destroy(self.handle)
+            self.handle = None
+
+
+_default: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+class DeviceArray:
+    """A block of device memory with a numpy dtype/shape (allocated through the C ABI)."""
+
+    def __init__(self, shape, dtype, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        call("cpvs_device_alloc", self.ctx.handle, self.nbytes, C.byref(p))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray, ctx: Optional[Context] = None) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype, ctx)
+        if a.nbytes:
+            call("cpvs_memcpy_h2d", d.ctx.handle, d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes)
+        return d
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            call("cpvs_memcpy_d2h", self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes)
+        return out
+
+    def free(self):
+        if self.ptr:
+            call("cpvs_device_free", self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.handle:
+                _lib.load().cpvs_device_free(self.ctx.handle, self.ptr)
+        except Exception:
+            pass
+
+
+class MinMaxHierarchy:
+    """MinMaxHierarchy(const ImageF&) (src/MinMaxHierarchy.h:23, .cpp:9-97) on the GPU."""
+
+    def __init__(self, image, all_levels: bool = False, borrow: bool = False, ctx: Optional[Context] = None,
+                 stream=None):
+        self.ctx = ctx or default_context()
+        flags = _lib.CPVS_PYR_ALL_LEVELS if all_levels else 0
+        h = C.c_void_p()
+        if isinstance(image, DeviceArray):
+            N = image.shape[0]
+            assert image.dtype == np.float32 and image.shape == (N, N)
+            if borrow:
+                flags |= _lib.CPVS_PYR_BORROW_DEPTH
+            self._keep = image if borrow else None
+            call("cpvs_pyramid_build", self.ctx.handle, image.ptr, 1, N, flags, stream, C.byref(h))
+        else:
+            img = np.ascontiguousarray(image, np.float32)
+            if img.ndim != 2 or img.shape[0] != img.shape[1]:
+                raise ValueError("MinMaxHierarchy needs a square 2-D image")
+            N = img.shape[0]
+            call("cpvs_pyramid_build", self.ctx.handle, img.ctypes.data_as(C.c_void_p), 0, N, flags, stream,
+                 C.byref(h))
+        self.handle = h
+        self.N = N
+
+    def getNumLevels(self) -> int:
+        v = C.c_int()
+        call("cpvs_pyramid_num_levels", self.handle, C.byref(v))
+        return v.value
+
+    def _get(self, level, x, y):
+        mn, mx = C.c_float(), C.c_float()
+        call("cpvs_pyramid_get", self.handle, level, x, y, C.byref(mn), C.byref(mx))
+        return mn.value, mx.value
+
+    def getMin(self, level: int, x: int, y: int) -> float:
+        return self._get(level, x, y)[0]
+
+    def getMax(self, level: int, x: int, y: int) -> float:
+        return self._get(level, x, y)[1]
+
+    def getLevel(self, level: int) -> np.ndarray:
+        """Level 0 as (N, N); level L >= 1 as (w, w, 2) of (min, max)."""
+        w = self.N >> level
+        out = np.empty((w, w) if level == 0 else (w, w, 2), np.float32)
+        call("cpvs_pyramid_copy_level", self.handle, level, out.ctypes.data_as(C.c_void_p), out.size)
+        return out
+
+    def estimate(self, zTileIndex: int = 0, zTileNum: int = 1):
+        """(nodes per level before merging, scratch bytes, max DAG bytes) of a build."""
+        n = self.getNumLevels()
+        nodes = np.zeros(max(n, 1), np.uint64)
+        sb, db = C.c_uint64(), C.c_uint64()
+        call("cpvs_estimate_build", self.ctx.handle, self.handle, zTileIndex, zTileNum,
+             nodes.ctypes.data_as(C.c_void_p), C.byref(sb), C.byref(db))
+        return nodes, sb.value, db.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().cpvs_pyramid_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CompressedShadow:
+    """CompressedShadow (src/CompressedShadow.h:20-125); the DAG words live on the GPU."""
+
+    SHADOW, VISIBLE, PARTIAL = 0, 1, 2
+
+    def __init__(self, handle, ctx: Context):
+        self.handle = handle
+        self.ctx = ctx
+
+    @staticmethod
+    def create(minMax: MinMaxHierarchy, zTileIndex: int = 0, zTileNum: int = 1, stream=None) -> "CompressedShadow":
+        h = C.c_void_p()
+        call("cpvs_build", minMax.ctx.handle, minMax.handle, zTileIndex, zTileNum, stream, C.byref(h))
+        return CompressedShadow(h, minMax.ctx)
+
+    def _info(self):
+        nl, nw, tv = C.c_uint32(), C.c_uint64(), C.c_int()
+        call("cpvs_dag_info", self._h(), C.byref(nl), C.byref(nw), C.byref(tv))
+        return nl.value, nw.value, tv.value
+
+    def _h(self):
+        if not self.handle:
+            raise CpvsError("CompressedShadow", _lib.CPVS_ESTATE, "moved into a container")
+        return self.handle
+
+    def getNumLevels(self) -> int:
+        return self._info()[0]
+
+    def getTotalVisibility(self) -> int:
+        return self._info()[2]
+
+    def num_words(self) -> int:
+        return self._info()[1]
+
+    def getDAG(self) -> np.ndarray:
+        n = self.num_words()
+        out = np.empty(max(n, 1), np.uint32)
+        call("cpvs_dag_copy_words", self._h(), out.ctypes.data_as(C.c_void_p), out.size)
+        return out[:n]
+
+    def level_stats(self):
+        nodes = np.zeros(24, np.uint64)
+        uniq = np.zeros(24, np.uint64)
+        call("cpvs_dag_level_stats", self._h(), nodes.ctypes.data_as(C.c_void_p), uniq.ctypes.data_as(C.c_void_p), 24)
+        n = self.getNumLevels()
+        return nodes[:n], uniq[:n]
+
+    def traverse_many(self, ndc: np.ndarray, tryLeafmasks: bool = True) -> np.ndarray:
+        ndc = np.ascontiguousarray(np.asarray(ndc, np.float32).reshape(-1, 3))
+        out = np.zeros(ndc.shape[0], np.int32)
+        call("cpvs_dag_traverse", self._h(), ndc.ctypes.data_as(C.c_void_p), ndc.shape[0], int(tryLeafmasks),
+             out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def traverse(self, position, tryLeafmasks: bool = True) -> int:
+        """CompressedShadow::traverse(vec3 ndc, bool tryLeafmasks) (src/CompressedShadow.cpp:404-463)."""
+        return int(self.traverse_many(np.asarray(position, np.float32).reshape(1, 3), tryLeafmasks)[0])
+
+    def _release(self):
+        h, self.handle = self.handle, None
+        return h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().cpvs_dag_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CompressedShadowContainer:
+    """CompressedShadowContainer (src/CompressedShadowContainer.h:18-92)."""
+
+    def __init__(self, length_or_shadow, ctx: Optional[Context] = None):
+        if isinstance(length_or_shadow, CompressedShadow):
+            self.ctx = length_or_shadow.ctx
+            self._create(1)
+            self.set(length_or_shadow, 0, 0, 0)
+        else:
+            self.ctx = ctx or default_context()
+            self._create(int(length_or_shadow))
+
+    def _create(self, length):
+        h = C.c_void_p()
+        call("cpvs_container_create", self.ctx.handle, length, C.byref(h))
+        self.handle = h
+        self.length = length
+
+    def set(self, shadow: CompressedShadow, x: int, y: int, z: int):
+        call("cpvs_container_set", self.handle, shadow._h(), x, y, z)
+        shadow._release()  # ownership moves into the container, as with unique_ptr
+
+    def get(self, x: int, y: int, z: int) -> Optional[int]:
+        """The slot's DAG handle (an int), or None after freeOnCPU."""
+        h = C.c_void_p()
+        call("cpvs_container_get", self.handle, x, y, z, C.byref(h))
+        return h.value
+
+    def setFilterSize(self, size: int):
+        call("cpvs_container_set_filter_size", self.handle, size)
+
+    def copyToGPU(self):
+        call("cpvs_container_copy_to_gpu", self.handle)
+
+    def freeOnCPU(self):
+        call("cpvs_container_free_on_cpu", self.handle)
+
+    def moveToGPU(self):
+        call("cpvs_container_move_to_gpu", self.handle)
+
+    def info(self):
+        dl, gl, nw = C.c_int(), C.c_int(), C.c_uint64()
+        call("cpvs_container_info", self.handle, C.byref(dl), C.byref(gl), C.byref(nw))
+        return dl.value, gl.value, nw.value
+
+    def words_and_grid(self):
+        _, _, nw = self.info()
+        words = np.empty(max(nw, 1), np.uint32)
+        grid = np.empty(self.length ** 3, np.uint32)
+        call("cpvs_container_copy_words", self.handle, words.ctypes.data_as(C.c_void_p), words.size,
+             grid.ctypes.data_as(C.c_void_p), grid.size)
+        return words[:nw], grid
+
+    def evaluate(self, positions: DeviceArray, lightViewProj=None, visibilities: Optional[DeviceArray] = None,
+                 stream=None) -> DeviceArray:
+        """evaluate(positionsWS, lightViewProj, visibilities) (src/CompressedShadowContainer.cpp:93-124).
+
+        positions: DeviceArray (count, 3|4) float32; lightViewProj: 4x4 (numpy, applied as
+        lightViewProj @ (x, y, z, 1)) or None for identity; returns a (count,) uint8 DeviceArray.
+        """
+        count, stride = positions.shape
+        if visibilities is None:
+            visibilities = DeviceArray((count,), np.uint8, self.ctx)
+        m = None
+        if lightViewProj is not None:
+            m = np.ascontiguousarray(np.asarray(lightViewProj, np.float32).T.reshape(16))  # column-major
+        call("cpvs_container_evaluate", self.handle, positions.ptr, stride, count,
+             None if m is None else m.ctypes.data_as(C.c_void_p), visibilities.ptr, stream)
+        return visibilities
+
+    def evaluate_image(self, positions: DeviceArray, lightViewProj=None, visibilities: Optional[DeviceArray] = None,
+                       stream=None) -> DeviceArray:
+        """Same over an (H, W, 4) rgba32f position image; returns an (H, W) uint8 DeviceArray."""
+        H, W, _ = positions.shape
+        if visibilities is None:
+            visibilities = DeviceArray((H, W), np.uint8, self.ctx)
+        m = None
+        if lightViewProj is not None:
+            m = np.ascontiguousarray(np.asarray(lightViewProj, np.float32).T.reshape(16))
+        call("cpvs_container_evaluate_image", self.handle, positions.ptr, W, H,
+             None if m is None else m.ctypes.data_as(C.c_void_p), visibilities.ptr, stream)
+        return visibilities
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().cpvs_container_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gen_depth(N: int, seed: int, nboxes: int, noise: bool, x0: int = 0, y0: int = 0, tileN: Optional[int] = None,
+              ctx: Optional[Context] = None) -> DeviceArray:
+    """Synthetic depth tile [x0, x0+tileN)^2 of an N x N map, generated on the device."""
+    ctx = ctx or default_context()
+    tileN = N if tileN is None else tileN
+    out = DeviceArray((tileN, tileN), np.float32, ctx)
+    call("cpvs_gen_depth", ctx.handle, out.ptr, N, seed, nboxes, int(noise), x0, y0, tileN, None)
+    return out
+
+
+def gen_queries_uniform(seed: int, count: int, stride: int = 4, ctx: Optional[Context] = None) -> DeviceArray:
+    ctx = ctx or default_context()
+    out = DeviceArray((count, stride), np.float32, ctx)
+    call("cpvs_gen_queries_uniform", ctx.handle, out.ptr, seed, count, stride, None)
+    ctx.synchronize()
+    return out
+
+
+def gen_queries_gbuffer(depth: DeviceArray, seed: int, W: int = 3840, H: int = 2160,
+                        ctx: Optional[Context] = None) -> DeviceArray:
+    ctx = ctx or default_context()
+    out = DeviceArray((2 * W * H, 4), np.float32, ctx)
+    call("cpvs_gen_queries_gbuffer", ctx.handle, out.ptr, depth.ptr, depth.shape[0], seed, W, H, None)
+    ctx.synchronize()
+    return out

@@ -1,0 +1,106 @@
+"""ctypes loader for the in-tree HIP library ``cpvs_amd/libcpvs_amd.so`` (C ABI: include/cpvs.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcpvs_amd.so")
+
+CPVS_OK, CPVS_EINVAL, CPVS_ENOMEM, CPVS_EHIP, CPVS_EINTERNAL, CPVS_ESTATE, CPVS_EBUDGET = range(7)
+STATUS_NAMES = {0: "CPVS_OK", 1: "CPVS_EINVAL", 2: "CPVS_ENOMEM", 3: "CPVS_EHIP", 4: "CPVS_EINTERNAL",
+                5: "CPVS_ESTATE", 6: "CPVS_EBUDGET"}
+CPVS_PYR_ALL_LEVELS = 1
+CPVS_PYR_BORROW_DEPTH = 2
+
+
+class CpvsError(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} -> {STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+vp = C.c_void_p
+u32, u64, i32 = C.c_uint32, C.c_uint64, C.c_int
+PP = C.POINTER(C.c_void_p)
+
+# name -> argtypes (all return int status except cpvs_last_error)
+_SIGS = {
+    "cpvs_ctx_create": [i32, PP],
+    "cpvs_ctx_destroy": [vp],
+    "cpvs_ctx_synchronize": [vp],
+    "cpvs_ctx_stream": [vp, PP],
+    "cpvs_ctx_profile": [vp, i32],
+    "cpvs_ctx_profile_read": [vp, C.c_char_p, u64],
+    "cpvs_pyramid_build": [vp, vp, i32, u32, u32, vp, PP],
+    "cpvs_pyramid_num_levels": [vp, C.POINTER(i32)],
+    "cpvs_pyramid_get": [vp, u32, u32, u32, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+    "cpvs_pyramid_copy_level": [vp, u32, vp, u64],
+    "cpvs_pyramid_destroy": [vp],
+    "cpvs_estimate_build": [vp, vp, u32, u32, vp, C.POINTER(u64), C.POINTER(u64)],
+    "cpvs_build": [vp, vp, u32, u32, vp, PP],
+    "cpvs_dag_info": [vp, C.POINTER(u32), C.POINTER(u64), C.POINTER(i32)],
+    "cpvs_dag_level_stats": [vp, vp, vp, i32],
+    "cpvs_dag_copy_words": [vp, vp, u64],
+    "cpvs_dag_device_words": [vp, PP],
+    "cpvs_dag_traverse": [vp, vp, u64, i32, vp],
+    "cpvs_dag_destroy": [vp],
+    "cpvs_container_create": [vp, u32, PP],
+    "cpvs_container_set": [vp, vp, u32, u32, u32],
+    "cpvs_container_get": [vp, u32, u32, u32, PP],
+    "cpvs_container_set_filter_size": [vp, u32],
+    "cpvs_container_copy_to_gpu": [vp],
+    "cpvs_container_free_on_cpu": [vp],
+    "cpvs_container_move_to_gpu": [vp],
+    "cpvs_container_info": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(u64)],
+    "cpvs_container_copy_words": [vp, vp, u64, vp, u64],
+    "cpvs_container_evaluate": [vp, vp, u32, u64, vp, vp, vp],
+    "cpvs_container_evaluate_image": [vp, vp, u32, u32, vp, vp, vp],
+    "cpvs_container_destroy": [vp],
+    "cpvs_device_alloc": [vp, u64, PP],
+    "cpvs_device_free": [vp, vp],
+    "cpvs_memcpy_h2d": [vp, vp, vp, u64],
+    "cpvs_memcpy_d2h": [vp, vp, vp, u64],
+    "cpvs_event_record": [vp, vp, PP],
+    "cpvs_event_elapsed_ms": [vp, vp, C.POINTER(C.c_float)],
+    "cpvs_event_destroy": [vp],
+    "cpvs_gen_depth": [vp, vp, u32, u64, u32, i32, u32, u32, u32, vp],
+    "cpvs_gen_queries_uniform": [vp, vp, u64, u64, u32, vp],
+    "cpvs_gen_queries_gbuffer": [vp, vp, vp, u32, u64, u32, u32, vp],
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises OSError with a clear message when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                      f"(make -C cpvs_amd/csrc); cpvs_amd has no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    lib.cpvs_last_error.restype = C.c_char_p
+    lib.cpvs_last_error.argtypes = []
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = C.c_int
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_names():
+    return ["cpvs_last_error"] + list(_SIGS)
+
+
+def call(name: str, *args) -> None:
+    """Call an entry point and raise CpvsError on a non-zero status."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != CPVS_OK:
+        raise CpvsError(name, rc, lib.cpvs_last_error().decode(errors="replace"))

@@ -1,0 +1,70 @@
+// common.hpp -- shared device/host helpers for the cpvs_amd HIP kernels (gfx950, wave64).
+//
+// Bit layout of a node mask follows src/CompressedShadowUtil.h:80-132 of the reference:
+// two bits per child c = x | y<<1 | z<<2, 01 = VISIBLE, 00 = SHADOW, 10 = PARTIAL.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "prof.hpp"
+
+namespace cpvs {
+
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+constexpr int kWave = 64;
+
+// Node coordinates (x, y, z) at one octree level, packed 21 bits each.
+__host__ __device__ inline u64 pack_xyz(u32 x, u32 y, u32 z) {
+    return (u64)x | ((u64)y << 21) | ((u64)z << 42);
+}
+__host__ __device__ inline u32 unpack_x(u64 c) { return (u32)(c & 0x1FFFFF); }
+__host__ __device__ inline u32 unpack_y(u64 c) { return (u32)((c >> 21) & 0x1FFFFF); }
+__host__ __device__ inline u32 unpack_z(u64 c) { return (u32)((c >> 42) & 0x1FFFFF); }
+
+// getNumChildren (CompressedShadowUtil.h:80-85): number of partial children.
+__host__ __device__ inline u32 num_children(u32 mask) { return (u32)__builtin_popcount(mask & 0xAAAAu); }
+
+// std::min<float>/std::max<float> exactly as the reference's find() applies them
+// (MinMaxHierarchy.cpp:29-33): min(a,b) = b<a ? b : a, max(a,b) = a<b ? b : a.
+__host__ __device__ inline float smin(float a, float b) { return (b < a) ? b : a; }
+__host__ __device__ inline float smax(float a, float b) { return (a < b) ? b : a; }
+
+// visible() (CompressedShadowUtil.h:35-43) on a cell [z, z+1) against scaled min/max.
+__device__ inline u32 classify_cell(u32 z, float mn_h, float mx_h) {
+    if ((float)(z + 1) <= mn_h) return 1u;
+    if ((float)z >= mx_h) return 0u;
+    return 2u;
+}
+
+// absoluteVisible() (CompressedShadowUtil.h:51-57): midpoint compare.
+__device__ inline u32 classify_mid(u32 z, float d_h) {
+    const float mid = ((float)z + (float)(z + 1)) * 0.5f;
+    return (mid <= d_h) ? 1u : 0u;
+}
+
+// 64-bit mixers for subtree keys (Stafford variant 13 finaliser of splitmix64).
+__host__ __device__ inline u64 mix64(u64 z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline u64 hash_step(u64 h, u64 v) { return mix64(h ^ (v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2))); }
+
+template <class T>
+__host__ __device__ inline T ceil_div(T a, T b) { return (a + b - 1) / b; }
+
+}  // namespace cpvs
+
+#define CPVS_HIP_CHECK(expr)                                   \
+    do {                                                       \
+        hipError_t _e = (expr);                                \
+        if (_e != hipSuccess) return ::cpvs::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+namespace cpvs {
+// Records a HIP failure in the thread's last-error slot; returns CPVS_EHIP.
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+}  // namespace cpvs

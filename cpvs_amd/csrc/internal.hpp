@@ -1,0 +1,110 @@
+// internal.hpp -- host-side objects behind the opaque C-ABI handles and the kernel launchers
+// each translation unit provides.
+#pragma once
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cpvs.h"
+#include "common.hpp"
+
+namespace cpvs {
+
+constexpr int kMaxLevels = 24;  // N <= 2^21 texels per side
+
+void set_error(const char* fmt, ...);
+
+// Bump allocator over one device buffer, reset per build.  Grows (after a stream sync) when a
+// build needs more than it holds.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    int reserve(size_t bytes, hipStream_t s);  // ensure capacity, resets `used`
+    template <class T>
+    T* take(size_t count) {
+        const size_t bytes = (count * sizeof(T) + 255) & ~(size_t)255;
+        char* p = base + used;
+        used += bytes;
+        return reinterpret_cast<T*>(p);
+    }
+    void release();
+};
+
+inline size_t arena_bytes(size_t count, size_t elem) { return (count * elem + 255) & ~(size_t)255; }
+
+}  // namespace cpvs
+
+struct cpvs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cpvs::Arena arena;
+    cpvs::u64* pinned = nullptr;  // host readback staging (pinned)
+    size_t pinned_words = 0;
+    std::mutex mu;
+    bool prof_on = false;
+    cpvs::Prof prof;
+};
+
+struct cpvs_pyramid {
+    cpvs_ctx* ctx = nullptr;
+    uint32_t N = 0;
+    int n = 0;                 // numLevels including the root image
+    float* depth = nullptr;    // N*N floats (owned unless borrowed)
+    bool borrowed = false;
+    float2* levels = nullptr;  // one allocation holding every stored level
+    float2* lv[cpvs::kMaxLevels] = {};  // per-level pointer (null when not stored)
+};
+
+struct cpvs_dag {
+    cpvs_ctx* ctx = nullptr;
+    int n = 0;                    // numLevels
+    uint64_t words = 0;           // word count
+    uint32_t* d_words = nullptr;  // device words (owned)
+    uint32_t root_mask = 0;
+    uint32_t N = 0, zTileIndex = 0, zTileNum = 1;
+    uint64_t nodes[cpvs::kMaxLevels] = {};    // nodes per level before merging
+    uint64_t uniques[cpvs::kMaxLevels] = {};  // nodes per level after merging
+};
+
+struct cpvs_container {
+    cpvs_ctx* ctx = nullptr;
+    uint32_t length = 1;
+    std::vector<cpvs_dag*> dags;  // length^3 slots, index z*L^2 + y*L + x
+    uint32_t filter_size = 1;
+    uint32_t* d_dag = nullptr;   // combined words
+    uint64_t dag_words = 0;
+    uint32_t* d_grid = nullptr;  // length^3 entries
+    int dag_levels = 0, grid_levels = 0;
+    bool on_gpu = false;
+    bool freed_cpu = false;
+};
+
+namespace cpvs {
+
+// pyramid.hip
+int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s);
+// Exact node counts per level (counts[L] = nodes at octree level L) for z-tile t of T.
+int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, hipStream_t s);
+
+// build.hip
+int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, cpvs_dag** out);
+
+// lookup.hip
+struct Mat4 {
+    float m[16];
+};
+int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 stride, u64 count,
+                    const Mat4& m, uint8_t* vis, hipStream_t s);
+int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 width,
+                          u32 height, const Mat4& m, uint8_t* vis, hipStream_t s);
+int launch_traverse_single(const u32* dag, u32 words, int n, const float* ndc, u64 count, int try_leaf, int* out,
+                           hipStream_t s);
+
+// gen.hip
+int launch_gen_depth(float* out, u32 fullN, u64 seed, u32 nboxes, int noise, u32 x0, u32 y0, u32 tileN, hipStream_t s);
+int launch_gen_queries_uniform(float* out, u64 seed, u64 count, u32 stride, hipStream_t s);
+int launch_gen_queries_gbuffer(float* out, const float* depth, u32 N, u64 seed, u32 W, u32 H, hipStream_t s);
+
+}  // namespace cpvs

@@ -1,0 +1,197 @@
+// lookup.hip -- shadow lookups: shader/traverse.cs (via CompressedShadowContainer::evaluate,
+// src/CompressedShadowContainer.cpp:93-124) and CompressedShadow::traverse
+// (src/CompressedShadow.cpp:404-463).
+//
+// One thread per query.  The walk is a chain of dependent 4-byte loads, one per level; the
+// upper levels are shared by every query and stay in L2, so throughput is set by how many
+// walks are in flight and by the locality of neighbouring queries.  The image entry point
+// gives each wave an 8x8 pixel tile so neighbouring lanes follow the same DAG paths.
+// Compiled with -ffp-contract=off: the matrix product, the divide and the quantisation round
+// exactly as written, as the reference's float expressions do.
+#include "internal.hpp"
+
+namespace cpvs {
+
+struct GridDag {
+    const u32* __restrict__ dag;
+    const u32* __restrict__ grid;
+    int dl, gl;
+    u32 words;  // DAG word count: every read is bounds-checked against it
+};
+
+// traverse.cs:43-48: path = int(((ndc + 1) * 0.5) * float(R - 1)), clamped to [0, R-1]
+// (out-of-range positions index outside the grid in the reference).
+__device__ inline u32 quantise(float v, u32 R) {
+    v = v + 1.0f;
+    v = v * 0.5f;
+    v = v * (float)(R - 1);
+    if (!(v > 0.0f)) return 0u;
+    if (v >= (float)(R - 1)) return R - 1;
+    return (u32)v;
+}
+
+// traverse.cs:75-133 (sentinels as tested at :85-88).
+__device__ inline u32 walk(const GridDag& g, u32 px, u32 py, u32 pz) {
+    const u32 G = 1u << g.gl;
+    const u32 sh = (u32)(g.dl - 1);
+    const u32 base = g.grid[((pz >> sh) * G + (py >> sh)) * G + (px >> sh)];
+    if (base == 0xFFFFFFFFu) return 0u;
+    if (base == 0xFFFFFFFEu) return 1u;
+    u32 off = base;
+    for (int level = g.dl - 2; level >= 3; --level) {
+        const u32 ci = (((px >> level) & 1u) << 1) | (((py >> level) & 1u) << 2) | (((pz >> level) & 1u) << 3);
+        if (off >= g.words) return 0u;
+        const u32 m = g.dag[off];
+        const u32 v = (m >> ci) & 3u;
+        if (v != 2u) return v;  // 0 shadow, 1 visible (3 never occurs)
+        const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
+        if (off + 1 + co >= g.words) return 0u;
+        off = base + g.dag[off + 1 + co];
+    }
+    if (off >= g.words) return 0u;
+    const u32 ci = (pz & 7u) * 2u;
+    const u32 m = g.dag[off];
+    const u32 v = (m >> ci) & 3u;
+    if (v != 2u) return v;
+    const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
+    const u32 bit = (px & 7u) + 8u * (py & 7u);
+    const u32 idx = off + co * 2 + 1 + (bit >> 5);
+    if (idx >= g.words) return 0u;
+    const u32 w = g.dag[idx];
+    return (w >> (bit & 31u)) & 1u;
+}
+
+// traverse.cs:141-144: p = lightViewProj * (pos, 1); p /= p.w; then the path.
+__device__ inline u32 lookup_point(const GridDag& g, const Mat4& M, float x, float y, float z) {
+    float r[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float acc = M.m[0 * 4 + c] * x;
+        acc = acc + M.m[1 * 4 + c] * y;
+        acc = acc + M.m[2 * 4 + c] * z;
+        acc = acc + M.m[3 * 4 + c] * 1.0f;
+        r[c] = acc;
+    }
+    const u32 R = 1u << (g.dl + g.gl - 1);
+    const u32 px = quantise(r[0] / r[3], R);
+    const u32 py = quantise(r[1] / r[3], R);
+    const u32 pz = quantise(r[2] / r[3], R);
+    return walk(g, px, py, pz);
+}
+
+__global__ __launch_bounds__(256) void k_evaluate4(GridDag g, Mat4 M, const float4* __restrict__ pos, u64 count,
+                                                   uint8_t* __restrict__ vis) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float4 q = pos[i];
+    vis[i] = (uint8_t)lookup_point(g, M, q.x, q.y, q.z);
+}
+
+__global__ __launch_bounds__(256) void k_evaluate3(GridDag g, Mat4 M, const float* __restrict__ pos, u64 count,
+                                                   uint8_t* __restrict__ vis) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float* q = pos + 3 * i;
+    vis[i] = (uint8_t)lookup_point(g, M, q[0], q[1], q[2]);
+}
+
+// Image form: a 256-thread block covers a 32x8 pixel strip as four 8x8 tiles, one per wave.
+__global__ __launch_bounds__(256) void k_evaluate_image(GridDag g, Mat4 M, const float4* __restrict__ pos, u32 width,
+                                                        u32 height, uint8_t* __restrict__ vis) {
+    const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const u32 x = blockIdx.x * 32 + wave * 8 + (lane & 7);
+    const u32 y = blockIdx.y * 8 + (lane >> 3);
+    if (x >= width || y >= height) return;
+    const size_t i = (size_t)y * width + x;
+    const float4 q = pos[i];
+    vis[i] = (uint8_t)lookup_point(g, M, q.x, q.y, q.z);
+}
+
+int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 stride, u64 count,
+                    const Mat4& m, uint8_t* vis, hipStream_t s) {
+    if (count == 0) return CPVS_OK;
+    GridDag g{dag, grid, dl, gl, words};
+    const u64 blocks = (count + 255) / 256;
+    if (blocks > 0x7FFFFFFFull) {
+        set_error("evaluate: %llu points exceed one launch", (unsigned long long)count);
+        return CPVS_EINVAL;
+    }
+    if (stride == 4)
+        CPVS_LAUNCH("evaluate4", k_evaluate4, dim3((u32)blocks), dim3(256), 0, s, g, m, reinterpret_cast<const float4*>(pos),
+                           count, vis);
+    else
+        CPVS_LAUNCH("evaluate3", k_evaluate3, dim3((u32)blocks), dim3(256), 0, s, g, m, pos, count, vis);
+    CPVS_HIP_CHECK(hipGetLastError());
+    return CPVS_OK;
+}
+
+int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 width,
+                          u32 height, const Mat4& m, uint8_t* vis, hipStream_t s) {
+    if (width == 0 || height == 0) return CPVS_OK;
+    GridDag g{dag, grid, dl, gl, words};
+    const dim3 gridDim(ceil_div<u32>(width, 32), ceil_div<u32>(height, 8));
+    CPVS_LAUNCH("evaluate_image", k_evaluate_image, gridDim, dim3(256), 0, s, g, m, reinterpret_cast<const float4*>(pos), width,
+                       height, vis);
+    CPVS_HIP_CHECK(hipGetLastError());
+    return CPVS_OK;
+}
+
+// CompressedShadow::traverse (src/CompressedShadow.cpp:404-463) with getPathFromNDC
+// (src/CompressedShadowUtil.h:70-75): no grid, pointers absolute, PARTIAL when the walk
+// passes level 0 without leafmasks.
+__global__ __launch_bounds__(256) void k_traverse_single(const u32* __restrict__ w, u32 words, int n,
+                                                         const float* __restrict__ ndc, u64 count, int try_leaf,
+                                                         int* __restrict__ out) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int res = (1 << (n - 1)) - 1;
+    int p[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float v = ndc[3 * i + c] + 1.0f;
+        v = v * 0.5f;
+        p[c] = (int)(v * (float)res);
+    }
+    u32 off = 0;
+    int level = n - 2;
+    const int minL = try_leaf ? 3 : 0;
+    int result = 2;
+    bool done = false;
+    while (level >= minL) {
+        const int b = 1 << level;
+        const u32 c = ((p[0] & b) ? 1u : 0u) + ((p[1] & b) ? 2u : 0u) + ((p[2] & b) ? 4u : 0u);
+        if (off >= words) { result = 0; done = true; break; }
+        const u32 m = w[off];
+        if (m & (1u << (2 * c))) { result = 1; done = true; break; }
+        if (!(m & (2u << (2 * c)))) { result = 0; done = true; break; }
+        const u32 slot = off + 1 + (u32)__builtin_popcount(m & 0xAAAAu & ((1u << (2 * c)) - 1u));
+        if (slot >= words) { result = 0; done = true; break; }
+        off = w[slot];
+        --level;
+    }
+    if (!done && try_leaf && off < words) {
+        const u32 c = (u32)(p[2] & 7);
+        const u32 m = w[off];
+        if (m & (1u << (2 * c))) result = 1;
+        else if (!(m & (2u << (2 * c)))) result = 0;
+        else {
+            const u32 idx = off + (u32)__builtin_popcount(m & 0xAAAAu & ((1u << (2 * c)) - 1u)) * 2 + 1;
+            const int bit = (p[0] & 7) + 8 * (p[1] & 7);
+            const u32 widx = idx + (bit < 32 ? 0u : 1u);
+            const u32 v = (widx >= words) ? 0u : (bit < 32) ? (w[widx] & (1u << bit)) : (w[widx] & (1u << (bit - 32)));
+            result = v ? 1 : 0;
+        }
+    }
+    out[i] = result;
+}
+
+int launch_traverse_single(const u32* dag, u32 words, int n, const float* ndc, u64 count, int try_leaf, int* out,
+                           hipStream_t s) {
+    if (count == 0) return CPVS_OK;
+    CPVS_LAUNCH("traverse_single", k_traverse_single, dim3((u32)((count + 255) / 256)), dim3(256), 0, s, dag, words, n, ndc, count,
+                       try_leaf, out);
+    CPVS_HIP_CHECK(hipGetLastError());
+    return CPVS_OK;
+}
+
+}  // namespace cpvs

@@ -1,0 +1,174 @@
+// scan.hpp -- device-wide scans over u32 with fused input/output functors.
+//
+// Reduce-then-scan in three launches: per-tile reduction, a single-block scan of the tile
+// partials, and a per-tile scan that hands (index, exclusive prefix, value) to an output
+// functor.  Input is read in striped order (coalesced); every launch is a plain grid with
+// no inter-block waiting.
+#pragma once
+
+#include "common.hpp"
+
+namespace cpvs {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanRounds = 8;
+constexpr u32 kScanTile = kScanThreads * kScanRounds;  // 2048 elements per block
+
+struct OpSum {
+    static constexpr u32 identity = 0;
+    __device__ u32 operator()(u32 a, u32 b) const { return a + b; }
+};
+struct OpMax {
+    static constexpr u32 identity = 0;
+    __device__ u32 operator()(u32 a, u32 b) const { return a > b ? a : b; }
+};
+
+template <class Op>
+__device__ inline u32 wave_inclusive_scan(u32 v, Op op) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const u32 o = __shfl_up(v, d, kWave);
+        if (lane >= d) v = op(v, o);
+    }
+    return v;
+}
+
+template <class Op>
+__device__ inline u32 wave_reduce(u32 v, Op op) {
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) v = op(v, __shfl_xor(v, d, kWave));
+    return v;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block; *total gets the block sum.
+// s_wave must hold kScanThreads/64 entries of shared memory.
+template <class Op>
+__device__ inline u32 block_exclusive_scan(u32 v, u32* s_wave, u32* total, Op op) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const u32 inc = wave_inclusive_scan(v, op);
+    if (lane == kWave - 1) s_wave[wid] = inc;
+    __syncthreads();
+    u32 wpre = Op::identity, tot = Op::identity;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / kWave; ++w) {
+        const u32 x = s_wave[w];
+        if (w < wid) wpre = op(wpre, x);
+        tot = op(tot, x);
+    }
+    __syncthreads();
+    u32 ex = __shfl_up(inc, 1, kWave);
+    if (lane == 0) ex = Op::identity;
+    *total = tot;
+    return op(wpre, ex);
+}
+
+template <class Op>
+__device__ inline u32 block_reduce(u32 v, u32* s_wave, Op op) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    v = wave_reduce(v, op);
+    if (lane == 0) s_wave[wid] = v;
+    __syncthreads();
+    u32 tot = Op::identity;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / kWave; ++w) tot = op(tot, s_wave[w]);
+    __syncthreads();
+    return tot;
+}
+
+template <class In, class Op>
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(In in, u32 n, u32* partials, Op op) {
+    __shared__ u32 s_wave[kScanThreads / kWave];
+    const u32 base = blockIdx.x * kScanTile;
+    u32 acc = Op::identity;
+#pragma unroll
+    for (int r = 0; r < kScanRounds; ++r) {
+        const u32 i = base + r * kScanThreads + threadIdx.x;
+        if (i < n) acc = op(acc, in(i));
+    }
+    const u32 tot = block_reduce(acc, s_wave, op);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the tile partials in place (one block); *total = overall reduction.
+template <class Op>
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(u32* partials, u32 nparts, u32* total, Op op) {
+    __shared__ u32 s_wave[kScanThreads / kWave];
+    u32 carry = Op::identity;
+    for (u32 base = 0; base < nparts; base += kScanThreads) {
+        const u32 i = base + threadIdx.x;
+        const u32 v = (i < nparts) ? partials[i] : Op::identity;
+        u32 tot;
+        const u32 ex = block_exclusive_scan(v, s_wave, &tot, op);
+        if (i < nparts) partials[i] = op(carry, ex);
+        carry = op(carry, tot);
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <class In, class Out, class Op>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(In in, Out out, u32 n, const u32* partials, Op op) {
+    __shared__ u32 s_wave[kScanThreads / kWave];
+    const u32 base = blockIdx.x * kScanTile;
+    u32 carry = partials[blockIdx.x];
+    for (int r = 0; r < kScanRounds; ++r) {
+        const u32 r0 = base + r * kScanThreads;
+        if (r0 >= n) break;  // uniform across the block
+        const u32 i = r0 + threadIdx.x;
+        const u32 v = (i < n) ? in(i) : Op::identity;
+        u32 tot;
+        const u32 ex = block_exclusive_scan(v, s_wave, &tot, op);
+        if (i < n) out(i, op(carry, ex), v);
+        carry = op(carry, tot);
+    }
+}
+
+// Plain-array functors for device_scan.
+struct ArrIn {
+    const u32* a;
+    __device__ u32 operator()(u32 i) const { return a[i]; }
+};
+struct ArrOutExclusive {
+    u32* a;
+    __device__ void operator()(u32 i, u32 ex, u32) const { a[i] = ex; }
+};
+
+// Scratch words needed by device_scan for n elements.
+inline size_t scan_scratch_words(u32 n) { return (size_t)ceil_div<u32>(n ? n : 1, kScanTile) + 1; }
+
+// out(i, exclusive_prefix_i, value_i) for all i < n; *d_total (device, may be null) gets the
+// reduction of all values.  scratch: scan_scratch_words(n) u32 of device memory.
+template <class In, class Out, class Op>
+inline void device_scan(In in, Out out, u32 n, Op op, u32* d_total, u32* scratch, hipStream_t s) {
+    const u32 nb = ceil_div<u32>(n ? n : 1, kScanTile);
+    if (n > 0) CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, n, scratch, op);
+    else CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op>), dim3(1), dim3(kScanThreads), 0, s, in, 0u, scratch, op);
+    CPVS_LAUNCH("scan_partials", k_scan_partials<Op>, dim3(1), dim3(kScanThreads), 0, s, scratch, nb, d_total, op);
+    if (n > 0) CPVS_LAUNCH("scan_down", (k_scan_down<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, scratch, op);
+}
+
+// ------------------------------------------------------------------------------------------
+// LSD radix sort of (u32 key, u32 value) pairs, 8 bits per pass, stable.
+// ------------------------------------------------------------------------------------------
+
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 16;
+constexpr u32 kSortTile = kSortThreads * kSortRounds;  // 4096 pairs per block
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const u32* __restrict__ keys, u32 n, int shift,
+                                                            u32* __restrict__ hist, u32 nblocks);
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const u32* __restrict__ kin, const u32* __restrict__ vin,
+                                                               u32* __restrict__ kout, u32* __restrict__ vout, u32 n,
+                                                               int shift, const u32* __restrict__ offsets, u32 nblocks);
+
+// Scratch words for radix_sort_pairs with n pairs.
+inline size_t sort_scratch_words(u32 n) {
+    const u32 nb = ceil_div<u32>(n ? n : 1, kSortTile);
+    return (size_t)256 * nb + scan_scratch_words(256 * nb);
+}
+
+// Sorts keys[0..n) with values by the low `bits` bits of the key (a multiple of 8).
+// keys/vals hold the result; ktmp/vtmp are ping-pong buffers of n entries.
+void radix_sort_pairs(u32* keys, u32* vals, u32* ktmp, u32* vtmp, u32 n, int bits, u32* scratch, hipStream_t s);
+
+}  // namespace cpvs

@@ -1,0 +1,164 @@
+/*
+ * cpvs.h -- C ABI of cpvs_amd, an MI355X (gfx950) implementation of the precomputed voxelized
+ * shadow build and lookup path of TobiasRp/cpvs.
+ *
+ * Every entry point returns a status code (CPVS_OK on success); no exception crosses the ABI.
+ * cpvs_last_error() gives the calling thread's message for the last failure.  Handles are
+ * opaque; one handle may be used from one thread at a time.  Streams are hipStream_t values
+ * passed as void*; NULL selects the context's own stream.  Device pointers are HIP device
+ * memory of the context's device.
+ *
+ * Each entry point names the reference interface it replaces (file:line in TobiasRp/cpvs).
+ */
+#ifndef CPVS_AMD_CPVS_H
+#define CPVS_AMD_CPVS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPVS_OK 0
+#define CPVS_EINVAL 1    /* bad argument (size not a power of two, index out of range, ...) */
+#define CPVS_ENOMEM 2    /* device or host allocation failed */
+#define CPVS_EHIP 3      /* a HIP runtime call failed */
+#define CPVS_EINTERNAL 4 /* an internal consistency check failed */
+#define CPVS_ESTATE 5    /* call not valid in the object's current state */
+#define CPVS_EBUDGET 6   /* admission refused: estimate exceeds the memory budget */
+
+/* CompressedShadow::NodeVisibility (src/CompressedShadow.h:22-26). */
+#define CPVS_SHADOW 0
+#define CPVS_VISIBLE 1
+#define CPVS_PARTIAL 2
+
+/* cpvs_pyramid_build flags. */
+#define CPVS_PYR_ALL_LEVELS 1u   /* also store levels 1 and 2, which the build never reads */
+#define CPVS_PYR_BORROW_DEPTH 2u /* depth is device memory the caller keeps alive; not copied */
+
+typedef struct cpvs_ctx cpvs_ctx;
+typedef struct cpvs_pyramid cpvs_pyramid;
+typedef struct cpvs_dag cpvs_dag;
+typedef struct cpvs_container cpvs_container;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *cpvs_last_error(void);
+
+/* Context: one HIP device, its own non-blocking stream and a scratch arena for builds.
+ * (The reference has no counterpart: it runs on the host and the GL context.) */
+int cpvs_ctx_create(int device, cpvs_ctx **out);
+int cpvs_ctx_destroy(cpvs_ctx *ctx);
+int cpvs_ctx_synchronize(cpvs_ctx *ctx);
+int cpvs_ctx_stream(cpvs_ctx *ctx, void **stream);
+/* Per-kernel timing: mode 1 brackets every kernel launched by this context's entry points with
+ * HIP events on its stream, 0 stops, -1 clears the totals.  profile_read writes one line per
+ * kernel name: "name<TAB>total_ms<TAB>launches".  (No reference counterpart: the reference
+ * times the whole precompute with std::chrono, src/main.cpp:228-234.) */
+int cpvs_ctx_profile(cpvs_ctx *ctx, int mode);
+int cpvs_ctx_profile_read(cpvs_ctx *ctx, char *buf, uint64_t cap);
+
+/* MinMaxHierarchy::MinMaxHierarchy(const ImageF&) (src/MinMaxHierarchy.h:23,
+ * src/MinMaxHierarchy.cpp:9-97).  depth: N x N row-major float32, host memory or (with
+ * depth_on_device) device memory.  N must be a power of two, 2 <= N <= 65536 (cpvs_build needs
+ * N >= 8, src/CompressedShadow.cpp:44-47). */
+int cpvs_pyramid_build(cpvs_ctx *ctx, const float *depth, int depth_on_device, uint32_t N, uint32_t flags,
+                       void *stream, cpvs_pyramid **out);
+/* MinMaxHierarchy::getNumLevels (src/MinMaxHierarchy.h:60-62). */
+int cpvs_pyramid_num_levels(const cpvs_pyramid *p, int *out);
+/* MinMaxHierarchy::getMin / getMax (src/MinMaxHierarchy.h:31-55); level 0 returns the depth
+ * for both.  Synchronous device read. */
+int cpvs_pyramid_get(const cpvs_pyramid *p, uint32_t level, uint32_t x, uint32_t y, float *min, float *max);
+/* MinMaxHierarchy::getLevel (src/MinMaxHierarchy.h:67-72): level 0 as N*N floats, level L >= 1
+ * as (N>>L)^2 interleaved (min, max) pairs.  Fails with CPVS_ESTATE for levels 1-2 unless the
+ * pyramid was built with CPVS_PYR_ALL_LEVELS (or N < 16). */
+int cpvs_pyramid_copy_level(const cpvs_pyramid *p, uint32_t level, float *host, uint64_t cap_floats);
+int cpvs_pyramid_destroy(cpvs_pyramid *p);
+
+/* Exact octree node counts per level before merging for z-tile zTileIndex of zTileNum
+ * (nodes_per_level[L] for L < numLevels, may be NULL), the scratch bytes cpvs_build will use,
+ * and an upper bound of the DAG's bytes.  Used for admission control; no reference
+ * counterpart (the reference sizes vectors as it goes, src/CompressedShadow.cpp:87-169). */
+int cpvs_estimate_build(cpvs_ctx *ctx, const cpvs_pyramid *p, uint32_t zTileIndex, uint32_t zTileNum,
+                        uint64_t *nodes_per_level, uint64_t *scratch_bytes, uint64_t *max_dag_bytes);
+
+/* CompressedShadow::create(const MinMaxHierarchy&, uint zTileIndex, uint zTileNum)
+ * (src/CompressedShadow.h:48-49, src/CompressedShadow.cpp:49-59).  zTileNum is explicit
+ * instead of the reference's process-global depthOffset (src/CompressedShadowUtil.cpp:10-14).
+ * The DAG's words stay in device memory.  Returns after the build completed on the stream. */
+int cpvs_build(cpvs_ctx *ctx, const cpvs_pyramid *p, uint32_t zTileIndex, uint32_t zTileNum, void *stream,
+               cpvs_dag **out);
+/* CompressedShadow::getNumLevels / getDAG().size() / getTotalVisibility
+ * (src/CompressedShadow.h:69-77, src/CompressedShadow.cpp:66-72). */
+int cpvs_dag_info(const cpvs_dag *d, uint32_t *num_levels, uint64_t *num_words, int *total_visibility);
+/* Node counts per level before and after merging (cap entries each, may be NULL). */
+int cpvs_dag_level_stats(const cpvs_dag *d, uint64_t *nodes, uint64_t *uniques, int cap);
+/* CompressedShadow::getDAG (src/CompressedShadow.h:75-77): copy the words to host memory. */
+int cpvs_dag_copy_words(const cpvs_dag *d, uint32_t *host, uint64_t cap_words);
+int cpvs_dag_device_words(const cpvs_dag *d, const uint32_t **device_words);
+/* CompressedShadow::traverse(vec3 ndc, bool tryLeafmasks) (src/CompressedShadow.cpp:404-463)
+ * for count NDC points (3 floats each, host memory), evaluated on the device; vis[i] gets
+ * CPVS_SHADOW, CPVS_VISIBLE or CPVS_PARTIAL. */
+int cpvs_dag_traverse(cpvs_dag *d, const float *ndc, uint64_t count, int try_leafmasks, int *vis);
+int cpvs_dag_destroy(cpvs_dag *d);
+
+/* CompressedShadowContainer(uint length) (src/CompressedShadowContainer.h:21-25): length^3
+ * DAG slots.  The single-shadow constructor (:28-32) is create(length=1) + set(dag,0,0,0). */
+int cpvs_container_create(cpvs_ctx *ctx, uint32_t length, cpvs_container **out);
+/* CompressedShadowContainer::set (src/CompressedShadowContainer.h:34-39); takes ownership of
+ * dag (a DAG already in the slot is destroyed). */
+int cpvs_container_set(cpvs_container *c, cpvs_dag *dag, uint32_t x, uint32_t y, uint32_t z);
+/* CompressedShadowContainer::get (src/CompressedShadowContainer.h:41-46); *out is NULL after
+ * free_on_cpu, as in the reference. */
+int cpvs_container_get(const cpvs_container *c, uint32_t x, uint32_t y, uint32_t z, const cpvs_dag **out);
+/* setFilterSize (src/CompressedShadowContainer.h:71-73); stored, unused by the lookup, as in
+ * the reference (shader/traverse.cs:16-17). */
+int cpvs_container_set_filter_size(cpvs_container *c, uint32_t size);
+/* copyToGPU (src/CompressedShadowContainer.cpp:31-46): combineDAGs (:52-69) and
+ * createTopLevelGrid (:71-91) on the device.  Grid sentinels are 0xFFFFFFFF (shadow) and
+ * 0xFFFFFFFE (visible), the values shader/traverse.cs:85-88 tests. */
+int cpvs_container_copy_to_gpu(cpvs_container *c);
+/* freeOnCPU / moveToGPU (src/CompressedShadowContainer.h:55-68). */
+int cpvs_container_free_on_cpu(cpvs_container *c);
+int cpvs_container_move_to_gpu(cpvs_container *c);
+int cpvs_container_info(const cpvs_container *c, int *dag_levels, int *grid_levels, uint64_t *num_words);
+int cpvs_container_copy_words(const cpvs_container *c, uint32_t *host_words, uint64_t cap_words,
+                              uint32_t *host_grid, uint64_t cap_grid);
+/* CompressedShadowContainer::evaluate (src/CompressedShadowContainer.cpp:93-124) running the
+ * lookup of shader/traverse.cs:135-149: positions (device, stride_floats = 3 or 4 floats per
+ * point) are multiplied by light_view_proj (16 floats, column-major, host memory; NULL =
+ * identity), divided by w, quantised and looked up; vis (device) gets 0 or 1 per point. */
+int cpvs_container_evaluate(cpvs_container *c, const float *positions, uint32_t stride_floats, uint64_t count,
+                            const float *light_view_proj, uint8_t *vis, void *stream);
+/* The same over a width x height rgba32f position image (row-major, 4 floats per texel), as the
+ * reference's Texture2D arguments; vis is a width x height byte image. */
+int cpvs_container_evaluate_image(cpvs_container *c, const float *positions_rgba, uint32_t width, uint32_t height,
+                                  const float *light_view_proj, uint8_t *vis, void *stream);
+int cpvs_container_destroy(cpvs_container *c);
+
+/* Device memory and timing helpers for callers without their own HIP allocator. */
+int cpvs_device_alloc(cpvs_ctx *ctx, uint64_t bytes, void **dptr);
+int cpvs_device_free(cpvs_ctx *ctx, void *dptr);
+int cpvs_memcpy_h2d(cpvs_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int cpvs_memcpy_d2h(cpvs_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int cpvs_event_record(cpvs_ctx *ctx, void *stream, void **event);
+int cpvs_event_elapsed_ms(void *start, void *stop, float *ms); /* synchronises on stop */
+int cpvs_event_destroy(void *event);
+
+/* Synthetic inputs (the reference renders scenes instead).
+ * Depth tile [x0,x0+tileN)^2 of a fullN x fullN map: plane 0.6+0.1u, optional 4-octave value
+ * noise of amplitude 0.05, nboxes boxes of side 16..128 and depth U[0.1,0.5). */
+int cpvs_gen_depth(cpvs_ctx *ctx, float *d_out, uint32_t fullN, uint64_t seed, uint32_t nboxes, int noise,
+                   uint32_t x0, uint32_t y0, uint32_t tileN, void *stream);
+/* count uniform light-space points as NDC (stride 3 or 4; the 4th component is 1). */
+int cpvs_gen_queries_uniform(cpvs_ctx *ctx, float *d_out, uint64_t seed, uint64_t count, uint32_t stride,
+                             void *stream);
+/* 2*W*H G-buffer points (float4 NDC) over the N x N depth map d_depth: per pixel one point on
+ * the surface and one jittered by U(-2,2) voxels in z. */
+int cpvs_gen_queries_gbuffer(cpvs_ctx *ctx, float *d_out, const float *d_depth, uint32_t N, uint64_t seed,
+                             uint32_t W, uint32_t H, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
