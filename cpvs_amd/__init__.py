@@ -57,6 +57,10 @@ class Context:
             out[name] = (float(ms), int(cnt))
         return out
 
+    def set_merge(self, mode: str):
+        """'table' (default) or 'sort': how builds merge common subtrees (same words)."""
+        call("cpvs_ctx_set_merge", self.handle, {"table": _lib.CPVS_MERGE_TABLE, "sort": _lib.CPVS_MERGE_SORT}[mode])
+
     def event(self, stream=None):
         e = C.c_void_p()
         call("cpvs_event_record", self.handle, stream, C.byref(e))
@@ -242,6 +246,12 @@ class CompressedShadow:
         call("cpvs_dag_level_stats", self._h(), nodes.ctypes.data_as(C.c_void_p), uniq.ctypes.data_as(C.c_void_p), 24)
         n = self.getNumLevels()
         return nodes[:n], uniq[:n]
+
+    def merge_info(self):
+        """('table' | 'sort', 32-bit sort-key collisions resolved by scanning)."""
+        mode, coll = C.c_int(), C.c_uint64()
+        call("cpvs_dag_merge_info", self._h(), C.byref(mode), C.byref(coll))
+        return ("table" if mode.value == _lib.CPVS_MERGE_TABLE else "sort"), coll.value
 
     def traverse_many(self, ndc: np.ndarray, tryLeafmasks: bool = True) -> np.ndarray:
         ndc = np.ascontiguousarray(np.asarray(ndc, np.float32).reshape(-1, 3))

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libcpvs_amd.so")
 CPVS_OK, CPVS_EINVAL, CPVS_ENOMEM, CPVS_EHIP, CPVS_EINTERNAL, CPVS_ESTATE, CPVS_EBUDGET = range(7)
 STATUS_NAMES = {0: "CPVS_OK", 1: "CPVS_EINVAL", 2: "CPVS_ENOMEM", 3: "CPVS_EHIP", 4: "CPVS_EINTERNAL",
                 5: "CPVS_ESTATE", 6: "CPVS_EBUDGET"}
+CPVS_MERGE_TABLE, CPVS_MERGE_SORT = 0, 1
 CPVS_PYR_ALL_LEVELS = 1
 CPVS_PYR_BORROW_DEPTH = 2
 
@@ -35,6 +36,8 @@ _SIGS = {
     "cpvs_ctx_stream": [vp, PP],
     "cpvs_ctx_profile": [vp, i32],
     "cpvs_ctx_profile_read": [vp, C.c_char_p, u64],
+    "cpvs_ctx_set_merge": [vp, i32],
+    "cpvs_dag_merge_info": [vp, C.POINTER(i32), C.POINTER(u64)],
     "cpvs_pyramid_build": [vp, vp, i32, u32, u32, vp, PP],
     "cpvs_pyramid_num_levels": [vp, C.POINTER(i32)],
     "cpvs_pyramid_get": [vp, u32, u32, u32, C.POINTER(C.c_float), C.POINTER(C.c_float)],

@@ -8,17 +8,23 @@
 //      texels and writes the 16-bit child mask; an exclusive scan of the partial-child counts
 //      gives each node's first child, and the scan's output stage writes the children's
 //      coordinates in (parent, child index) order -- the reference's breadth-first order.
-//   2. Leaves (level 2): one wave64 per node, lane = x + 8y of the 8x8 texel block, eight
-//      __ballot()s give the eight 64-bit leafmasks (src/CompressedShadowUtil.cpp:59-99).
-//   3. Merge, bottom-up from the leaves to the root's children: a 64-bit hash of each node's
-//      key (mask + partial leafmasks, or mask + merged ranks of its children), a stable LSD
-//      radix sort of (hash, node index), segment starts by a max-scan, a full-key compare
-//      against the segment head (a mismatch -- a hash collision -- falls back to a scan of
-//      the segment), and a scan of first-occurrence flags: rank = order of first occurrence,
-//      as mergeLevel's linear search yields (src/CompressedShadowUtil.h:154-182).
+//   2. Leaves (level 2), column by column: one wave per 8x8 texel column (a level-3 texel),
+//      lane = texel.  The leaf nodes of a column are its partial level-3 cells, contiguous in z;
+//      a node equal to its z-neighbour (the common case along walls) is copied, not recomputed.
+//      The level-3 scan records where each (column, z) leaf sits in breadth-first order.
+//   3. Merge, bottom-up from the leaves to the root's children.  Key = (mask, partial leafmasks)
+//      or (mask, merged ranks of the partial children).  Default (a): a 64-bit key hash inserted
+//      into an open-addressing table (CAS on the hash, atomicMin of the node index), so every
+//      node finds the first occurrence of its key; the full key is compared against it, and a
+//      64-bit collision re-runs the build with (b).  (b): a stable LSD radix sort of (32-bit
+//      hash, node index), segment starts by a max-scan and a full-key compare against the
+//      segment head (a mismatch falls back to a scan of the segment).  Either way a scan of
+//      first-occurrence flags gives each node's rank = order of first occurrence, as mergeLevel's
+//      linear search yields (src/CompressedShadowUtil.h:154-182).
 //   4. Compress: per level a scan of unique node sizes (1+k words, 1+2k at the leaf level),
 //      level bases root first, then one pass writes [mask, absolute child offsets] or
 //      [mask, lo, hi, ...] (src/CompressedShadow.cpp:326-392).
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 
@@ -28,8 +34,18 @@
 namespace cpvs {
 
 constexpr u64 kHashSeed = 0x6A09E667F3BCC909ull;
+constexpr u32 kNoRep = 0xFFFFFFFFu;
 
 __device__ inline u32 fold32(u64 h) { return (u32)(h ^ (h >> 32)); }
+
+__device__ inline u32 fmix32(u32 h) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
 
 // ---------------------------------------------------------------------------------------
 // 1. Inner classification: createChildmask (src/CompressedShadowUtil.cpp:20-54).
@@ -67,6 +83,20 @@ __global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord,
     mask_out[i] = mask;
 }
 
+// Leaf columns: the partial z range [zs, ze) of a level-3 texel, as the node counts define it.
+struct LeafColumns {
+    const float2* lv3;  // pyramid level 3
+    const u32* start;   // exclusive scan of leaves per column
+    u32* bfs;           // (column start + z - zs) -> breadth-first index of the leaf node
+    u32 w3;             // N >> 3
+    float h3;           // (N >> 3) * zTileNum
+    int zlo;            // first level-3 cell of the z-tile
+};
+
+__device__ inline int column_zs(const LeafColumns& lc, u32 col) {
+    return max(lc.zlo, (int)floorf(lc.lv3[col].x * lc.h3));
+}
+
 // Scan functors: partial-child count in, (first child index, children coordinates) out.
 struct ChildCountIn {
     const u32* mask;
@@ -76,8 +106,9 @@ struct EmitChildrenOut {
     const u32* mask;
     const u64* coord;
     u32* cstart;
-    u64* child_coord;
-    u32 child_cap;  // nodes of the child level (exact count); writes beyond it are dropped
+    u64* child_coord;   // used when the child level is classified node by node
+    u32 child_cap;      // nodes of the child level (exact count); writes beyond it are dropped
+    LeafColumns cols;   // used when cols.bfs != null (the children are leaves)
     __device__ void operator()(u32 i, u32 first, u32 k) const {
         cstart[i] = first;
         if (!k) return;
@@ -88,86 +119,118 @@ struct EmitChildrenOut {
         // getChildCoordinates (src/CompressedShadowUtil.cpp:101-117): partial children in
         // ascending child index, child cell doubled.
 #pragma unroll
-        for (u32 ch = 0; ch < 8; ++ch)
-            if (m & (2u << (2 * ch))) {
-                if (j < child_cap) child_coord[j] = pack_xyz(2 * (x + (ch & 1)), 2 * (y + ((ch >> 1) & 1)), 2 * (z + (ch >> 2)));
-                ++j;
+        for (u32 ch = 0; ch < 8; ++ch) {
+            if (!(m & (2u << (2 * ch)))) continue;
+            const u32 cx = x + (ch & 1), cy = y + ((ch >> 1) & 1), cz = z + (ch >> 2);
+            if (j < child_cap) {
+                if (cols.bfs) {
+                    const u32 col = cy * cols.w3 + cx;
+                    const u32 slot = cols.start[col] + (u32)((int)cz - column_zs(cols, col));
+                    if (slot < child_cap) cols.bfs[slot] = j;
+                } else {
+                    child_coord[j] = pack_xyz(2 * cx, 2 * cy, 2 * cz);
+                }
             }
+            ++j;
+        }
     }
 };
 
 // ---------------------------------------------------------------------------------------
 // 2. Leaves: createChildmask1x1x8 + createLeafmask (src/CompressedShadowUtil.cpp:59-99).
-// One wave per node, lane = x + 8y; slice s is partial unless its ballot is 0 or ~0.
+//
+// Slice s of leaf node z3 (depth z = 8*z3 + s) is lit at texel t iff float(z) + 0.5 <= d_t*H
+// (absoluteVisible, H = N*T), i.e. iff 8*z3 + s <= f_t with f_t = floor(d_t*H - 0.5).  With
+// c_t = clamp(f_t - 8*z3 + 1, 0, 8) every texel is a thermometer code over the 8 slices: slices
+// below min(c) are fully lit, slices at or above max(c) fully shadowed, and slice s in between is
+// partial with leafmask ballot(c > s).  The node's merge key (mask and partial leafmasks) and its
+// c vector determine each other, so the key hash is taken over the c vector, lane-parallel.
+// Node z3 equals node z3-1 of the same column unless some f_t - 8*z3 lies in [-8, 6].
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_leaf(const u64* __restrict__ coord, u32 count, const float* __restrict__ depth,
-                                              u32 N, u32 T, u32* __restrict__ mask_out, u64* __restrict__ leafm,
-                                              u32* __restrict__ skey, u32* __restrict__ sval) {
+// One wave per column (the kernel is bound by per-wave latency, so as many independent waves as
+// possible are kept in flight).  Only a run's first node stores its mask, leafmasks and key
+// hash: the merge and the final write read those of first occurrences only, and a key's first
+// occurrence is always the first node of a run (breadth-first order grows with z inside a
+// column).  The other nodes of a run store just the index of the run's first node.
+__global__ __launch_bounds__(256) void k_leaf_columns(LeafColumns lc, const u32* __restrict__ colcnt, u32 ncol,
+                                                      u32 nleaf, const float* __restrict__ depth, u32 N, u32 T,
+                                                      u32* __restrict__ mask_out, u64* __restrict__ leafm,
+                                                      u64* __restrict__ hkey, u32* __restrict__ runhead) {
     const u32 lane = threadIdx.x & 63;
-    const u32 node = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (node >= count) return;  // uniform per wave
-    const u64 c = coord[node];
-    const u32 bx = 4 * unpack_x(c), by = 4 * unpack_y(c), bz = 4 * unpack_z(c);
-    const float d = depth[(size_t)(by + (lane >> 3)) * N + bx + (lane & 7)];
-    const float dh = d * (float)(N * T);
-    u32 mask = 0, k = 0;
-    u64 mine = 0;
-    u64 h = hash_step(kHashSeed, 0);
-#pragma unroll
-    for (u32 s = 0; s < 8; ++s) {
-        const u64 m = __ballot(classify_mid(bz + s, dh) != 0);
-        if (m == ~0ull) {
-            mask |= 1u << (2 * s);
-        } else if (m != 0ull) {
-            mask |= 2u << (2 * s);
-            if (lane == k) mine = m;
-            h = hash_step(h, m);
-            ++k;
+    const u32 col = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
+    if (col >= ncol) return;  // uniform per wave
+    const u32 cnt = colcnt[col];
+    if (cnt == 0) return;
+    const u32 base = lc.start[col];
+    const u32 x3 = col % lc.w3, y3 = col / lc.w3;
+    const int zs = column_zs(lc, col);
+    u32 bfs_chunk = (lane < cnt) ? lc.bfs[base + lane] : 0u;
+    const float d = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
+    const float D = d * (float)(N * T);
+    int f = -1;  // NaN is never lit
+    if (D == D) f = (int)floorf(fminf(fmaxf(D, -2.0f), 4194304.0f) - 0.5f);  // exact for |D| < 2^22
+    const u32 salt1 = (fmix32(lane * 0x9E3779B9u + 0x7F4A7C15u) & 0xFFFFFFu) | 1u;
+    const u32 salt2 = (fmix32(lane * 0x85EBCA77u + 0x165667B1u) & 0xFFFFFFu) | 1u;
+
+    u32 head = 0;
+    for (u32 q = 0; q < cnt; ++q) {
+        if (q > 0 && (q & 63) == 0) {  // walls: the next 64 leaf indices
+            const u32 qi = q + lane;
+            bfs_chunk = (qi < cnt) ? lc.bfs[base + qi] : 0u;
         }
-    }
-    h = hash_step(h, mask);
-    if (lane < 8) leafm[(size_t)node * 8 + lane] = (lane < k) ? mine : 0ull;
-    if (lane == 0) {
-        mask_out[node] = mask;
-        skey[node] = fold32(h);
-        sval[node] = node;
+        const int u = f - 8 * (zs + (int)q);
+        const u32 b = (u32)__builtin_amdgcn_readlane((int)bfs_chunk, (int)(q & 63));
+        if (b >= nleaf) continue;  // only reachable if the column map were inconsistent
+        const bool changed = (q == 0) || (__ballot((u32)(u + 8) <= 14u) != 0ull);
+        if (changed) {
+            const int c = min(max(u + 1, 0), 8);
+            const u32 present = __ockl_wfred_or_u32(1u << c);  // bit v: some texel has c == v
+            const u32 s1 = __ockl_wfred_add_u32(__umul24((u32)c + 1u, salt1));
+            const u32 s2 = __ockl_wfred_add_u32(__umul24((u32)c + 1u, salt2));
+            const u32 cmin = (u32)__builtin_ctz(present), cmax = 31u - (u32)__builtin_clz(present);
+            const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
+            const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min);
+            const u32 k = cmax - cmin;
+            u64 mine = 0;
+            for (u32 s = cmin; s < cmax; ++s) {
+                const u64 m = __ballot(c > (int)s);
+                if (lane == s - cmin) mine = m;
+            }
+            head = b;
+            if (lane < k) leafm[(size_t)b * 8 + lane] = mine;
+            if (lane == 0) {
+                mask_out[b] = mask;
+                hkey[b] = mix64(((u64)s2 << 32 | s1) ^ ((u64)mask * 0x9E3779B97F4A7C15ull));
+            }
+        }
+        if (lane == 0) runhead[b] = head;
     }
 }
 
 // ---------------------------------------------------------------------------------------
 // 3. Merge.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_hash_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
-                                                    const u32* __restrict__ child_rank, u32 count, u32* __restrict__ skey,
-                                                    u32* __restrict__ sval) {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+__device__ inline u64 inner_key_hash(const u32* mask, const u32* cstart, const u32* child_rank, u32 i) {
     const u32 m = mask[i];
     const u32 k = num_children(m);
-    u64 h = hash_step(kHashSeed, 1);
+    u64 h = hash_step(kHashSeed, m);
     if (k) {
         const u32 s = cstart[i];
         for (u32 j = 0; j < k; ++j) h = hash_step(h, child_rank[s + j]);
     }
-    h = hash_step(h, m);
-    skey[i] = fold32(h);
-    sval[i] = i;
+    return h;
 }
-
-// Segment start of each sorted position: max-scan of (key differs from predecessor ? p : 0).
-struct SegHeadIn {
-    const u32* skey;
-    __device__ u32 operator()(u32 p) const { return (p == 0 || skey[p] != skey[p - 1]) ? p : 0u; }
-};
-struct SegStartOut {
-    u32* seg;
-    __device__ void operator()(u32 p, u32 ex, u32 v) const { seg[p] = ex > v ? ex : v; }
-};
 
 struct LeafEq {
     const u32* mask;
     const u64* leafm;
+    const u32* runhead;  // when set, a node's data is read from its run's first node
     __device__ bool operator()(u32 a, u32 b) const {
+        if (runhead) {
+            a = runhead[a];
+            b = runhead[b];
+            if (a == b) return true;
+        }
         const u32 m = mask[a];
         if (m != mask[b]) return false;
         const u32 k = num_children(m);
@@ -192,6 +255,99 @@ struct InnerEq {
             if (child_rank[sa + j] != child_rank[sb + j]) return false;
         return true;
     }
+};
+
+// --- (a) hash table --------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ th, u32* __restrict__ ti, u32 S) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    th[i] = 0ull;
+    ti[i] = kNoRep;
+}
+
+// Open addressing, linear probing; the table has more slots than keys, so every probe ends.
+// Large groups of equal keys hit one slot, so the slot is read first with plain loads: a slot
+// only ever goes from empty to one hash, and its index only decreases, so a stale read merely
+// leads to an atomic that was not needed.
+__device__ inline u32 table_insert(u64* th, u32* ti, u32 S, u64 h, u32 i) {
+    if (h == 0ull) h = 1ull;  // 0 marks an empty slot
+    u32 slot = (u32)(h ^ (h >> 31)) & (S - 1);
+    for (;;) {
+        u64 cur = __builtin_nontemporal_load(th + slot);
+        if (cur == 0ull)
+            cur = atomicCAS(reinterpret_cast<unsigned long long*>(th + slot), 0ull, (unsigned long long)h);
+        if (cur == 0ull || cur == h) {
+            if (__builtin_nontemporal_load(ti + slot) > i) atomicMin(ti + slot, i);
+            return slot;
+        }
+        slot = (slot + 1) & (S - 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_insert_leaf(const u64* __restrict__ hkey, const u32* __restrict__ runhead, u32 n,
+                                                     u64* __restrict__ th, u32* __restrict__ ti, u32 S,
+                                                     u32* __restrict__ tslot) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || runhead[i] != i) return;  // a run's first node has the run's smallest index
+    tslot[i] = table_insert(th, ti, S, hkey[i], i);
+}
+
+__global__ __launch_bounds__(256) void k_insert_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
+                                                      const u32* __restrict__ child_rank, u32 n, u64* __restrict__ th,
+                                                      u32* __restrict__ ti, u32 S, u32* __restrict__ tslot) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tslot[i] = table_insert(th, ti, S, inner_key_hash(mask, cstart, child_rank, i), i);
+}
+
+template <class Eq>
+__global__ __launch_bounds__(256) void k_resolve(const u32* __restrict__ tslot, const u32* __restrict__ ti,
+                                                 const u32* __restrict__ runhead, u32 n, Eq eq, u32* __restrict__ rep,
+                                                 u32* __restrict__ collisions) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (runhead && runhead[i] != i) {
+        rep[i] = kNoRep;  // resolved through its run's first node
+        return;
+    }
+    u32 r = ti[tslot[i]];
+    if (r >= n) r = i;
+    if (r != i && !eq(i, r)) {
+        atomicAdd(collisions, 1u);  // two keys share a 64-bit hash: the build is redone by sorting
+        r = i;
+    }
+    rep[i] = r;
+}
+
+// --- (b) radix sort ----------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_sortkeys_leaf(const u64* __restrict__ hkey, const u32* __restrict__ runhead, u32 n,
+                                                       u32* __restrict__ skey, u32* __restrict__ sval) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u32 h = runhead[i];
+    skey[i] = fold32(hkey[h < n ? h : i]);
+    sval[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_sortkeys_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
+                                                        const u32* __restrict__ child_rank, u32 n, u32* __restrict__ skey,
+                                                        u32* __restrict__ sval) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    skey[i] = fold32(inner_key_hash(mask, cstart, child_rank, i));
+    sval[i] = i;
+}
+
+// Segment start of each sorted position: max-scan of (key differs from predecessor ? p : 0).
+struct SegHeadIn {
+    const u32* skey;
+    __device__ u32 operator()(u32 p) const { return (p == 0 || skey[p] != skey[p - 1]) ? p : 0u; }
+};
+struct SegStartOut {
+    u32* seg;
+    __device__ void operator()(u32 p, u32 ex, u32 v) const { seg[p] = ex > v ? ex : v; }
 };
 
 // Representative (first occurrence) of each node.  Within a segment of equal 32-bit keys the
@@ -219,6 +375,8 @@ __global__ __launch_bounds__(256) void k_representative(const u32* __restrict__ 
     rep[i] = r;
 }
 
+// --- ranks -------------------------------------------------------------------------------------
+
 struct FirstFlagIn {
     const u32* rep;
     __device__ u32 operator()(u32 i) const { return rep[i] == i ? 1u : 0u; }
@@ -234,11 +392,17 @@ struct FirstRankOut {
     }
 };
 
-__global__ __launch_bounds__(256) void k_rank_copies(const u32* __restrict__ rep, u32 count, u32* __restrict__ rank) {
+__global__ __launch_bounds__(256) void k_rank_copies(const u32* __restrict__ rep, const u32* __restrict__ runhead, u32 count,
+                                                     u32* __restrict__ rank) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    const u32 r = rep[i];
-    if (r != i) rank[i] = rank[r];
+    u32 r = rep[i];
+    if (r == i) return;
+    if (r == kNoRep) {
+        const u32 h = runhead[i];
+        r = (h < count) ? rep[h] : kNoRep;
+    }
+    if (r < count) rank[i] = rank[r];
 }
 
 __global__ void k_root_init(u32* rank, u32* uniq, u32* n_uniq) {
@@ -324,35 +488,46 @@ struct Level {
 
 // Device metadata block (u32 words).
 enum : int {
-    kMetaU = 0,                    // uniques per level
-    kMetaS = kMaxLevels,           // words per level
-    kMetaB = 2 * kMaxLevels,       // level base offsets
-    kMetaC = 3 * kMaxLevels,       // children emitted per level (scan totals)
-    kMetaW = 4 * kMaxLevels,       // total words
-    kMetaColl = 4 * kMaxLevels + 1,  // hash collisions seen
-    kMetaRoot = 4 * kMaxLevels + 2,  // root mask
+    kMetaU = 0,                          // uniques per level
+    kMetaS = kMaxLevels,                 // words per level
+    kMetaB = 2 * kMaxLevels,             // level base offsets
+    kMetaC = 3 * kMaxLevels,             // children emitted per level (scan totals)
+    kMetaW = 4 * kMaxLevels,             // total words
+    kMetaColl = 4 * kMaxLevels + 1,      // 64-bit hash collisions (table merge)
+    kMetaRoot = 4 * kMaxLevels + 2,      // root mask
+    kMetaSortColl = 4 * kMaxLevels + 3,  // 32-bit key collisions resolved by a scan (sort merge)
     kMetaWords = 4 * kMaxLevels + 4,
 };
 
 inline dim3 grid1d(u32 n, u32 block = 256) { return dim3(ceil_div<u32>(n ? n : 1, block)); }
 
-}  // namespace
+constexpr int kRetryWithSort = -1000;
 
-int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, cpvs_dag** out) {
+u32 table_slots(u32 n) {
+    u32 S = 1024;
+    while (S < n + n / 2 + 1) S <<= 1;
+    return S;
+}
+
+int build_once(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, int merge_mode, cpvs_dag** out) {
     const int n = p->n;
     const int R = n - 2;
     const bool leaf = n >= 5;
     const int bottom = leaf ? 2 : 0;
     const u32 N = p->N;
+    const bool use_table = merge_mode == CPVS_MERGE_TABLE;
 
     // --- exact node counts (one readback) ---------------------------------------------
     Arena& A = ctx->arena;
+    const u32 w3 = leaf ? (N >> 3) : 0;
+    const u32 ncol = w3 * w3;
     const size_t meta_bytes = arena_bytes(kMaxLevels, sizeof(unsigned long long)) + arena_bytes(kMetaWords, sizeof(u32));
-    // The counts live at the front of the arena until they are read back.
-    int rc = A.reserve(meta_bytes, s);
+    const size_t col_bytes = leaf ? arena_bytes(ncol, 4) : 0;
+    int rc = A.reserve(meta_bytes + col_bytes, s);
     if (rc) return rc;
     unsigned long long* d_counts = A.take<unsigned long long>(kMaxLevels);
-    rc = pyramid_node_counts(p, t, T, d_counts, s);
+    u32* colcnt = leaf ? A.take<u32>(ncol) : nullptr;
+    rc = pyramid_node_counts(p, t, T, d_counts, colcnt, s);
     if (rc) return rc;
     CPVS_HIP_CHECK(hipMemcpyAsync(ctx->pinned, d_counts, sizeof(u64) * kMaxLevels, hipMemcpyDeviceToHost, s));
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
@@ -368,20 +543,31 @@ int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
 
     // --- scratch sizing ----------------------------------------------------------------------
     u32 cmax = 1;
-    for (int L = bottom; L <= R; ++L) cmax = cnt[L] > cmax ? (u32)cnt[L] : cmax;
-    size_t need = meta_bytes;
+    for (int L = bottom; L < R; ++L) cmax = cnt[L] > cmax ? (u32)cnt[L] : cmax;
+    const u32 S = table_slots(cmax);
+    const u32 scan_n = (leaf && ncol > cmax) ? ncol : cmax;
+    size_t need = meta_bytes + col_bytes;
     for (int L = bottom; L <= R; ++L) {
         const size_t c = cnt[L] ? cnt[L] : 1;
         need += arena_bytes(c, 8) + 5 * arena_bytes(c, 4);
     }
-    if (leaf) need += arena_bytes((size_t)(cnt[2] ? cnt[2] : 1) * 8, 8);
-    need += 6 * arena_bytes(cmax, 4);
-    need += arena_bytes(sort_scratch_words(cmax), 4) + arena_bytes(scan_scratch_words(cmax), 4);
+    const size_t c2 = leaf ? (cnt[2] ? cnt[2] : 1) : 1;
+    if (leaf) need += arena_bytes(c2 * 8, 8) + arena_bytes(c2, 8) + 2 * arena_bytes(c2, 4) + arena_bytes(ncol, 4);
+    need += 2 * arena_bytes(cmax, 4);
+    if (use_table) need += arena_bytes(S, 8) + arena_bytes(S, 4);
+    else need += 4 * arena_bytes(cmax, 4) + arena_bytes(sort_scratch_words(cmax), 4);
+    need += arena_bytes(scan_scratch_words(scan_n), 4);
     need += 1 << 20;
-    rc = A.reserve(need, s);  // resets the arena (the counts were already read back)
+    bool moved = false;
+    rc = A.reserve(need, s, &moved);  // keeps the contents when large enough
     if (rc) return rc;
     (void)A.take<unsigned long long>(kMaxLevels);
+    colcnt = leaf ? A.take<u32>(ncol) : nullptr;
     u32* meta = A.take<u32>(kMetaWords);
+    if (leaf && moved) {  // the arena was reallocated: count the columns again
+        rc = pyramid_node_counts(p, t, T, reinterpret_cast<unsigned long long*>(A.base), colcnt, s);
+        if (rc) return rc;
+    }
     CPVS_HIP_CHECK(hipMemsetAsync(meta, 0, sizeof(u32) * kMetaWords, s));
 
     Level lv[kMaxLevels];
@@ -395,15 +581,40 @@ int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
         lv[L].uniq = A.take<u32>(c);
         lv[L].off = A.take<u32>(c);
     }
-    u64* leafm = leaf ? A.take<u64>((size_t)(cnt[2] ? cnt[2] : 1) * 8) : nullptr;
-    u32* skey = A.take<u32>(cmax);
-    u32* sval = A.take<u32>(cmax);
-    u32* ktmp = A.take<u32>(cmax);
-    u32* vtmp = A.take<u32>(cmax);
-    u32* seg = A.take<u32>(cmax);
+    u64* leafm = leaf ? A.take<u64>(c2 * 8) : nullptr;
+    u64* hkey = leaf ? A.take<u64>(c2) : nullptr;
+    u32* runhead = leaf ? A.take<u32>(c2) : nullptr;
+    u32* colbfs = leaf ? A.take<u32>(c2) : nullptr;
+    u32* colstart = leaf ? A.take<u32>(ncol) : nullptr;
     u32* rep = A.take<u32>(cmax);
-    u32* sort_scratch = A.take<u32>(sort_scratch_words(cmax));
-    u32* scan_scratch = A.take<u32>(scan_scratch_words(cmax));
+    u32* tslot = A.take<u32>(cmax);  // table slot of each node, or segment starts when sorting
+    u32* seg = tslot;
+    u64* th = use_table ? A.take<u64>(S) : nullptr;
+    u32* ti = use_table ? A.take<u32>(S) : nullptr;
+    u32 *skey = nullptr, *sval = nullptr, *ktmp = nullptr, *vtmp = nullptr, *sort_scratch = nullptr;
+    if (!use_table) {
+        skey = A.take<u32>(cmax);
+        sval = A.take<u32>(cmax);
+        ktmp = A.take<u32>(cmax);
+        vtmp = A.take<u32>(cmax);
+        sort_scratch = A.take<u32>(sort_scratch_words(cmax));
+    }
+    u32* scan_scratch = A.take<u32>(scan_scratch_words(scan_n));
+    if (A.used > A.cap) {
+        set_error("build: scratch layout exceeds its reservation (%zu > %zu)", A.used, A.cap);
+        return CPVS_EINTERNAL;
+    }
+
+    LeafColumns lc{};
+    if (leaf) {
+        lc.lv3 = p->lv[3];
+        lc.start = colstart;
+        lc.bfs = colbfs;
+        lc.w3 = w3;
+        lc.h3 = (float)(w3 * T);
+        lc.zlo = (int)(t * w3);
+        device_scan(ArrIn{colcnt}, ArrOutExclusive{colstart}, ncol, OpSum{}, nullptr, scan_scratch, s);
+    }
 
     // --- 1/2. BFS classification -------------------------------------------------------------
     const u64 root = pack_xyz(0, 0, 2 * t);  // create(): root offset (0,0,2*zTileIndex)
@@ -414,16 +625,16 @@ int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
         if (c == 0) break;
         deepest = L;
         if (leaf && L == 2) {
-            CPVS_LAUNCH("leaf", k_leaf, grid1d(c, 4), dim3(256), 0, s, lv[2].coord, c, p->depth, N, T, lv[2].mask, leafm,
-                               skey, sval);
+            CPVS_LAUNCH("leaf_columns", k_leaf_columns, grid1d(ncol, 4), dim3(256), 0, s, lc, colcnt, ncol, c, p->depth, N,
+                        T, lv[2].mask, leafm, hkey, runhead);
             break;
         }
         CPVS_LAUNCH("classify", k_classify, grid1d(c), dim3(256), 0, s, lv[L].coord, c, p->lv[L], p->depth, N, L, T,
-                           lv[L].mask);
+                    lv[L].mask);
         if (L > bottom) {
-            device_scan(ChildCountIn{lv[L].mask},
-                        EmitChildrenOut{lv[L].mask, lv[L].coord, lv[L].cstart, lv[L - 1].coord, lv[L - 1].count}, c,
-                        OpSum{}, meta + kMetaC + L, scan_scratch, s);
+            EmitChildrenOut emit{lv[L].mask, lv[L].coord, lv[L].cstart, lv[L - 1].coord, lv[L - 1].count, LeafColumns{}};
+            if (leaf && L == 3) emit.cols = lc;
+            device_scan(ChildCountIn{lv[L].mask}, emit, c, OpSum{}, meta + kMetaC + L, scan_scratch, s);
         }
     }
     CPVS_HIP_CHECK(hipGetLastError());
@@ -431,24 +642,40 @@ int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
     // --- 3. merge, bottom-up (the root is never merged) --------------------------------------
     for (int L = deepest; L < R; ++L) {
         const u32 c = lv[L].count;
-        if (leaf && L == 2) {
-            // keys already written by k_leaf
+        const bool is_leaf = leaf && L == 2;
+        const u32* child_rank = (L > deepest) ? lv[L - 1].rank : nullptr;
+        const u32* rh = is_leaf ? runhead : nullptr;
+        if (use_table) {
+            const u32 Sl = table_slots(c);
+            CPVS_LAUNCH("table_clear", k_table_clear, grid1d(Sl), dim3(256), 0, s, th, ti, Sl);
+            if (is_leaf) {
+                CPVS_LAUNCH("insert_leaf", k_insert_leaf, grid1d(c), dim3(256), 0, s, hkey, runhead, c, th, ti, Sl, tslot);
+                CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c), dim3(256), 0, s, tslot, ti, runhead, c,
+                            LeafEq{lv[2].mask, leafm, nullptr}, rep, meta + kMetaColl);
+            } else {
+                CPVS_LAUNCH("insert_inner", k_insert_inner, grid1d(c), dim3(256), 0, s, lv[L].mask, lv[L].cstart, child_rank, c,
+                            th, ti, Sl, tslot);
+                CPVS_LAUNCH("resolve", k_resolve<InnerEq>, grid1d(c), dim3(256), 0, s, tslot, ti, nullptr, c,
+                            InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaColl);
+            }
         } else {
-            const u32* child_rank = (L > deepest) ? lv[L - 1].rank : nullptr;
-            CPVS_LAUNCH("hash_inner", k_hash_inner, grid1d(c), dim3(256), 0, s, lv[L].mask, lv[L].cstart, child_rank, c, skey, sval);
-        }
-        radix_sort_pairs(skey, sval, ktmp, vtmp, c, 32, sort_scratch, s);
-        device_scan(SegHeadIn{skey}, SegStartOut{seg}, c, OpMax{}, nullptr, scan_scratch, s);
-        if (leaf && L == 2) {
-            CPVS_LAUNCH("representative", k_representative<LeafEq>, grid1d(c), dim3(256), 0, s, sval, seg, c, LeafEq{lv[2].mask, leafm},
-                               rep, meta + kMetaColl);
-        } else {
-            const u32* child_rank = (L > deepest) ? lv[L - 1].rank : nullptr;
-            CPVS_LAUNCH("representative", k_representative<InnerEq>, grid1d(c), dim3(256), 0, s, sval, seg, c,
-                               InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaColl);
+            if (is_leaf)
+                CPVS_LAUNCH("sortkeys", k_sortkeys_leaf, grid1d(c), dim3(256), 0, s, hkey, runhead, c, skey, sval);
+            else
+                CPVS_LAUNCH("sortkeys", k_sortkeys_inner, grid1d(c), dim3(256), 0, s, lv[L].mask, lv[L].cstart, child_rank, c,
+                            skey, sval);
+            radix_sort_pairs(skey, sval, ktmp, vtmp, c, 32, sort_scratch, s);
+            device_scan(SegHeadIn{skey}, SegStartOut{seg}, c, OpMax{}, nullptr, scan_scratch, s);
+            if (is_leaf)
+                CPVS_LAUNCH("representative", k_representative<LeafEq>, grid1d(c), dim3(256), 0, s, sval, seg, c,
+                            LeafEq{lv[2].mask, leafm, runhead}, rep, meta + kMetaSortColl);
+            else
+                CPVS_LAUNCH("representative", k_representative<InnerEq>, grid1d(c), dim3(256), 0, s, sval, seg, c,
+                            InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaSortColl);
+            rh = nullptr;  // every node got its representative directly
         }
         device_scan(FirstFlagIn{rep}, FirstRankOut{lv[L].rank, lv[L].uniq}, c, OpSum{}, meta + kMetaU + L, scan_scratch, s);
-        CPVS_LAUNCH("rank_copies", k_rank_copies, grid1d(c), dim3(256), 0, s, rep, c, lv[L].rank);
+        CPVS_LAUNCH("rank_copies", k_rank_copies, grid1d(c), dim3(256), 0, s, rep, rh, c, lv[L].rank);
     }
     CPVS_LAUNCH("root_init", k_root_init, dim3(1), dim3(1), 0, s, lv[R].rank, lv[R].uniq, meta + kMetaU + R);
     CPVS_HIP_CHECK(hipGetLastError());
@@ -468,6 +695,7 @@ int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
     u32 hm[kMetaWords];
     std::memcpy(hm, ctx->pinned, sizeof(hm));
 
+    if (use_table && hm[kMetaColl] != 0) return kRetryWithSort;
     for (int L = deepest + 1; L <= R; ++L) {
         if (hm[kMetaC + L] != lv[L - 1].count) {
             set_error("build: level %d emitted %u children, expected %u", L, hm[kMetaC + L], lv[L - 1].count);
@@ -487,6 +715,7 @@ int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
 
     u32* d_words = nullptr;
     if (hipMalloc(&d_words, sizeof(u32) * (total ? total : 1)) != hipSuccess) {
+        (void)hipGetLastError();
         set_error("build: hipMalloc of %llu DAG words failed", (unsigned long long)total);
         return CPVS_ENOMEM;
     }
@@ -494,13 +723,13 @@ int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
         const u32 U = hm[kMetaU + L];
         const u32 base = hm[kMetaB + L];
         if (leaf && L == 2) {
-            CPVS_LAUNCH("write_leaf", k_write_leaf, grid1d(U), dim3(256), 0, s, lv[2].uniq, U, lv[2].mask, leafm, lv[2].off, base,
-                               d_words);
+            CPVS_LAUNCH("write_leaf", k_write_leaf, grid1d(U), dim3(256), 0, s, lv[2].uniq, U, lv[2].mask, leafm, lv[2].off,
+                        base, d_words);
         } else {
             const bool has_children = L > deepest;
-            CPVS_LAUNCH("write_inner", k_write_inner, grid1d(U), dim3(256), 0, s, lv[L].uniq, U, lv[L].mask, lv[L].cstart, lv[L].off,
-                               base, has_children ? lv[L - 1].rank : nullptr, has_children ? lv[L - 1].off : nullptr,
-                               has_children ? hm[kMetaB + L - 1] : 0u, d_words);
+            CPVS_LAUNCH("write_inner", k_write_inner, grid1d(U), dim3(256), 0, s, lv[L].uniq, U, lv[L].mask, lv[L].cstart,
+                        lv[L].off, base, has_children ? lv[L - 1].rank : nullptr, has_children ? lv[L - 1].off : nullptr,
+                        has_children ? hm[kMetaB + L - 1] : 0u, d_words);
         }
     }
     hipError_t e = hipGetLastError();
@@ -519,12 +748,22 @@ int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
     d->N = N;
     d->zTileIndex = t;
     d->zTileNum = T;
+    d->merge_mode = merge_mode;
+    d->key_collisions = hm[kMetaSortColl];
     for (int L = 0; L < n && L < kMaxLevels; ++L) {
         d->nodes[L] = (L >= bottom && L <= R) ? cnt[L] : 0;
         d->uniques[L] = (L >= deepest && L <= R) ? hm[kMetaU + L] : 0;
     }
     *out = d;
     return CPVS_OK;
+}
+
+}  // namespace
+
+int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, cpvs_dag** out) {
+    int rc = build_once(ctx, p, t, T, s, ctx->merge_mode, out);
+    if (rc == kRetryWithSort) rc = build_once(ctx, p, t, T, s, CPVS_MERGE_SORT, out);
+    return rc;
 }
 
 }  // namespace cpvs

@@ -47,9 +47,11 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
     return CPVS_EHIP;
 }
 
-int Arena::reserve(size_t bytes, hipStream_t s) {
+int Arena::reserve(size_t bytes, hipStream_t s, bool* moved) {
     used = 0;
+    if (moved) *moved = false;
     if (bytes <= cap) return CPVS_OK;
+    if (moved) *moved = true;
     if (base) {
         CPVS_HIP_CHECK(hipStreamSynchronize(s));
         CPVS_HIP_CHECK(hipFree(base));
@@ -116,6 +118,9 @@ int cpvs_ctx_create(int device, cpvs_ctx** out) {
     CPVS_HIP_CHECK(hipSetDevice(device));
     cpvs_ctx* c = new cpvs_ctx();
     c->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->num_cus = cus;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -172,6 +177,21 @@ int cpvs_ctx_profile_read(cpvs_ctx* ctx, char* buf, uint64_t cap) {
         return CPVS_EINVAL;
     }
     std::memcpy(buf, out.c_str(), out.size() + 1);
+    return CPVS_OK;
+}
+
+int cpvs_ctx_set_merge(cpvs_ctx* ctx, int mode) {
+    CHECK_ARG(ctx, "ctx_set_merge: NULL ctx");
+    CHECK_ARG(mode == CPVS_MERGE_TABLE || mode == CPVS_MERGE_SORT, "ctx_set_merge: mode %d", mode);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->merge_mode = mode;
+    return CPVS_OK;
+}
+
+int cpvs_dag_merge_info(const cpvs_dag* d, int* mode, uint64_t* key_collisions) {
+    CHECK_ARG(d, "dag_merge_info: NULL dag");
+    if (mode) *mode = d->merge_mode;
+    if (key_collisions) *key_collisions = d->key_collisions;
     return CPVS_OK;
 }
 
@@ -317,7 +337,7 @@ int cpvs_estimate_build(cpvs_ctx* ctx, const cpvs_pyramid* p, uint32_t t, uint32
     rc = ctx->arena.reserve(arena_bytes(kMaxLevels, 8), s);
     if (rc) return rc;
     unsigned long long* d_counts = ctx->arena.take<unsigned long long>(kMaxLevels);
-    rc = pyramid_node_counts(p, t, T, d_counts, s);
+    rc = pyramid_node_counts(p, t, T, d_counts, nullptr, s);
     if (rc) return rc;
     CPVS_HIP_CHECK(hipMemcpyAsync(ctx->pinned, d_counts, sizeof(u64) * kMaxLevels, hipMemcpyDeviceToHost, s));
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
@@ -334,9 +354,18 @@ int cpvs_estimate_build(cpvs_ctx* ctx, const cpvs_pyramid* p, uint32_t t, uint32
         // Pre-merge words: mask + one pointer per child (two words per partial slice at leaves).
         words += cnt[L] + ((L > bottom) ? cnt[L - 1] : 0) + ((bottom == 2 && L == 2) ? 16 * cnt[L] : 0);
     }
-    const u64 leaf = (bottom == 2) ? arena_bytes((cnt[2] ? cnt[2] : 1) * 8, 8) : 0;
-    const u64 temps = 6 * arena_bytes(cmax, 4) + arena_bytes(sort_scratch_words((u32)cmax), 4) +
-                      arena_bytes(scan_scratch_words((u32)cmax), 4) + (1 << 20);
+    const u64 c2 = (bottom == 2) ? (cnt[2] ? cnt[2] : 1) : 0;
+    const u64 ncol = (bottom == 2) ? (u64)(p->N >> 3) * (p->N >> 3) : 0;
+    const u64 leaf = (bottom == 2) ? arena_bytes(c2 * 8, 8) + arena_bytes(c2, 8) + 2 * arena_bytes(c2, 4) +
+                                         2 * arena_bytes(ncol, 4)
+                                   : 0;
+    u64 slots = 1024;
+    while (slots < cmax + cmax / 2 + 1) slots <<= 1;
+    const u64 merge = (ctx->merge_mode == CPVS_MERGE_TABLE)
+                          ? arena_bytes(slots, 8) + arena_bytes(slots, 4)
+                          : 4 * arena_bytes(cmax, 4) + arena_bytes(sort_scratch_words((u32)cmax), 4);
+    const u64 temps = 2 * arena_bytes(cmax, 4) + merge + arena_bytes(scan_scratch_words((u32)(cmax > ncol ? cmax : ncol)), 4) +
+                      (1 << 20);
     if (nodes)
         for (int L = 0; L < n; ++L) nodes[L] = cnt[L];
     if (scratch_bytes) *scratch_bytes = per_level + leaf + temps;

@@ -21,7 +21,9 @@ struct Arena {
     char* base = nullptr;
     size_t cap = 0;
     size_t used = 0;
-    int reserve(size_t bytes, hipStream_t s);  // ensure capacity, resets `used`
+    // Ensure capacity and reset `used`; *moved (optional) tells whether the buffer was reallocated
+    // (its contents are then lost).
+    int reserve(size_t bytes, hipStream_t s, bool* moved = nullptr);
     template <class T>
     T* take(size_t count) {
         const size_t bytes = (count * sizeof(T) + 255) & ~(size_t)255;
@@ -38,6 +40,7 @@ inline size_t arena_bytes(size_t count, size_t elem) { return (count * elem + 25
 
 struct cpvs_ctx {
     int device = 0;
+    int num_cus = 256;  // compute units of the device (persistent grids size themselves to it)
     hipStream_t stream = nullptr;
     cpvs::Arena arena;
     cpvs::u64* pinned = nullptr;  // host readback staging (pinned)
@@ -45,6 +48,7 @@ struct cpvs_ctx {
     std::mutex mu;
     bool prof_on = false;
     cpvs::Prof prof;
+    int merge_mode = CPVS_MERGE_TABLE;
 };
 
 struct cpvs_pyramid {
@@ -66,6 +70,8 @@ struct cpvs_dag {
     uint32_t N = 0, zTileIndex = 0, zTileNum = 1;
     uint64_t nodes[cpvs::kMaxLevels] = {};    // nodes per level before merging
     uint64_t uniques[cpvs::kMaxLevels] = {};  // nodes per level after merging
+    int merge_mode = CPVS_MERGE_TABLE;        // merge path that produced the words
+    uint32_t key_collisions = 0;              // 32-bit sort-key collisions resolved by scanning
 };
 
 struct cpvs_container {
@@ -86,7 +92,8 @@ namespace cpvs {
 // pyramid.hip
 int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s);
 // Exact node counts per level (counts[L] = nodes at octree level L) for z-tile t of T.
-int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, hipStream_t s);
+// col_count (may be null) receives the partial-cell count of every level-3 texel (leafmask mode).
+int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, u32* col_count, hipStream_t s);
 
 // build.hip
 int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, cpvs_dag** out);

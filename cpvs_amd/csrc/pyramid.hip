@@ -200,28 +200,39 @@ struct CountArgs {
     const float2* lv[kMaxLevels];
     u32 N, T, t;
     unsigned long long* counts;  // counts[L-1] += partial cells of level L
+    u32* col_count;              // per-texel partial count of level 3 (leaf columns), may be null
 };
+
+constexpr u32 kCountPerThread = 8;
+constexpr u32 kCountTile = 256 * kCountPerThread;
 
 __global__ __launch_bounds__(256) void k_node_counts(CountArgs a) {
     __shared__ u32 s_wave[4];
     int k = 0;
     while (k + 1 < a.nlev && blockIdx.x >= a.block_start[k + 1]) ++k;
     const u32 w = a.width[k];
-    const u32 j = (blockIdx.x - a.block_start[k]) * 256 + threadIdx.x;
     const int L = a.level[k];
-    u32 c = 0;
-    if (j < w * w) {
-        const float2 v = a.lv[k][j];
-        const float h = (float)((a.N >> L) * a.T);
-        const float lo = floorf(v.x * h), hi = ceilf(v.y * h);
-        const int zlo = (int)(a.t * (a.N >> L)), zhi = (int)((a.t + 1) * (a.N >> L));
-        const int s = max(zlo, (int)lo), e = min(zhi, (int)hi);
-        c = (e > s) ? (u32)(e - s) : 0u;
+    const float h = (float)((a.N >> L) * a.T);
+    const int zlo = (int)(a.t * (a.N >> L)), zhi = (int)((a.t + 1) * (a.N >> L));
+    u32* colc = (L == 3) ? a.col_count : nullptr;
+    const u32 base = (blockIdx.x - a.block_start[k]) * kCountTile;
+    u32 acc = 0;
+#pragma unroll
+    for (u32 r = 0; r < kCountPerThread; ++r) {
+        const u32 j = base + r * 256 + threadIdx.x;
+        if (j < w * w) {
+            const float2 v = a.lv[k][j];
+            const float lo = floorf(v.x * h), hi = ceilf(v.y * h);
+            const int s = max(zlo, (int)lo), e = min(zhi, (int)hi);
+            const u32 c = (e > s) ? (u32)(e - s) : 0u;
+            if (colc) colc[j] = c;
+            acc += c;
+        }
     }
     // Block reduction, then one 64-bit atomic per block.
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-    if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = c;
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+    if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         const u32 tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
@@ -229,7 +240,7 @@ __global__ __launch_bounds__(256) void k_node_counts(CountArgs a) {
     }
 }
 
-int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, hipStream_t s) {
+int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, u32* col_count, hipStream_t s) {
     CountArgs a = {};
     const int n = p->n;
     const int R = n - 2;
@@ -238,6 +249,7 @@ int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long*
     a.T = T;
     a.t = t;
     a.counts = d_counts;
+    a.col_count = (n >= 5) ? col_count : nullptr;
     u32 blocks = 0;
     for (int L = Lmin; L <= R; ++L) {
         const u32 w = p->N >> L;
@@ -249,7 +261,7 @@ int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long*
             set_error("node counts: pyramid level %d not stored", L);
             return CPVS_EINTERNAL;
         }
-        blocks += ceil_div<u32>(w * w, 256);
+        blocks += ceil_div<u32>(w * w, kCountTile);
         ++a.nlev;
     }
     a.block_start[a.nlev] = blocks;

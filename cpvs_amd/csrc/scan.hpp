@@ -123,6 +123,54 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(In in, Out out, u32 
     }
 }
 
+// Single-block scan for n <= kScanTile: one launch.
+template <class In, class Out, class Op>
+__global__ __launch_bounds__(kScanThreads) void k_scan_single(In in, Out out, u32 n, u32* total, Op op) {
+    __shared__ u32 s_wave[kScanThreads / kWave];
+    u32 carry = Op::identity;
+    for (int r = 0; r < kScanRounds; ++r) {
+        const u32 r0 = r * kScanThreads;
+        if (r0 >= n) break;  // uniform across the block
+        const u32 i = r0 + threadIdx.x;
+        const u32 v = (i < n) ? in(i) : Op::identity;
+        u32 tot;
+        const u32 ex = block_exclusive_scan(v, s_wave, &tot, op);
+        if (i < n) out(i, op(carry, ex), v);
+        carry = op(carry, tot);
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+// Down-sweep that derives its own carry from the tile partials (nparts <= kScanPartialsInline):
+// two launches in total instead of three.
+constexpr u32 kScanPartialsInline = 4 * kScanThreads;
+
+template <class In, class Out, class Op>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down_inline(In in, Out out, u32 n, const u32* partials, u32 nparts,
+                                                                   u32* total, Op op) {
+    __shared__ u32 s_wave[kScanThreads / kWave];
+    u32 pre = Op::identity, all = Op::identity;
+    for (u32 k = threadIdx.x; k < nparts; k += kScanThreads) {
+        const u32 v = partials[k];
+        if (k < blockIdx.x) pre = op(pre, v);
+        all = op(all, v);
+    }
+    u32 carry = block_reduce(pre, s_wave, op);
+    const u32 grand = block_reduce(all, s_wave, op);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && total) *total = grand;
+    const u32 base = blockIdx.x * kScanTile;
+    for (int r = 0; r < kScanRounds; ++r) {
+        const u32 r0 = base + r * kScanThreads;
+        if (r0 >= n) break;  // uniform across the block
+        const u32 i = r0 + threadIdx.x;
+        const u32 v = (i < n) ? in(i) : Op::identity;
+        u32 tot;
+        const u32 ex = block_exclusive_scan(v, s_wave, &tot, op);
+        if (i < n) out(i, op(carry, ex), v);
+        carry = op(carry, tot);
+    }
+}
+
 // Plain-array functors for device_scan.
 struct ArrIn {
     const u32* a;
@@ -138,13 +186,22 @@ inline size_t scan_scratch_words(u32 n) { return (size_t)ceil_div<u32>(n ? n : 1
 
 // out(i, exclusive_prefix_i, value_i) for all i < n; *d_total (device, may be null) gets the
 // reduction of all values.  scratch: scan_scratch_words(n) u32 of device memory.
+// One launch for n <= 2048, two up to 2M elements, three beyond.
 template <class In, class Out, class Op>
 inline void device_scan(In in, Out out, u32 n, Op op, u32* d_total, u32* scratch, hipStream_t s) {
-    const u32 nb = ceil_div<u32>(n ? n : 1, kScanTile);
-    if (n > 0) CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, n, scratch, op);
-    else CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op>), dim3(1), dim3(kScanThreads), 0, s, in, 0u, scratch, op);
+    if (n <= kScanTile) {
+        CPVS_LAUNCH("scan_single", (k_scan_single<In, Out, Op>), dim3(1), dim3(kScanThreads), 0, s, in, out, n, d_total, op);
+        return;
+    }
+    const u32 nb = ceil_div<u32>(n, kScanTile);
+    CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, n, scratch, op);
+    if (nb <= kScanPartialsInline) {
+        CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, scratch,
+                    nb, d_total, op);
+        return;
+    }
     CPVS_LAUNCH("scan_partials", k_scan_partials<Op>, dim3(1), dim3(kScanThreads), 0, s, scratch, nb, d_total, op);
-    if (n > 0) CPVS_LAUNCH("scan_down", (k_scan_down<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, scratch, op);
+    CPVS_LAUNCH("scan_down", (k_scan_down<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, scratch, op);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -160,6 +217,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const u32* __restri
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const u32* __restrict__ kin, const u32* __restrict__ vin,
                                                                u32* __restrict__ kout, u32* __restrict__ vout, u32 n,
                                                                int shift, const u32* __restrict__ offsets, u32 nblocks);
+
+// Pairs up to this count are sorted by one block in LDS (bitonic network on key<<32 | value).
+constexpr u32 kSmallSort = 4096;
 
 // Scratch words for radix_sort_pairs with n pairs.
 inline size_t sort_scratch_words(u32 n) {

@@ -33,6 +33,10 @@ extern "C" {
 #define CPVS_VISIBLE 1
 #define CPVS_PARTIAL 2
 
+/* Subtree merge paths (cpvs_ctx_set_merge); both give the same words. */
+#define CPVS_MERGE_TABLE 0 /* 64-bit key hash table, first occurrence by atomic min (default) */
+#define CPVS_MERGE_SORT 1  /* LSD radix sort of (32-bit key hash, node index) */
+
 /* cpvs_pyramid_build flags. */
 #define CPVS_PYR_ALL_LEVELS 1u   /* also store levels 1 and 2, which the build never reads */
 #define CPVS_PYR_BORROW_DEPTH 2u /* depth is device memory the caller keeps alive; not copied */
@@ -56,6 +60,11 @@ int cpvs_ctx_stream(cpvs_ctx *ctx, void **stream);
  * kernel name: "name<TAB>total_ms<TAB>launches".  (No reference counterpart: the reference
  * times the whole precompute with std::chrono, src/main.cpp:228-234.) */
 int cpvs_ctx_profile(cpvs_ctx *ctx, int mode);
+/* Selects how builds merge common subtrees, the replacement of mergeLevel's linear search
+ * (src/CompressedShadowUtil.h:154-182): CPVS_MERGE_TABLE (default) or CPVS_MERGE_SORT.  A
+ * 64-bit hash collision in the table path is detected by the full-key compare and the build is
+ * redone with the sort path. */
+int cpvs_ctx_set_merge(cpvs_ctx *ctx, int mode);
 int cpvs_ctx_profile_read(cpvs_ctx *ctx, char *buf, uint64_t cap);
 
 /* MinMaxHierarchy::MinMaxHierarchy(const ImageF&) (src/MinMaxHierarchy.h:23,
@@ -93,6 +102,8 @@ int cpvs_build(cpvs_ctx *ctx, const cpvs_pyramid *p, uint32_t zTileIndex, uint32
 int cpvs_dag_info(const cpvs_dag *d, uint32_t *num_levels, uint64_t *num_words, int *total_visibility);
 /* Node counts per level before and after merging (cap entries each, may be NULL). */
 int cpvs_dag_level_stats(const cpvs_dag *d, uint64_t *nodes, uint64_t *uniques, int cap);
+/* Merge path that produced the DAG and the 32-bit sort-key collisions it resolved. */
+int cpvs_dag_merge_info(const cpvs_dag *d, int *mode, uint64_t *key_collisions);
 /* CompressedShadow::getDAG (src/CompressedShadow.h:75-77): copy the words to host memory. */
 int cpvs_dag_copy_words(const cpvs_dag *d, uint32_t *host, uint64_t cap_words);
 int cpvs_dag_device_words(const cpvs_dag *d, const uint32_t **device_words);

@@ -135,6 +135,31 @@ def test_generator_maps_bit_exact(ctx, N, T):
         assert cs.getTotalVisibility() == O.total_visibility(wr)
 
 
+@pytest.fixture(scope="module")
+def sort_ctx():
+    c = cp.Context(0)
+    c.set_merge("sort")
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("N,T", [(16, 1), (64, 2), (256, 1), (1024, 1), (1024, 4)])
+def test_sort_merge_path_bit_exact(sort_ctx, N, T):
+    """The radix-sort merge gives the same words as the hash-table merge and the oracle."""
+    D = O.gen_depth(N, SEED0 + 3 * N + T, max(1, 32 * N * N // (1 << 20)), N >= 128)
+    pyr = O.Pyramid(D)
+    mm_sort = cp.MinMaxHierarchy(D, ctx=sort_ctx)
+    mm_table = cp.MinMaxHierarchy(D)
+    for t in range(T):
+        _say("sort merge", N, T, t)
+        a = cp.CompressedShadow.create(mm_sort, t, T)
+        b = cp.CompressedShadow.create(mm_table, t, T)
+        assert a.merge_info()[0] == "sort" and b.merge_info()[0] == "table"
+        wr, _ = O.create(pyr, t, T, merge_mode=1)
+        assert np.array_equal(a.getDAG(), wr)
+        assert np.array_equal(b.getDAG(), wr)
+
+
 @pytest.mark.parametrize("N", [16, 64, 256])
 def test_random_maps_bit_exact(ctx, N):
     _say("random map", N)
@@ -316,6 +341,7 @@ def test_full_size_c1_properties(ctx):
     cs = cp.CompressedShadow.create(mm)
     n = cs.getNumLevels()
     w = cs.getDAG()
+    assert cs.merge_info()[0] == "table"  # no 64-bit key collision forced the sort path
     _say("C1 check layout", len(w))
     check_dag(w, n)
     D = d.numpy()
