@@ -41,9 +41,31 @@ def pct(xs, p):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-def leaf_bytes(nodes_leaf):
-    # k_leaf: 64 depth texels (256 B) read, mask 4 B + 8 leafmasks 64 B + sort key/value 8 B written.
-    return nodes_leaf * (256 + 4 + 64 + 8)
+def leaf_columns_bytes(N, nodes_leaf):
+    """Compulsory traffic of k_leaf_columns: the 8x8 depth block of every level-3 column (256 B),
+    and per leaf node its breadth-first index read and its run-head index written (8 B).  The
+    first node of each run also writes mask, hash and leafmasks; those bytes are left out, so
+    the figure is a lower bound."""
+    return 256 * (N // 8) ** 2 + 8 * nodes_leaf
+
+
+def pyramid_root_bytes(N):
+    """k_pyr_tile root launch: read the N^2 depth map, write pyramid levels 3..6 as (min,max)."""
+    return 4 * N * N + 8 * sum((N >> L) ** 2 for L in range(3, 7))
+
+
+def load_traffic(N):
+    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
+    (committed summary, calibrated against a device copy of known size); {} if not measured."""
+    path = os.path.join(ROOT, "profiles", f"c1_{N}_traffic.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith("#") or not line.strip():
+                continue
+            name, fb, wb = line.split()
+            out[name] = int(fb) + int(wb)
+    return out
 
 
 def main():
@@ -77,6 +99,24 @@ def main():
     nq = queries.shape[0]
     vis = cp.DeviceArray((nq,), np.uint8, ctx)
     stream = ctx.stream
+
+    # Device copy of known size: achievable HBM bandwidth, and the reference point for scaling
+    # rocprof FETCH_SIZE / WRITE_SIZE counters.
+    copy_bytes = 256 << 20
+    src = cp.DeviceArray((copy_bytes // 4,), np.float32, ctx)
+    dst = cp.DeviceArray((copy_bytes // 4,), np.float32, ctx)
+    copy_ms = []
+    for _ in range(5):
+        c0 = ctx.event(stream)
+        cp._lib.call("cpvs_memcpy_d2d", ctx.handle, dst.ptr, src.ptr, copy_bytes, stream)
+        c1 = ctx.event(stream)
+        copy_ms.append(ctx.elapsed_ms(c0, c1))
+        ctx.destroy_event(c0)
+        ctx.destroy_event(c1)
+    copy_gbs = 2 * copy_bytes / (min(copy_ms) * 1e-3) / 1e9
+    src.free()
+    dst.free()
+    log(f"[bench] device copy of {copy_bytes >> 20} MiB: {min(copy_ms):.3f} ms = {copy_gbs:.0f} GB/s (read + write)")
 
     build_ms, lookup_ms, step_ms = [], [], []
     words = None
@@ -135,29 +175,29 @@ def main():
         log(f"    {k:18s} {ms / prof_steps:9.3f} ms/step  {cnt // prof_steps:5d} launches/step")
 
     nodes, uniq = levels
-    algo = {
-        # bytes per step for the kernels whose traffic has a closed form
-        "leaf": leaf_bytes(int(nodes[2])),
-        "pyr_tile_root": 4 * texels + 8 * sum((N >> L) ** 2 for L in range(3, 7)),
+    algo = {  # algorithmic bytes per profiled step of the kernels with a closed form
+        "leaf_columns": leaf_columns_bytes(N, int(nodes[2])),
+        "pyr_tile_root": pyramid_root_bytes(N),
         "evaluate4": 17 * nq,
     }
-    rname, (rms, rcnt) = dominant
-    if rname not in algo:
-        # fall back to the largest kernel with a closed form
-        cands = [(k, prof[k]) for k in algo if k in prof and not k.startswith("evaluate")]
-        rname, (rms, rcnt) = max(cands, key=lambda kv: kv[1][0])
-    per_launch_ms = rms / rcnt
-    launches_per_step = rcnt / prof_steps
-    bytes_per_launch = algo[rname] / launches_per_step
-    achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
-    roofline = {"kernel": rname, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+
+    traffic = load_traffic(N)
+
+    def roof(name):
+        ms, cnt = prof[name]
+        per_launch_ms = ms / cnt
+        bytes_per_launch = algo[name] / (cnt / prof_steps)
+        ach = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+        return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic.get(name),
                 "avg_launch_us": round(per_launch_ms * 1e3, 2),
                 "bytes_per_launch": int(bytes_per_launch)}
-    lk_ms = prof.get("evaluate4", (0.0, 1))
-    lookup_roofline = {"kernel": "evaluate4", "bound": "latency (dependent loads)",
-                       "achieved": round(17 * nq / (lk_ms[0] / lk_ms[1] * 1e-3) / 1e9, 1), "unit": "GB/s",
-                       "peak": HBM_PEAK_GBS}
+
+    rname = dominant[0] if dominant[0] in algo else "leaf_columns"
+    roofline = roof(rname)
+    pyramid_roofline = roof("pyr_tile_root")
+    lookup_roofline = roof("evaluate4")
+    lookup_roofline["bound"] = "latency (dependent loads; DAG cache-resident)"
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -170,17 +210,22 @@ def main():
         t1 = time.perf_counter()
         assert len(w) == words, (len(w), words)
         threads = max(1, min(16, os.cpu_count() or 1))
-        sample = 2_000_000
-        qh = queries.numpy()[:sample]
+        sample = nq
+        qh = queries.numpy()
         grid = O.top_level_grid([w])
         t2 = time.perf_counter()
         O.evaluate(w, grid, len(nodes), 0, qh, nthreads=threads)
         t3 = time.perf_counter()
+        t4 = time.perf_counter()
+        O.evaluate(w, grid, len(nodes), 0, qh[:2_000_000], nthreads=1)
+        t5 = time.perf_counter()
         cpu = {"value": round(texels / (t1 - t0) / 1e6, 3), "unit": "Mtexels/s", "cores": 4, "kind": "port",
                "sample": f"one full {N}^2 build: MinMaxHierarchy on 4 threads + create() single-threaded with the "
                          f"hash-table mergeLevel (word-identical to the linear search), {t1 - t0:.2f} s",
                "lookup_G_per_s": round(sample / (t3 - t2) / 1e9, 4), "lookup_threads": threads,
-               "lookup_sample": f"traverse.cs transcription over the first {sample} queries, {t3 - t2:.2f} s"}
+               "lookup_G_per_s_1thread": round(2_000_000 / (t5 - t4) / 1e9, 4),
+               "lookup_sample": f"traverse.cs transcription over all {sample} queries on {threads} threads "
+                                f"({t3 - t2:.2f} s) and the first 2,000,000 on 1 thread ({t5 - t4:.2f} s)"}
 
     out = {
         "metric": METRIC,
@@ -205,7 +250,9 @@ def main():
         "lookup_ms_p50": round(pct(lookup_ms, 50), 3),
         "lookup_ms_p99": round(pct(lookup_ms, 99), 3),
         "roofline": roofline,
+        "pyramid_roofline": pyramid_roofline,
         "lookup_roofline": lookup_roofline,
+        "device_copy_GBs": round(copy_gbs, 1),
         "kernel_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in prof.items()},
         "cpu_baseline": cpu,
     }

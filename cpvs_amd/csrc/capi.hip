@@ -685,6 +685,13 @@ int cpvs_memcpy_h2d(cpvs_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
     return CPVS_OK;
 }
 
+int cpvs_memcpy_d2d(cpvs_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream) {
+    CHECK_ARG(ctx && dst && src, "memcpy_d2d: NULL argument");
+    CPVS_HIP_CHECK(hipSetDevice(ctx->device));
+    CPVS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, pick_stream(ctx, stream)));
+    return CPVS_OK;
+}
+
 int cpvs_memcpy_d2h(cpvs_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
     CHECK_ARG(ctx && dst && src, "memcpy_d2h: NULL argument");
     CPVS_HIP_CHECK(hipSetDevice(ctx->device));

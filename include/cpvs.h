@@ -152,6 +152,7 @@ int cpvs_device_alloc(cpvs_ctx *ctx, uint64_t bytes, void **dptr);
 int cpvs_device_free(cpvs_ctx *ctx, void *dptr);
 int cpvs_memcpy_h2d(cpvs_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int cpvs_memcpy_d2h(cpvs_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int cpvs_memcpy_d2d(cpvs_ctx *ctx, void *dst, const void *src, uint64_t bytes, void *stream); /* asynchronous */
 int cpvs_event_record(cpvs_ctx *ctx, void *stream, void **event);
 int cpvs_event_elapsed_ms(void *start, void *stop, float *ms); /* synchronises on stop */
 int cpvs_event_destroy(void *event);

@@ -1,0 +1,126 @@
+// The reference's own unit tests (test/MinMaxTest.cpp, test/CompressedShadowTest.cpp),
+// re-expressed against the C++ facade include/cpvs_amd.hpp.  Reads its cases from a text file
+// written by tests/test_gpu_facade.py from tests/golden/reference_tests.json:
+//   image <name> <N> <N*N floats>
+//   minmax <name> <level> <x> <y> <min|max> <expected> <tolerance>
+//   levels <name> <expected numLevels>
+//   traverse <name> <x> <y> <z> <tryLeafmasks> <expected>
+//   sweep <name> <x0> <x1> <y0> <y1> <z> <expected>
+// Exit status 0 when every case passes.
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+
+#include "cpvs_amd.hpp"
+
+using namespace cpvs_amd;
+
+struct V3 {
+    float x, y, z;
+};
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s cases.txt\n", argv[0]);
+        return 2;
+    }
+    std::ifstream in(argv[1]);
+    std::map<std::string, std::unique_ptr<ImageF>> images;
+    std::map<std::string, std::unique_ptr<MinMaxHierarchy>> pyramids;
+    std::map<std::string, std::unique_ptr<CompressedShadow>> shadows;
+    auto pyr = [&](const std::string& n) -> MinMaxHierarchy& {
+        auto& p = pyramids[n];
+        if (!p) p.reset(new MinMaxHierarchy(*images.at(n)));
+        return *p;
+    };
+    auto dag = [&](const std::string& n) -> CompressedShadow& {
+        auto& s = shadows[n];
+        if (!s) s = CompressedShadow::create(pyr(n));
+        return *s;
+    };
+    int fails = 0, cases = 0;
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        std::string kind, name;
+        ss >> kind >> name;
+        try {
+            if (kind == "image") {
+                size_t N;
+                ss >> N;
+                std::unique_ptr<ImageF> img(new ImageF(N, N, 1));
+                for (size_t i = 0; i < N * N; ++i) ss >> img->data()[i];
+                images[name] = std::move(img);
+            } else if (kind == "levels") {
+                int exp;
+                ss >> exp;
+                ++cases;
+                if (pyr(name).getNumLevels() != exp) {
+                    std::printf("FAIL levels %s\n", name.c_str());
+                    ++fails;
+                }
+            } else if (kind == "minmax") {
+                size_t L, x, y;
+                std::string which;
+                double exp, tol;
+                ss >> L >> x >> y >> which >> exp >> tol;
+                ++cases;
+                const float v = which == "min" ? pyr(name).getMin(L, x, y) : pyr(name).getMax(L, x, y);
+                const bool ok = tol == 0 ? v == (float)exp : std::fabs(v - exp) < tol;
+                if (!ok) {
+                    std::printf("FAIL minmax %s %zu %zu %zu %s: %g\n", name.c_str(), L, x, y, which.c_str(), v);
+                    ++fails;
+                }
+            } else if (kind == "traverse") {
+                V3 p;
+                int tryLeaf, exp;
+                ss >> p.x >> p.y >> p.z >> tryLeaf >> exp;
+                ++cases;
+                if (dag(name).traverse(p, tryLeaf != 0) != exp) {
+                    std::printf("FAIL traverse %s (%g,%g,%g)\n", name.c_str(), p.x, p.y, p.z);
+                    ++fails;
+                }
+            } else if (kind == "sweep") {
+                int x0, x1, y0, y1, exp;
+                float z;
+                ss >> x0 >> x1 >> y0 >> y1 >> z >> exp;
+                for (int y = y0; y < y1; ++y)
+                    for (int x = x0; x < x1; ++x) {
+                        ++cases;
+                        // convertToNdc(vec3(x/32, y/32, z)) = v * 2 - 1 (test/CompressedShadowTest.cpp:46-48)
+                        V3 p{(x / 32.0f) * 2.0f + -1.0f, (y / 32.0f) * 2.0f + -1.0f, z * 2.0f + -1.0f};
+                        if (dag(name).traverse(p) != exp) {
+                            std::printf("FAIL sweep %s %d %d\n", name.c_str(), x, y);
+                            ++fails;
+                        }
+                    }
+            }
+        } catch (const Error& e) {
+            std::printf("ERROR %s %s: %s\n", kind.c_str(), name.c_str(), e.what());
+            ++fails;
+        }
+    }
+    // Container round trip through the facade: total visibility and DAG words survive the move.
+    try {
+        auto& d = dag("depths32x32");
+        const size_t words = d.getDAG().size();
+        std::unique_ptr<CompressedShadow> copy = CompressedShadow::create(pyr("depths32x32"));
+        CompressedShadowContainer c(std::move(copy));
+        c.setFilterSize(3);
+        c.moveToGPU();
+        ++cases;
+        if (c.get(0, 0, 0) != nullptr || words == 0) {
+            std::printf("FAIL container state\n");
+            ++fails;
+        }
+    } catch (const Error& e) {
+        std::printf("ERROR container: %s\n", e.what());
+        ++fails;
+    }
+    std::printf("%d cases, %d failures\n", cases, fails);
+    return fails ? 1 : 0;
+}
