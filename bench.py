@@ -68,6 +68,65 @@ def load_traffic(N):
     return out
 
 
+def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26):
+    """The tiled configuration: full^2 depth as (full/tile)^2 xy tiles, each built into full/tile
+    z-tile DAGs (DeferredRenderer.cpp:150-187), one container with the top-level grid, and nq
+    uniform lookups.  All 16 GiB of depth are generated on the device before timing."""
+    L = full // tile
+    stream = ctx.stream
+    log(f"[bench] tiled: generating {L * L} tiles of {tile}^2")
+    tiles = {}
+    for ty in range(L):
+        for tx in range(L):
+            tiles[(tx, ty)] = cp.gen_depth(full, SEED0 + 2, boxes, True, tx * tile, ty * tile, tile)
+    q = cp.gen_queries_uniform(SEED0 + 3, nq, 4)
+    vis = cp.DeviceArray((nq,), np.uint8, ctx)
+    c = cp.CompressedShadowContainer(L)
+    per_dag = []
+    t0 = time.perf_counter()
+    for ty in range(L):
+        for tx in range(L):
+            e0 = ctx.event(stream)
+            mm = cp.MinMaxHierarchy(tiles[(tx, ty)], borrow=True)
+            e1 = ctx.event(stream)
+            pyr_ms = ctx.elapsed_ms(e0, e1)
+            for tz in range(L):
+                a = ctx.event(stream)
+                c.set(cp.CompressedShadow.create(mm, tz, L), tx, ty, tz)
+                b = ctx.event(stream)
+                per_dag.append(ctx.elapsed_ms(a, b) + pyr_ms / L)
+                ctx.destroy_event(a)
+                ctx.destroy_event(b)
+            ctx.destroy_event(e0)
+            ctx.destroy_event(e1)
+            mm.close()
+    ctx.synchronize()
+    build_s = time.perf_counter() - t0
+    c.moveToGPU()
+    dl, gl, words = c.info()
+    for d in tiles.values():
+        d.free()
+    c.evaluate(q, None, vis)
+    ctx.synchronize()
+    lk = []
+    for _ in range(3):
+        a = ctx.event(stream)
+        c.evaluate(q, None, vis)
+        b = ctx.event(stream)
+        lk.append(ctx.elapsed_ms(a, b))
+        ctx.destroy_event(a)
+        ctx.destroy_event(b)
+    c.close()
+    q.free()
+    vis.free()
+    return {"workload": f"{full}x{full} depth ({L}x{L} tiles of {tile}^2, {boxes} boxes + noise), "
+                        f"{L}^3 = {L ** 3} z-tile DAGs, {nq} uniform lookups",
+            "dags": L ** 3, "dag_words": words, "grid_levels": gl,
+            "build_s": round(build_s, 4), "Mtexels_per_s": round(full * full / build_s / 1e6, 1),
+            "dag_build_ms_p50": round(pct(per_dag, 50), 3), "dag_build_ms_p99": round(pct(per_dag, 99), 3),
+            "lookup_G_per_s": round(nq / (min(lk) * 1e-3) / 1e9, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +137,7 @@ def main():
     ap.add_argument("--width", type=int, default=3840)
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tiled", action="store_true", help="skip the 65536^2 tiled configuration")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -199,6 +259,11 @@ def main():
     lookup_roofline = roof("evaluate4")
     lookup_roofline["bound"] = "latency (dependent loads; DAG cache-resident)"
 
+    tiled = None
+    if not args.no_tiled:
+        tiled = run_tiled(cp, np, ctx)
+        log(f"[bench] tiled: {tiled}")
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle as O
@@ -254,6 +319,7 @@ def main():
         "lookup_roofline": lookup_roofline,
         "device_copy_GBs": round(copy_gbs, 1),
         "kernel_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in prof.items()},
+        "tiled_65536": tiled,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

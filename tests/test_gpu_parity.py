@@ -358,3 +358,50 @@ def test_full_size_c1_properties(ctx):
     bf = O.brute_visible(D, 1, p[:, 0], p[:, 1], p[:, 2])
     assert np.array_equal(vis, bf)
     assert 0.2 < vis.mean() < 0.8
+
+
+# --- tiled 65536^2 (8x8 xy tiles of 8192^2 x 8 z-tiles = 512 DAGs) ----------------------------
+
+def test_tiled_65536_container_properties(ctx):
+    """The 65536^2 configuration (DeferredRenderer.cpp:150-187): per-tile device generation, 512
+    z-tile DAGs in an 8^3 grid, and lookups equal to the brute-force depth compare on 2 tiles."""
+    full, tile, L = 65536, 8192, 8
+    seed, boxes = SEED0 + 2, 16384
+    c = cp.CompressedShadowContainer(L)
+    kept = {}
+    for ty in range(L):
+        for tx in range(L):
+            d = cp.gen_depth(full, seed, boxes, True, tx * tile, ty * tile, tile)
+            mm = cp.MinMaxHierarchy(d, borrow=True)
+            for tz in range(L):
+                c.set(cp.CompressedShadow.create(mm, tz, L), tx, ty, tz)
+            if (tx, ty) in ((0, 0), (5, 3)):
+                kept[(tx, ty)] = d.numpy()
+            mm.close()
+            d.free()
+        _say("tiled row", ty)
+    c.moveToGPU()
+    dl, gl, nw = c.info()
+    assert (dl, gl) == (14, 3)
+    words, grid = c.words_and_grid()
+    assert (grid == 0xFFFFFFFF).any()  # fully shadowed z-tiles use the sentinel the lookup tests
+    rng = np.random.default_rng(5)
+    R = np.float32(full - 1)
+    for (tx, ty), D in kept.items():
+        n = 200000
+        px = rng.integers(tx * tile, (tx + 1) * tile, n)
+        py = rng.integers(ty * tile, (ty + 1) * tile, n)
+        # z near the surface so that both outcomes occur
+        z = np.clip(D[py - ty * tile, px - tx * tile] + rng.uniform(-0.002, 0.002, n).astype(np.float32), 0, 1)
+        q = np.stack([(px + 0.5) / full, (py + 0.5) / full, z, np.ones(n)], 1).astype(np.float32)
+        q[:, :3] = q[:, :3] * np.float32(2) - np.float32(1)
+        vis = c.evaluate(cp.DeviceArray.from_numpy(q)).numpy()
+        p = (((q[:, :3] + np.float32(1)) * np.float32(0.5)) * R).astype(np.int64)
+        inside = (p[:, 0] // tile == tx) & (p[:, 1] // tile == ty)  # quantisation can step off the tile edge
+        assert inside.mean() > 0.99
+        p, v = p[inside], vis[inside]
+        # z resolution of a tile DAG is tile * L = full; brute force at leaf resolution
+        mid = (p[:, 2].astype(np.float32) + (p[:, 2] + 1).astype(np.float32)) * np.float32(0.5)
+        bf = (mid <= D[p[:, 1] - ty * tile, p[:, 0] - tx * tile] * np.float32(full)).astype(np.uint8)
+        assert np.array_equal(v, bf), (tx, ty)
+        assert 0.05 < v.mean() < 0.95
