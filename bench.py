@@ -14,6 +14,7 @@ the lookup rate, p50/p99 latencies, the dominant kernel's roofline and the CPU b
 oracle timed on this machine's host cores) ride along in the same line.
 """
 import argparse
+import ctypes
 import json
 import os
 import statistics
@@ -127,6 +128,150 @@ def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26):
             "lookup_G_per_s": round(nq / (min(lk) * 1e-3) / 1e9, 3)}
 
 
+def splitmix64(seed):
+    """splitmix64 stream (the generator's PRNG): yields 64-bit ints."""
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    M = 0xFFFFFFFFFFFFFFFF
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & M
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        yield z ^ (z >> 31)
+
+
+def run_batched(cp, np, ctx, maps=256, kmin=8, kmax=13, qper=1 << 20, reps=3):
+    """Many maps: `maps` depth maps of side 2^k, k uniform in [kmin, kmax] (boxes 32*(N/1024)^2,
+    at least 1, plus noise; map i seeded SEED0 + 3 + 1000*(i+1)), built plainly (one
+    MinMaxHierarchy + create chain after another on one stream, as the reference's loop) and as one
+    batch (cpvs_build_batch, lane count chosen by cpvs_tune_batch).  Then maps*qper uniform
+    lookups: plainly one launch per DAG over its own qper queries, and batched as one launch
+    over all of them interleaved round-robin (query j -> DAG j mod maps) through a DAG set.
+    Plain lookups read each DAG's queries from the interleaved buffer (stride maps*3 floats); the
+    same on a map-major copy made before timing is reported beside them.  Every map arrives at
+    t=0; its latency is its completion time."""
+    seed = SEED0 + 3
+    rng = splitmix64(seed)
+    sizes = [1 << (kmin + next(rng) % (kmax - kmin + 1)) for _ in range(maps)]
+    log(f"[bench] batched: generating {maps} maps (sizes {sorted(set(sizes))})")
+    depths = []
+    for i, N in enumerate(sizes):
+        depths.append(cp.gen_depth(N, seed + 1000 * (i + 1), max(1, (32 * N * N) // (1024 * 1024)), True))
+    ctx.synchronize()
+    texels = float(sum(N * N for N in sizes))
+
+    def plain_once():
+        out, done = [], []
+        t0 = time.perf_counter()
+        for d in depths:
+            mm = cp.MinMaxHierarchy(d, borrow=True)
+            out.append(cp.CompressedShadow.create(mm))
+            mm.close()
+            done.append((time.perf_counter() - t0) * 1e3)
+        ctx.synchronize()
+        return out, done, time.perf_counter() - t0
+
+    def batch_once():
+        t0 = time.perf_counter()
+        out, done = cp.build_batch(depths)
+        return out, list(done), time.perf_counter() - t0
+
+    # warm-up both paths (arena sizing, lane creation), then the tuner picks the lane count
+    for sh in plain_once()[0]:
+        sh.close()
+    best, tune_ms = cp.tune_batch(depths, candidates=(1, 2, 3, 4, 6, 8))
+    log(f"[bench] batched: tuner picked {best} lanes ({tune_ms})")
+    plain_runs, batch_runs = [], []
+    plain, batch = None, None
+    for r in range(reps):
+        sh, done, sec = plain_once()
+        plain_runs.append((sec, done))
+        if r < reps - 1:
+            for x in sh:
+                x.close()
+        else:
+            plain = sh
+        sh, done, sec = batch_once()
+        batch_runs.append((sec, done))
+        if r < reps - 1:
+            for x in sh:
+                x.close()
+        else:
+            batch = sh
+        log(f"[bench] batched: rep {r}: plain {plain_runs[-1][0] * 1e3:.1f} ms, batch {sec * 1e3:.1f} ms")
+    p_sec, p_done = sorted(plain_runs)[len(plain_runs) // 2]
+    b_sec, b_done = sorted(batch_runs)[len(batch_runs) // 2]
+    words_plain = [x.num_words() for x in plain]
+    words_batch = [x.num_words() for x in batch]
+    same = words_plain == words_batch
+    small = sorted(range(maps), key=lambda i: sizes[i])[:32]
+    for i in small:
+        same = same and bool(np.array_equal(plain[i].getDAG(), batch[i].getDAG()))
+    for x in plain:
+        x.close()
+    for d in depths:
+        d.free()
+
+    # lookups: the same maps*qper points, interleaved round-robin (query j -> DAG j mod maps)
+    nq = maps * qper
+    stream = ctx.stream
+    q_rr = cp.gen_queries_uniform(seed + 17, nq, 3)
+    vis_rr = cp.DeviceArray((nq,), np.uint8, ctx)
+    vis_st = cp.DeviceArray((nq,), np.uint8, ctx)
+    q_mm = cp.DeviceArray((nq, 3), np.float32, ctx)
+    for i in range(maps):  # map-major copy of the same points: block i = points i, i+maps, ...
+        cp._lib.call("cpvs_gen_queries_uniform_range", ctx.handle, ctypes.c_void_p(q_mm.ptr.value + 12 * qper * i),
+                     seed + 17, i, maps, qper, 3, None)
+    vis_mm = cp.DeviceArray((nq,), np.uint8, ctx)
+    dset = cp.DagSet(batch)
+    lk = {"plain": [], "plain_grouped": [], "batched": []}
+
+    def plain_pass(q, qstride, qstep, out):
+        for i, sh in enumerate(batch):
+            cp._lib.call("cpvs_dag_evaluate", sh.handle, ctypes.c_void_p(q.ptr.value + qstep * i), qstride, qper,
+                         ctypes.c_void_p(out.ptr.value + qper * i), stream)
+
+    for r in range(reps + 1):
+        ev = [ctx.event(stream)]
+        plain_pass(q_rr, 3 * maps, 12, vis_st)  # one launch per DAG over its strided queries
+        ev.append(ctx.event(stream))
+        plain_pass(q_mm, 3, 12 * qper, vis_mm)  # the same on a pre-grouped copy
+        ev.append(ctx.event(stream))
+        dset.evaluate(q_rr, None, vis_rr, stream)  # one launch over all, DAG set round-robin
+        ev.append(ctx.event(stream))
+        if r:
+            lk["plain"].append(ctx.elapsed_ms(ev[0], ev[1]))
+            lk["plain_grouped"].append(ctx.elapsed_ms(ev[1], ev[2]))
+            lk["batched"].append(ctx.elapsed_ms(ev[2], ev[3]))
+        for e in ev:
+            ctx.destroy_event(e)
+    vr = vis_rr.numpy().reshape(qper, maps).T
+    same_lookup = bool(np.array_equal(vr, vis_mm.numpy().reshape(maps, qper))) and \
+        bool(np.array_equal(vr, vis_st.numpy().reshape(maps, qper)))
+    dset.close()
+    for x in batch:
+        x.close()
+    for a in (q_rr, vis_rr, vis_st, q_mm, vis_mm):
+        a.free()
+    rate = {k: nq / (min(v) * 1e-3) / 1e9 for k, v in lk.items()}
+    return {"workload": f"{maps} maps of 2^k x 2^k, k uniform in [{kmin},{kmax}] (boxes 32*(N/1024)^2 + noise), "
+                        f"{nq} uniform lookups ({qper} per map, interleaved round-robin)",
+            "maps": maps, "texels": int(texels), "lanes": best,
+            "tune_ms": {str(k): round(v, 2) for k, v in (tune_ms or {}).items()},
+            "plain_build_ms": round(p_sec * 1e3, 2), "batched_build_ms": round(b_sec * 1e3, 2),
+            "plain_Mtexels_per_s": round(texels / p_sec / 1e6, 1),
+            "batched_Mtexels_per_s": round(texels / b_sec / 1e6, 1),
+            "build_speedup": round(p_sec / b_sec, 2),
+            "plain_latency_ms_p50": round(pct(p_done, 50), 2), "plain_latency_ms_p99": round(pct(p_done, 99), 2),
+            "batched_latency_ms_p50": round(pct(b_done, 50), 2), "batched_latency_ms_p99": round(pct(b_done, 99), 2),
+            "dag_words": int(sum(words_batch)), "words_identical": bool(same),
+            "lookup_plain_G_per_s": round(rate["plain"], 3),
+            "lookup_plain_pregrouped_G_per_s": round(rate["plain_grouped"], 3),
+            "lookup_batched_G_per_s": round(rate["batched"], 3),
+            "lookup_speedup": round(rate["batched"] / rate["plain"], 3),
+            "lookups_identical": same_lookup}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +283,7 @@ def main():
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tiled", action="store_true", help="skip the 65536^2 tiled configuration")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-map batched configuration")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -264,6 +410,11 @@ def main():
         tiled = run_tiled(cp, np, ctx)
         log(f"[bench] tiled: {tiled}")
 
+    batched = None
+    if not args.no_batched:
+        batched = run_batched(cp, np, ctx)
+        log(f"[bench] batched: {batched}")
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle as O
@@ -320,6 +471,7 @@ def main():
         "device_copy_GBs": round(copy_gbs, 1),
         "kernel_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in prof.items()},
         "tiled_65536": tiled,
+        "batched_256_maps": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -14,7 +14,9 @@ Every computation runs as HIP kernels on the GPU; DAGs stay in device memory.
 from __future__ import annotations
 
 import ctypes as C
-from typing import Optional
+import json
+import os
+from typing import Optional, Sequence
 
 import numpy as np
 
@@ -22,7 +24,8 @@ from . import _lib
 from ._lib import CpvsError, call
 
 __all__ = ["Context", "DeviceArray", "MinMaxHierarchy", "CompressedShadow", "CompressedShadowContainer",
-           "CpvsError", "default_context", "gen_depth", "gen_queries_uniform", "gen_queries_gbuffer"]
+           "DagSet", "CpvsError", "default_context", "build_batch", "tune_batch", "gen_depth",
+           "gen_queries_uniform", "gen_queries_uniform_range", "gen_queries_gbuffer"]
 
 
 class Context:
@@ -56,6 +59,19 @@ class Context:
             name, ms, cnt = line.split("\t")
             out[name] = (float(ms), int(cnt))
         return out
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        call("cpvs_ctx_device_name", self.handle, buf, len(buf))
+        return buf.value.decode()
+
+    def batch_lanes(self) -> int:
+        v = C.c_uint32()
+        call("cpvs_ctx_batch_lanes", self.handle, C.byref(v))
+        return int(v.value)
+
+    def set_batch_lanes(self, lanes: int):
+        call("cpvs_ctx_set_batch_lanes", self.handle, lanes)
 
     def set_merge(self, mode: str):
         """'table' (default) or 'sort': how builds merge common subtrees (same words)."""
@@ -264,6 +280,18 @@ class CompressedShadow:
         """CompressedShadow::traverse(vec3 ndc, bool tryLeafmasks) (src/CompressedShadow.cpp:404-463)."""
         return int(self.traverse_many(np.asarray(position, np.float32).reshape(1, 3), tryLeafmasks)[0])
 
+    def evaluate(self, ndc: DeviceArray, visibilities: Optional[DeviceArray] = None, count: Optional[int] = None,
+                 offset: int = 0, stride: Optional[int] = None, stream=None) -> DeviceArray:
+        """Device lookup of NDC points as a one-DAG container with an identity light matrix
+        (src/CompressedShadowContainer.h:27-32 + shader/traverse.cs); 0/1 per point."""
+        st = ndc.shape[1] if stride is None else stride
+        n = (ndc.shape[0] - offset) if count is None else count
+        if visibilities is None:
+            visibilities = DeviceArray((n,), np.uint8, self.ctx)
+        base = (ndc.ptr.value or 0) + 4 * offset * (ndc.shape[1] if len(ndc.shape) > 1 else 1)
+        call("cpvs_dag_evaluate", self._h(), C.c_void_p(base), st, n, visibilities.ptr, stream)
+        return visibilities
+
     def _release(self):
         h, self.handle = self.handle, None
         return h
@@ -375,6 +403,105 @@ class CompressedShadowContainer:
             pass
 
 
+class DagSet:
+    """Many DAGs queried in one launch (cpvs_dagset): query i uses DAG ids[i], or i mod len(set)."""
+
+    def __init__(self, shadows: Sequence[CompressedShadow], ctx: Optional[Context] = None):
+        self.ctx = ctx or (shadows[0].ctx if len(shadows) else default_context())
+        self.handle = None
+        self.shadows = list(shadows)  # kept alive: the set borrows their words
+        arr = (C.c_void_p * len(self.shadows))(*[sh._h() for sh in self.shadows])
+        h = C.c_void_p()
+        call("cpvs_dagset_create", self.ctx.handle, arr, len(self.shadows), C.byref(h))
+        self.handle = h
+
+    def __len__(self):
+        return len(self.shadows)
+
+    def evaluate(self, ndc: DeviceArray, ids: Optional[DeviceArray] = None, visibilities: Optional[DeviceArray] = None,
+                 stream=None) -> DeviceArray:
+        n = ndc.shape[0]
+        if visibilities is None:
+            visibilities = DeviceArray((n,), np.uint8, self.ctx)
+        call("cpvs_dagset_evaluate", self.handle, ndc.ptr, ndc.shape[1], n, ids.ptr if ids is not None else None,
+             visibilities.ptr, stream)
+        return visibilities
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().cpvs_dagset_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _batch_items(depths: Sequence[DeviceArray], tiles=None):
+    items = (_lib.BatchItem * len(depths))()
+    for i, d in enumerate(depths):
+        t, T = (0, 1) if tiles is None else tiles[i]
+        items[i] = _lib.BatchItem(d.ptr.value, d.shape[0], t, T, 0)
+    return items
+
+
+def build_batch(depths: Sequence[DeviceArray], tiles=None, lanes: int = 0, ctx: Optional[Context] = None):
+    """MinMaxHierarchy + CompressedShadow::create for every depth map at once (cpvs_build_batch).
+
+    depths: square float32 DeviceArrays; tiles: optional (zTileIndex, zTileNum) per map; lanes: 0 =
+    the context's tuned count.  Returns (shadows, done_ms) with done_ms[i] the completion time of
+    map i in ms since the call began."""
+    ctx = ctx or default_context()
+    n = len(depths)
+    items = _batch_items(depths, tiles)
+    out = (C.c_void_p * max(n, 1))()
+    done = np.zeros(max(n, 1), np.float64)
+    call("cpvs_build_batch", ctx.handle, items, n, lanes, out, done.ctypes.data_as(C.c_void_p))
+    return [CompressedShadow(C.c_void_p(out[i]), ctx) for i in range(n)], done[:n]
+
+
+def _tune_key(ctx: Context, depths) -> str:
+    import collections
+    hist = collections.Counter(int(d.shape[0]) for d in depths)
+    return ctx.device_name() + "|" + ",".join(f"{k}x{hist[k]}" for k in sorted(hist))
+
+
+def tune_batch(depths: Sequence[DeviceArray], tiles=None, candidates=(1, 2, 3, 4, 6, 8), reps: int = 1,
+               cache: Optional[str] = None, ctx: Optional[Context] = None):
+    """Pick the batch lane count for this mix of map sizes (cpvs_tune_batch) and make it the
+    context's default.  With cache (a JSON file path), a result stored for the same device and
+    size mix is reused without timing.  Returns (best, {lanes: ms} or None when cached)."""
+    ctx = ctx or default_context()
+    key = _tune_key(ctx, depths)
+    table = {}
+    if cache and os.path.exists(cache):
+        try:
+            with open(cache) as f:
+                table = json.load(f)
+        except (OSError, ValueError):
+            table = {}
+        if key in table:
+            best = int(table[key]["lanes"])
+            call("cpvs_ctx_set_batch_lanes", ctx.handle, best)
+            return best, None
+    items = _batch_items(depths, tiles)
+    cand = (C.c_uint32 * len(candidates))(*candidates)
+    ms = np.zeros(len(candidates), np.float64)
+    best = C.c_uint32()
+    call("cpvs_tune_batch", ctx.handle, items, len(depths), cand, len(candidates), reps, C.byref(best),
+         ms.ctypes.data_as(C.c_void_p))
+    timings = {int(c): float(m) for c, m in zip(candidates, ms)}
+    if cache:
+        table[key] = {"lanes": int(best.value), "ms": timings}
+        tmp = cache + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, cache)
+    return int(best.value), timings
+
+
 def gen_depth(N: int, seed: int, nboxes: int, noise: bool, x0: int = 0, y0: int = 0, tileN: Optional[int] = None,
               ctx: Optional[Context] = None) -> DeviceArray:
     """Synthetic depth tile [x0, x0+tileN)^2 of an N x N map, generated on the device."""
@@ -389,6 +516,16 @@ def gen_queries_uniform(seed: int, count: int, stride: int = 4, ctx: Optional[Co
     ctx = ctx or default_context()
     out = DeviceArray((count, stride), np.float32, ctx)
     call("cpvs_gen_queries_uniform", ctx.handle, out.ptr, seed, count, stride, None)
+    ctx.synchronize()
+    return out
+
+
+def gen_queries_uniform_range(seed: int, first: int, step: int, count: int, stride: int = 3,
+                              ctx: Optional[Context] = None) -> DeviceArray:
+    """Points first, first+step, ... of the gen_queries_uniform stream."""
+    ctx = ctx or default_context()
+    out = DeviceArray((count, stride), np.float32, ctx)
+    call("cpvs_gen_queries_uniform_range", ctx.handle, out.ptr, seed, first, step, count, stride, None)
     ctx.synchronize()
     return out
 

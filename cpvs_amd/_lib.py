@@ -74,7 +74,23 @@ _SIGS = {
     "cpvs_gen_depth": [vp, vp, u32, u64, u32, i32, u32, u32, u32, vp],
     "cpvs_gen_queries_uniform": [vp, vp, u64, u64, u32, vp],
     "cpvs_gen_queries_gbuffer": [vp, vp, vp, u32, u64, u32, u32, vp],
+    "cpvs_gen_queries_uniform_range": [vp, vp, u64, u64, u64, u64, u32, vp],
+    "cpvs_build_batch": [vp, vp, u32, u32, vp, vp],
+    "cpvs_ctx_set_batch_lanes": [vp, u32],
+    "cpvs_ctx_batch_lanes": [vp, C.POINTER(u32)],
+    "cpvs_ctx_device_name": [vp, C.c_char_p, u64],
+    "cpvs_tune_batch": [vp, vp, u32, vp, u32, u32, C.POINTER(u32), vp],
+    "cpvs_dagset_create": [vp, vp, u32, PP],
+    "cpvs_dagset_evaluate": [vp, vp, u32, u64, vp, vp, vp],
+    "cpvs_dagset_destroy": [vp],
+    "cpvs_dag_evaluate": [vp, vp, u32, u64, vp, vp],
 }
+
+
+class BatchItem(C.Structure):
+    """cpvs_batch_item: one map of a batch build (device depth, N, zTileIndex, zTileNum)."""
+    _fields_ = [("depth", C.c_void_p), ("N", C.c_uint32), ("zTileIndex", C.c_uint32), ("zTileNum", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 _lib = None
 

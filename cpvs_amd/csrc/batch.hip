@@ -1,0 +1,358 @@
+// batch.hip -- many maps at once: concurrent build lanes, the lane-count tuner, and DAG sets
+// (many DAGs queried in one launch).
+//
+// The reference builds maps one after another (src/DeferredRenderer.cpp:150-187, one
+// CompressedShadow::create per tile and z-tile).  One build is a chain of ~60 short launches
+// with three host readbacks; small maps leave the GPU idle between them.  A batch runs L lanes,
+// each a host thread with its own stream, scratch arena, pyramid-level arena and pinned staging,
+// pulling maps from one queue in decreasing size (so a lane's arenas grow on its first map and
+// are reused after).  Every map goes through the same kernels as cpvs_build, so the DAG words
+// are those of a plain build.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "abi.hpp"
+
+namespace cpvs {
+
+int Worker::create() {
+    CPVS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&pinned), sizeof(u64) * kPinnedWords, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(stream);
+        stream = nullptr;
+        pinned = nullptr;
+        return hip_fail(e, "hipHostMalloc", __FILE__, __LINE__);
+    }
+    return CPVS_OK;
+}
+
+void Worker::destroy() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    arena.release();
+    levels.release();
+    if (pinned) (void)hipHostFree(pinned);
+    if (stream) (void)hipStreamDestroy(stream);
+    pinned = nullptr;
+    stream = nullptr;
+}
+
+namespace {
+
+constexpr uint32_t kMaxLanes = 16;
+constexpr uint32_t kDefaultLanes = 4;
+
+int levels_of(u32 N) {
+    int n = 1;
+    while ((1u << (n - 1)) < N) ++n;
+    return n;
+}
+
+int check_item(const cpvs_batch_item& it, uint32_t i) {
+    CHECK_ARG(it.depth, "batch item %u: NULL depth", i);
+    CHECK_ARG(is_pow2(it.N) && it.N >= 8 && it.N <= 65536, "batch item %u: N=%u must be a power of two in [8, 65536]", i,
+              it.N);
+    CHECK_ARG(it.zTileNum >= 1 && it.zTileIndex < it.zTileNum, "batch item %u: zTileIndex %u must be < zTileNum %u", i,
+              it.zTileIndex, it.zTileNum);
+    CHECK_ARG((uint64_t)it.N * it.zTileNum <= (1ull << 21), "batch item %u: N*zTileNum exceeds 2^21", i);
+    return CPVS_OK;
+}
+
+// MinMaxHierarchy + CompressedShadow::create of one item on lane w (levels in the lane's arena).
+int build_item(cpvs_ctx* ctx, Worker& w, const cpvs_batch_item& it, int merge_mode, cpvs_dag** out) {
+    cpvs_pyramid p;
+    p.ctx = ctx;
+    p.N = it.N;
+    p.n = levels_of(it.N);
+    p.depth = const_cast<float*>(it.depth);
+    p.borrowed = true;
+    int rc = pyramid_build_levels(&p, false, w.stream, &w.levels);
+    if (rc) return rc;
+    return build_dag(ctx, w.arena, w.pinned, merge_mode, &p, it.zTileIndex, it.zTileNum, w.stream, out);
+}
+
+int ensure_lanes(cpvs_ctx* ctx, uint32_t lanes) {
+    while (ctx->lanes.size() < lanes) {
+        Worker* w = new Worker();
+        const int rc = w->create();
+        if (rc) {
+            delete w;
+            return rc;
+        }
+        ctx->lanes.push_back(w);
+    }
+    return CPVS_OK;
+}
+
+// Runs the batch on `lanes` lanes; the calling thread is lane 0.  Caller holds ctx->mu.
+int run_batch(cpvs_ctx* ctx, const cpvs_batch_item* items, uint32_t count, uint32_t lanes, cpvs_dag** out,
+              double* done_ms) {
+    for (uint32_t i = 0; i < count; ++i) out[i] = nullptr;
+    if (!count) return CPVS_OK;
+    int rc = ensure_lanes(ctx, lanes);
+    if (rc) return rc;
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return (u64)items[a].N * items[a].N > (u64)items[b].N * items[b].N;
+    });
+    std::atomic<uint32_t> next{0};
+    std::atomic<bool> failed{false};
+    std::mutex emu;
+    int first_rc = CPVS_OK;
+    std::string first_msg;
+    const int merge_mode = ctx->merge_mode;
+    const auto t0 = std::chrono::steady_clock::now();
+
+    auto record = [&](int lrc, const char* what) {
+        std::lock_guard<std::mutex> lk(emu);
+        if (first_rc == CPVS_OK) {
+            first_rc = lrc;
+            first_msg = what;
+        }
+        failed.store(true);
+    };
+    auto body = [&](Worker* w) {
+        if (hipSetDevice(ctx->device) != hipSuccess) {
+            record(CPVS_EHIP, "batch: hipSetDevice failed on a lane thread");
+            return;
+        }
+        while (!failed.load(std::memory_order_relaxed)) {
+            const uint32_t j = next.fetch_add(1, std::memory_order_relaxed);
+            if (j >= count) break;
+            const uint32_t i = order[j];
+            cpvs_dag* d = nullptr;
+            int lrc;
+            try {
+                lrc = build_item(ctx, *w, items[i], merge_mode, &d);
+            } catch (const std::bad_alloc&) {
+                set_error("host allocation failed");
+                lrc = CPVS_ENOMEM;
+            } catch (...) {
+                set_error("internal exception");
+                lrc = CPVS_EINTERNAL;
+            }
+            if (lrc != CPVS_OK) {
+                char msg[600];
+                snprintf(msg, sizeof(msg), "batch item %u: %s", i, cpvs_last_error());
+                record(lrc, msg);
+                break;
+            }
+            out[i] = d;
+            if (done_ms)
+                done_ms[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        (void)hipStreamSynchronize(w->stream);
+    };
+
+    std::vector<std::thread> threads;
+    threads.reserve(lanes);
+    try {
+        for (uint32_t k = 1; k < lanes; ++k) threads.emplace_back(body, ctx->lanes[k]);
+    } catch (...) {
+        failed.store(true);
+        for (auto& t : threads) t.join();
+        set_error("batch: could not start %u lane threads", lanes - 1);
+        rc = CPVS_EINTERNAL;
+        for (uint32_t i = 0; i < count; ++i) {
+            cpvs_dag_destroy(out[i]);
+            out[i] = nullptr;
+        }
+        return rc;
+    }
+    body(ctx->lanes[0]);
+    for (auto& t : threads) t.join();
+    if (first_rc != CPVS_OK) {
+        for (uint32_t i = 0; i < count; ++i) {
+            cpvs_dag_destroy(out[i]);
+            out[i] = nullptr;
+        }
+        set_error("%s", first_msg.c_str());
+        return first_rc;
+    }
+    return CPVS_OK;
+}
+
+uint32_t pick_lanes(const cpvs_ctx* ctx, uint32_t lanes) {
+    if (lanes) return lanes;
+    return ctx->batch_workers ? ctx->batch_workers : kDefaultLanes;
+}
+
+DagDesc desc_of(const cpvs_dag* d) {
+    DagDesc r{};
+    r.words = d->d_words;
+    r.nwords = (u32)d->words;
+    // createTopLevelGrid (src/CompressedShadowContainer.cpp:71-91) sentinels for a one-cell grid.
+    r.base = (d->root_mask == 0u) ? 0xFFFFFFFFu : (d->root_mask == 0x5555u) ? 0xFFFFFFFEu : 0u;
+    r.dl = (u32)d->n;
+    return r;
+}
+
+}  // namespace
+}  // namespace cpvs
+
+using namespace cpvs;
+
+struct cpvs_dagset {
+    cpvs_ctx* ctx = nullptr;
+    uint32_t count = 0;
+    DagDesc* d_desc = nullptr;
+};
+
+extern "C" {
+
+int cpvs_build_batch(cpvs_ctx* ctx, const cpvs_batch_item* items, uint32_t count, uint32_t lanes, cpvs_dag** out,
+                     double* done_ms) {
+    CHECK_ARG(ctx && (items || !count) && (out || !count), "build_batch: NULL argument");
+    CHECK_ARG(lanes <= kMaxLanes, "build_batch: %u lanes (at most %u)", lanes, kMaxLanes);
+    for (uint32_t i = 0; i < count; ++i) {
+        const int rc = check_item(items[i], i);
+        if (rc) return rc;
+    }
+    GUARD_BEGIN
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CPVS_HIP_CHECK(hipSetDevice(ctx->device));
+    // Work already queued on the context stream (e.g. the depth generator) comes first.
+    CPVS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return run_batch(ctx, items, count, pick_lanes(ctx, lanes), out, done_ms);
+    GUARD_END
+}
+
+int cpvs_ctx_set_batch_lanes(cpvs_ctx* ctx, uint32_t lanes) {
+    CHECK_ARG(ctx, "ctx_set_batch_lanes: NULL ctx");
+    CHECK_ARG(lanes <= kMaxLanes, "ctx_set_batch_lanes: %u lanes (at most %u)", lanes, kMaxLanes);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->batch_workers = lanes;
+    return CPVS_OK;
+}
+
+int cpvs_ctx_device_name(cpvs_ctx* ctx, char* buf, uint64_t cap) {
+    CHECK_ARG(ctx && buf && cap, "ctx_device_name: NULL argument");
+    hipDeviceProp_t prop;
+    CPVS_HIP_CHECK(hipGetDeviceProperties(&prop, ctx->device));
+    snprintf(buf, cap, "%s %s %dCU", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return CPVS_OK;
+}
+
+int cpvs_ctx_batch_lanes(cpvs_ctx* ctx, uint32_t* lanes) {
+    CHECK_ARG(ctx && lanes, "ctx_batch_lanes: NULL argument");
+    *lanes = pick_lanes(ctx, 0);
+    return CPVS_OK;
+}
+
+int cpvs_tune_batch(cpvs_ctx* ctx, const cpvs_batch_item* items, uint32_t count, const uint32_t* candidates,
+                    uint32_t ncand, uint32_t reps, uint32_t* best, double* ms) {
+    CHECK_ARG(ctx && items && count && candidates && ncand && best, "tune_batch: NULL or empty argument");
+    CHECK_ARG(reps >= 1 && reps <= 100, "tune_batch: reps %u must be in [1, 100]", reps);
+    uint32_t maxl = 0;
+    for (uint32_t k = 0; k < ncand; ++k) {
+        CHECK_ARG(candidates[k] >= 1 && candidates[k] <= kMaxLanes, "tune_batch: candidate %u lanes", candidates[k]);
+        maxl = std::max(maxl, candidates[k]);
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        const int rc = check_item(items[i], i);
+        if (rc) return rc;
+    }
+    GUARD_BEGIN
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CPVS_HIP_CHECK(hipSetDevice(ctx->device));
+    CPVS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::vector<cpvs_dag*> out(count, nullptr);
+    auto destroy_all = [&]() {
+        for (auto& d : out) {
+            cpvs_dag_destroy(d);
+            d = nullptr;
+        }
+    };
+    // Warm-up with the widest candidate: creates the lanes and sizes their arenas.
+    int rc = run_batch(ctx, items, count, maxl, out.data(), nullptr);
+    destroy_all();
+    if (rc) return rc;
+    double best_ms = 0.0;
+    uint32_t best_l = candidates[0];
+    for (uint32_t k = 0; k < ncand; ++k) {
+        double m = 0.0;
+        for (uint32_t r = 0; r < reps; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = run_batch(ctx, items, count, candidates[k], out.data(), nullptr);
+            const double e = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            destroy_all();
+            if (rc) return rc;
+            m = (r == 0) ? e : std::min(m, e);
+        }
+        if (ms) ms[k] = m;
+        if (k == 0 || m < best_ms) {
+            best_ms = m;
+            best_l = candidates[k];
+        }
+    }
+    ctx->batch_workers = best_l;
+    *best = best_l;
+    return CPVS_OK;
+    GUARD_END
+}
+
+int cpvs_dagset_create(cpvs_ctx* ctx, const cpvs_dag* const* dags, uint32_t count, cpvs_dagset** out) {
+    CHECK_ARG(ctx && dags && count && out, "dagset_create: NULL or empty argument");
+    std::vector<DagDesc> h(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const cpvs_dag* d = dags[i];
+        CHECK_ARG(d, "dagset_create: DAG %u is NULL", i);
+        CHECK_ARG(d->ctx == ctx, "dagset_create: DAG %u belongs to another context", i);
+        CHECK_ARG(d->n >= 5, "dagset_create: DAG %u has %d levels; the lookup walks leafmask DAGs (>= 5)", i, d->n);
+        CHECK_ARG(d->words < 0xFFFFFFFEull, "dagset_create: DAG %u exceeds 32-bit offsets", i);
+        h[i] = desc_of(d);
+    }
+    GUARD_BEGIN
+    CPVS_HIP_CHECK(hipSetDevice(ctx->device));
+    cpvs_dagset* s = new cpvs_dagset();
+    s->ctx = ctx;
+    s->count = count;
+    if (hipMalloc(&s->d_desc, sizeof(DagDesc) * count) != hipSuccess) {
+        (void)hipGetLastError();
+        delete s;
+        set_error("dagset_create: hipMalloc failed");
+        return CPVS_ENOMEM;
+    }
+    const hipError_t e = hipMemcpy(s->d_desc, h.data(), sizeof(DagDesc) * count, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(s->d_desc);
+        delete s;
+        return hip_fail(e, "dagset_create copy", __FILE__, __LINE__);
+    }
+    *out = s;
+    return CPVS_OK;
+    GUARD_END
+}
+
+int cpvs_dagset_evaluate(cpvs_dagset* s, const float* ndc, uint32_t stride, uint64_t count, const uint32_t* ids,
+                         uint8_t* vis, void* stream) {
+    CHECK_ARG(s && ((ndc && vis) || !count), "dagset_evaluate: NULL argument");
+    CHECK_ARG(stride >= 3, "dagset_evaluate: stride %u < 3", stride);
+    CPVS_HIP_CHECK(hipSetDevice(s->ctx->device));
+    return launch_dagset_eval(s->d_desc, s->count, DagDesc{}, ids, ndc, stride, count, vis, pick_stream(s->ctx, stream));
+}
+
+int cpvs_dagset_destroy(cpvs_dagset* s) {
+    if (!s) return CPVS_OK;
+    hipSetDevice(s->ctx->device);
+    if (s->d_desc) hipFree(s->d_desc);
+    delete s;
+    return CPVS_OK;
+}
+
+int cpvs_dag_evaluate(const cpvs_dag* d, const float* ndc, uint32_t stride, uint64_t count, uint8_t* vis, void* stream) {
+    CHECK_ARG(d && ((ndc && vis) || !count), "dag_evaluate: NULL argument");
+    CHECK_ARG(stride >= 3, "dag_evaluate: stride %u < 3", stride);
+    CHECK_ARG(d->n >= 5, "dag_evaluate: the lookup walks leafmask DAGs (numLevels >= 5), got %d", d->n);
+    CHECK_ARG(d->words < 0xFFFFFFFEull, "dag_evaluate: DAG exceeds 32-bit offsets");
+    CPVS_HIP_CHECK(hipSetDevice(d->ctx->device));
+    return launch_dagset_eval(nullptr, 0, desc_of(d), nullptr, ndc, stride, count, vis, pick_stream(d->ctx, stream));
+}
+
+}  // extern "C"

@@ -140,18 +140,35 @@ struct EmitChildrenOut {
 // 2. Leaves: createChildmask1x1x8 + createLeafmask (src/CompressedShadowUtil.cpp:59-99).
 //
 // Slice s of leaf node z3 (depth z = 8*z3 + s) is lit at texel t iff float(z) + 0.5 <= d_t*H
-// (absoluteVisible, H = N*T), i.e. iff 8*z3 + s <= f_t with f_t = floor(d_t*H - 0.5).  With
-// c_t = clamp(f_t - 8*z3 + 1, 0, 8) every texel is a thermometer code over the 8 slices: slices
-// below min(c) are fully lit, slices at or above max(c) fully shadowed, and slice s in between is
-// partial with leafmask ballot(c > s).  The node's merge key (mask and partial leafmasks) and its
-// c vector determine each other, so the key hash is taken over the c vector, lane-parallel.
-// Node z3 equals node z3-1 of the same column unless some f_t - 8*z3 lies in [-8, 6].
+// (absoluteVisible, H = N*T), i.e. iff u_t >= s with f_t = floor(d_t*H - 0.5) and
+// u_t = f_t - 8*z3.  With one texel per lane, slice s's leafmask is B_s = ballot(u >= s): eight
+// compares give the whole node.  The B_s are nested (B_0 >= B_1 >= ...), so the slices with
+// B_s all ones (fully lit) come first, then the partial ones, then those with B_s = 0 (shadow).
+// Everything past the compares is wave-uniform scalar work: the childmask, the key hash over
+// (mask, partial leafmasks), and the lane placement of the leafmasks for the store.
+// Node z3 equals node z3-1 of the same column unless some u_t lies in [-8, 6].
 // ---------------------------------------------------------------------------------------
-// One wave per column (the kernel is bound by per-wave latency, so as many independent waves as
-// possible are kept in flight).  Only a run's first node stores its mask, leafmasks and key
-// hash: the merge and the final write read those of first occurrences only, and a key's first
-// occurrence is always the first node of a run (breadth-first order grows with z inside a
-// column).  The other nodes of a run store just the index of the run's first node.
+
+// Key hash of a leaf node (wave-uniform): two 32-bit multiply-xorshift streams over the low and
+// high halves of the partial leafmasks, seeded by the childmask, then a 64-bit finaliser.
+struct LeafHash {
+    u32 a, b;
+    __device__ explicit LeafHash(u32 mask) : a(mask * 0x9E3779B1u + 0x7F4A7C15u), b((mask ^ 0xA5A5u) * 0x85EBCA77u + 1u) {}
+    __device__ void add(u64 m) {
+        a = (a ^ (u32)m) * 0x9E3779B1u;
+        a ^= a >> 15;
+        b = (b ^ (u32)(m >> 32)) * 0x85EBCA77u;
+        b ^= b >> 13;
+        a += b;
+    }
+    __device__ u64 value() const { return mix64(((u64)b << 32) | a); }
+};
+
+// One wave per column (one texel per lane; the nodes of the column in z order).  Only a run's
+// first node stores its mask, leafmasks and key hash: the merge and the final write read those
+// of first occurrences only, and a key's first occurrence is always the first node of a run
+// (breadth-first order grows with z inside a column).  The other nodes of a run store just the
+// index of the run's first node.
 __global__ __launch_bounds__(256) void k_leaf_columns(LeafColumns lc, const u32* __restrict__ colcnt, u32 ncol,
                                                       u32 nleaf, const float* __restrict__ depth, u32 N, u32 T,
                                                       u32* __restrict__ mask_out, u64* __restrict__ leafm,
@@ -164,46 +181,49 @@ __global__ __launch_bounds__(256) void k_leaf_columns(LeafColumns lc, const u32*
     const u32 base = lc.start[col];
     const u32 x3 = col % lc.w3, y3 = col / lc.w3;
     const int zs = column_zs(lc, col);
-    u32 bfs_chunk = (lane < cnt) ? lc.bfs[base + lane] : 0u;
     const float d = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
     const float D = d * (float)(N * T);
-    int f = -1;  // NaN is never lit
+    int f = -(1 << 30);  // NaN is never lit
     if (D == D) f = (int)floorf(fminf(fmaxf(D, -2.0f), 4194304.0f) - 0.5f);  // exact for |D| < 2^22
-    const u32 salt1 = (fmix32(lane * 0x9E3779B9u + 0x7F4A7C15u) & 0xFFFFFFu) | 1u;
-    const u32 salt2 = (fmix32(lane * 0x85EBCA77u + 0x165667B1u) & 0xFFFFFFu) | 1u;
+    // min(c) and max(c) of node z3 are clamp(fmin - 8*z3 + 1, 0, 8) and the same for fmax.
+    const int fmin = __ockl_wfred_min_i32(f), fmax = __ockl_wfred_max_i32(f);
 
     u32 head = 0;
-    for (u32 q = 0; q < cnt; ++q) {
-        if (q > 0 && (q & 63) == 0) {  // walls: the next 64 leaf indices
-            const u32 qi = q + lane;
-            bfs_chunk = (qi < cnt) ? lc.bfs[base + qi] : 0u;
-        }
-        const int u = f - 8 * (zs + (int)q);
-        const u32 b = (u32)__builtin_amdgcn_readlane((int)bfs_chunk, (int)(q & 63));
-        if (b >= nleaf) continue;  // only reachable if the column map were inconsistent
-        const bool changed = (q == 0) || (__ballot((u32)(u + 8) <= 14u) != 0ull);
-        if (changed) {
-            const int c = min(max(u + 1, 0), 8);
-            const u32 present = __ockl_wfred_or_u32(1u << c);  // bit v: some texel has c == v
-            const u32 s1 = __ockl_wfred_add_u32(__umul24((u32)c + 1u, salt1));
-            const u32 s2 = __ockl_wfred_add_u32(__umul24((u32)c + 1u, salt2));
-            const u32 cmin = (u32)__builtin_ctz(present), cmax = 31u - (u32)__builtin_clz(present);
-            const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
-            const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min);
-            const u32 k = cmax - cmin;
-            u64 mine = 0;
-            for (u32 s = cmin; s < cmax; ++s) {
-                const u64 m = __ballot(c > (int)s);
-                if (lane == s - cmin) mine = m;
+    for (u32 q0 = 0; q0 < cnt; q0 += 64) {
+        // Breadth-first indices of the next 64 nodes, one per lane.  Reading lane 0 here makes the
+        // wait for this load happen once, not in every iteration of the node loop below (whose
+        // stores would otherwise be waited for too).
+        const u32 bfs_chunk = (q0 + lane < cnt) ? lc.bfs[base + q0 + lane] : 0u;
+        u32 b = (u32)__builtin_amdgcn_readfirstlane((int)bfs_chunk);
+        const u32 qn = min(cnt - q0, 64u);
+        for (u32 qq = 0; qq < qn; ++qq) {
+            if (qq) b = (u32)__builtin_amdgcn_readlane((int)bfs_chunk, (int)qq);
+            const u32 q = q0 + qq;
+            const int z8 = 8 * (zs + (int)q);
+            const int u = f - z8;
+            if (b >= nleaf) continue;  // only reachable if the column map were inconsistent
+            const bool changed = (q == 0) || (__ballot((u32)(u + 8) <= 14u) != 0ull);
+            if (changed) {
+                const u32 cmin = (u32)min(max(fmin - z8 + 1, 0), 8);
+                const u32 cmax = (u32)min(max(fmax - z8 + 1, 0), 8);
+                const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
+                const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min);
+                LeafHash h(mask);
+                u64 mine = 0;  // lane j gets the leafmask of partial slice cmin + j
+                for (u32 sl = cmin; sl < cmax; ++sl) {
+                    const u64 m = __ballot(u >= (int)sl);
+                    h.add(m);
+                    if (lane + cmin == sl) mine = m;
+                }
+                head = b;
+                if (lane < cmax - cmin) leafm[(size_t)b * 8 + lane] = mine;
+                if (lane == 0) {
+                    mask_out[b] = mask;
+                    hkey[b] = h.value();
+                }
             }
-            head = b;
-            if (lane < k) leafm[(size_t)b * 8 + lane] = mine;
-            if (lane == 0) {
-                mask_out[b] = mask;
-                hkey[b] = mix64(((u64)s2 << 32 | s1) ^ ((u64)mask * 0x9E3779B97F4A7C15ull));
-            }
+            if (lane == 0) runhead[b] = head;
         }
-        if (lane == 0) runhead[b] = head;
     }
 }
 
@@ -509,7 +529,8 @@ u32 table_slots(u32 n) {
     return S;
 }
 
-int build_once(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, int merge_mode, cpvs_dag** out) {
+int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, int merge_mode,
+               cpvs_dag** out) {
     const int n = p->n;
     const int R = n - 2;
     const bool leaf = n >= 5;
@@ -518,7 +539,6 @@ int build_once(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s
     const bool use_table = merge_mode == CPVS_MERGE_TABLE;
 
     // --- exact node counts (one readback) ---------------------------------------------
-    Arena& A = ctx->arena;
     const u32 w3 = leaf ? (N >> 3) : 0;
     const u32 ncol = w3 * w3;
     const size_t meta_bytes = arena_bytes(kMaxLevels, sizeof(unsigned long long)) + arena_bytes(kMetaWords, sizeof(u32));
@@ -529,11 +549,11 @@ int build_once(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s
     u32* colcnt = leaf ? A.take<u32>(ncol) : nullptr;
     rc = pyramid_node_counts(p, t, T, d_counts, colcnt, s);
     if (rc) return rc;
-    CPVS_HIP_CHECK(hipMemcpyAsync(ctx->pinned, d_counts, sizeof(u64) * kMaxLevels, hipMemcpyDeviceToHost, s));
+    CPVS_HIP_CHECK(hipMemcpyAsync(pinned, d_counts, sizeof(u64) * kMaxLevels, hipMemcpyDeviceToHost, s));
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
     u64 cnt[kMaxLevels] = {};
     cnt[R] = 1;
-    for (int L = bottom; L < R; ++L) cnt[L] = ctx->pinned[L];
+    for (int L = bottom; L < R; ++L) cnt[L] = pinned[L];
     for (int L = bottom; L <= R; ++L) {
         if (cnt[L] > 0xFFFFFFF0ull) {
             set_error("build: level %d has %llu nodes (limit 2^32)", L, (unsigned long long)cnt[L]);
@@ -690,10 +710,10 @@ int build_once(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s
     CPVS_LAUNCH("level_bases", k_level_bases, dim3(1), dim3(1), 0, s, meta + kMetaS, meta + kMetaB, meta + kMetaW, R, deepest);
     CPVS_HIP_CHECK(hipMemcpyAsync(meta + kMetaRoot, lv[R].mask, sizeof(u32), hipMemcpyDeviceToDevice, s));
     CPVS_HIP_CHECK(hipGetLastError());
-    CPVS_HIP_CHECK(hipMemcpyAsync(ctx->pinned, meta, sizeof(u32) * kMetaWords, hipMemcpyDeviceToHost, s));
+    CPVS_HIP_CHECK(hipMemcpyAsync(pinned, meta, sizeof(u32) * kMetaWords, hipMemcpyDeviceToHost, s));
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
     u32 hm[kMetaWords];
-    std::memcpy(hm, ctx->pinned, sizeof(hm));
+    std::memcpy(hm, pinned, sizeof(hm));
 
     if (use_table && hm[kMetaColl] != 0) return kRetryWithSort;
     for (int L = deepest + 1; L <= R; ++L) {
@@ -760,9 +780,10 @@ int build_once(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s
 
 }  // namespace
 
-int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, cpvs_dag** out) {
-    int rc = build_once(ctx, p, t, T, s, ctx->merge_mode, out);
-    if (rc == kRetryWithSort) rc = build_once(ctx, p, t, T, s, CPVS_MERGE_SORT, out);
+int build_dag(cpvs_ctx* ctx, Arena& A, u64* pinned, int merge_mode, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
+              cpvs_dag** out) {
+    int rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out);
+    if (rc == kRetryWithSort) rc = build_once(ctx, A, pinned, p, t, T, s, CPVS_MERGE_SORT, out);
     return rc;
 }
 

@@ -5,6 +5,7 @@
 #include <new>
 #include <string>
 
+#include "abi.hpp"
 #include "internal.hpp"
 #include "scan.hpp"
 
@@ -79,31 +80,7 @@ void Arena::release() {
 
 using namespace cpvs;
 
-#define CHECK_ARG(cond, ...)          \
-    do {                              \
-        if (!(cond)) {                \
-            set_error(__VA_ARGS__);   \
-            return CPVS_EINVAL;       \
-        }                             \
-    } while (0)
 
-#define GUARD_BEGIN try {
-#define GUARD_END                                               \
-    }                                                           \
-    catch (const std::bad_alloc&) {                             \
-        set_error("host allocation failed");                    \
-        return CPVS_ENOMEM;                                     \
-    }                                                           \
-    catch (...) {                                               \
-        set_error("internal exception");                        \
-        return CPVS_EINTERNAL;                                  \
-    }
-
-static hipStream_t pick_stream(cpvs_ctx* ctx, void* stream) {
-    return stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
-}
-
-static bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
 
 extern "C" {
 
@@ -126,7 +103,7 @@ int cpvs_ctx_create(int device, cpvs_ctx** out) {
         delete c;
         return hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
     }
-    c->pinned_words = 256;
+    c->pinned_words = kPinnedWords;
     e = hipHostMalloc(reinterpret_cast<void**>(&c->pinned), sizeof(u64) * c->pinned_words, hipHostMallocDefault);
     if (e != hipSuccess) {
         hipStreamDestroy(c->stream);
@@ -143,6 +120,11 @@ int cpvs_ctx_destroy(cpvs_ctx* ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     ctx->arena.release();
+    for (cpvs::Worker* w : ctx->lanes) {
+        w->destroy();
+        delete w;
+    }
+    ctx->lanes.clear();
     if (ctx->pinned) hipHostFree(ctx->pinned);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -247,7 +229,7 @@ int cpvs_pyramid_build(cpvs_ctx* ctx, const float* depth, int depth_on_device, u
     }
     if (rc != CPVS_OK) {
         if (!p->borrowed && p->depth) hipFree(p->depth);
-        if (p->levels) hipFree(p->levels);
+        if (p->levels && p->levels_owned) hipFree(p->levels);
         delete p;
         return rc;
     }
@@ -308,7 +290,7 @@ int cpvs_pyramid_destroy(cpvs_pyramid* p) {
     if (!p) return CPVS_OK;
     hipSetDevice(p->ctx->device);
     if (!p->borrowed && p->depth) hipFree(p->depth);
-    if (p->levels) hipFree(p->levels);
+    if (p->levels && p->levels_owned) hipFree(p->levels);
     delete p;
     return CPVS_OK;
 }
@@ -383,7 +365,7 @@ int cpvs_build(cpvs_ctx* ctx, const cpvs_pyramid* p, uint32_t t, uint32_t T, voi
     std::lock_guard<std::mutex> lk(ctx->mu);
     ProfInstall prof(ctx->prof_on ? &ctx->prof : nullptr);
     CPVS_HIP_CHECK(hipSetDevice(ctx->device));
-    return build_dag(ctx, p, t, T, pick_stream(ctx, stream), out);
+    return build_dag(ctx, ctx->arena, ctx->pinned, ctx->merge_mode, p, t, T, pick_stream(ctx, stream), out);
     GUARD_END
 }
 
@@ -737,7 +719,15 @@ int cpvs_gen_queries_uniform(cpvs_ctx* ctx, float* d_out, uint64_t seed, uint64_
     CHECK_ARG(ctx && (d_out || !count), "gen_queries_uniform: NULL argument");
     CHECK_ARG(stride == 3 || stride == 4, "gen_queries_uniform: stride %u must be 3 or 4", stride);
     CPVS_HIP_CHECK(hipSetDevice(ctx->device));
-    return launch_gen_queries_uniform(d_out, seed, count, stride, pick_stream(ctx, stream));
+    return launch_gen_queries_uniform(d_out, seed, 0, 1, count, stride, pick_stream(ctx, stream));
+}
+
+int cpvs_gen_queries_uniform_range(cpvs_ctx* ctx, float* d_out, uint64_t seed, uint64_t first, uint64_t step,
+                                   uint64_t count, uint32_t stride, void* stream) {
+    CHECK_ARG(ctx && (d_out || !count), "gen_queries_uniform_range: NULL argument");
+    CHECK_ARG(stride == 3 || stride == 4, "gen_queries_uniform_range: stride %u must be 3 or 4", stride);
+    CPVS_HIP_CHECK(hipSetDevice(ctx->device));
+    return launch_gen_queries_uniform(d_out, seed, first, step, count, stride, pick_stream(ctx, stream));
 }
 
 int cpvs_gen_queries_gbuffer(cpvs_ctx* ctx, float* d_out, const float* d_depth, uint32_t N, uint64_t seed, uint32_t W,

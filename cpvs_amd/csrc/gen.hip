@@ -131,19 +131,23 @@ int launch_gen_depth(float* out, u32 fullN, u64 seed, u32 nboxes, int noise, u32
     return CPVS_OK;
 }
 
-__global__ __launch_bounds__(256) void k_gen_uniform(float* __restrict__ out, u64 seed, u64 count, u32 stride) {
+// Output point i is point first + i*step of the counter-based stream.
+__global__ __launch_bounds__(256) void k_gen_uniform(float* __restrict__ out, u64 seed, u64 first, u64 step, u64 count,
+                                                     u32 stride) {
     const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
+    const u64 k = first + i * step;
     for (u32 c = 0; c < 3; ++c) {
-        const float p = u01(sm64_at(seed, 3 * i + c));
+        const float p = u01(sm64_at(seed, 3 * k + c));
         out[i * stride + c] = 2.0f * p - 1.0f;
     }
     if (stride == 4) out[i * stride + 3] = 1.0f;
 }
 
-int launch_gen_queries_uniform(float* out, u64 seed, u64 count, u32 stride, hipStream_t s) {
+int launch_gen_queries_uniform(float* out, u64 seed, u64 first, u64 step, u64 count, u32 stride, hipStream_t s) {
     if (!count) return CPVS_OK;
-    CPVS_LAUNCH("gen_uniform", k_gen_uniform, dim3((u32)((count + 255) / 256)), dim3(256), 0, s, out, seed, count, stride);
+    CPVS_LAUNCH("gen_uniform", k_gen_uniform, dim3((u32)((count + 255) / 256)), dim3(256), 0, s, out, seed, first, step, count,
+                stride);
     CPVS_HIP_CHECK(hipGetLastError());
     return CPVS_OK;
 }

@@ -36,6 +36,19 @@ struct Arena {
 
 inline size_t arena_bytes(size_t count, size_t elem) { return (count * elem + 255) & ~(size_t)255; }
 
+constexpr size_t kPinnedWords = 256;  // pinned readback staging per stream lane (u64 words)
+
+// One build lane of a batch: a stream with its own scratch arena, pyramid-level arena and
+// pinned readback staging.  Lanes run concurrently from their own host threads.
+struct Worker {
+    hipStream_t stream = nullptr;
+    Arena arena;   // build scratch
+    Arena levels;  // pyramid levels of the item being built
+    u64* pinned = nullptr;
+    int create();  // on the current device
+    void destroy();
+};
+
 }  // namespace cpvs
 
 struct cpvs_ctx {
@@ -49,6 +62,8 @@ struct cpvs_ctx {
     bool prof_on = false;
     cpvs::Prof prof;
     int merge_mode = CPVS_MERGE_TABLE;
+    std::vector<cpvs::Worker*> lanes;  // batch build lanes, created on first use and kept
+    uint32_t batch_workers = 0;        // lanes a batch uses when the caller passes 0 (0 = default)
 };
 
 struct cpvs_pyramid {
@@ -57,6 +72,7 @@ struct cpvs_pyramid {
     int n = 0;                 // numLevels including the root image
     float* depth = nullptr;    // N*N floats (owned unless borrowed)
     bool borrowed = false;
+    bool levels_owned = true;  // false when the levels live in a batch lane's arena
     float2* levels = nullptr;  // one allocation holding every stored level
     float2* lv[cpvs::kMaxLevels] = {};  // per-level pointer (null when not stored)
 };
@@ -89,14 +105,17 @@ struct cpvs_container {
 
 namespace cpvs {
 
-// pyramid.hip
-int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s);
+// pyramid.hip.  Levels go to a new allocation owned by p, or, when `into` is given, to that
+// arena (p then does not own them; the arena is reset).
+int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* into = nullptr);
 // Exact node counts per level (counts[L] = nodes at octree level L) for z-tile t of T.
 // col_count (may be null) receives the partial-cell count of every level-3 texel (leafmask mode).
 int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, u32* col_count, hipStream_t s);
 
-// build.hip
-int build_dag(cpvs_ctx* ctx, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, cpvs_dag** out);
+// build.hip: one DAG from a pyramid on stream s, scratch from A, readbacks staged in pinned
+// (kPinnedWords words of pinned host memory).
+int build_dag(cpvs_ctx* ctx, Arena& A, u64* pinned, int merge_mode, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
+              cpvs_dag** out);
 
 // lookup.hip
 struct Mat4 {
@@ -106,12 +125,24 @@ int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, 
                     const Mat4& m, uint8_t* vis, hipStream_t s);
 int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 width,
                           u32 height, const Mat4& m, uint8_t* vis, hipStream_t s);
+// One DAG of a set, walked as a one-cell container.  base is 0, or the grid sentinel
+// 0xFFFFFFFF / 0xFFFFFFFE when the root is all shadow / all lit.
+struct DagDesc {
+    const u32* words;
+    u32 nwords;
+    u32 base;
+    u32 dl;  // numLevels
+    u32 pad;
+};
+constexpr uint8_t kBadId = 0xFF;  // visibility written for a query whose DAG id is outside the set
+int launch_dagset_eval(const DagDesc* set, u32 nset, const DagDesc& one, const u32* ids, const float* ndc, u32 stride,
+                       u64 count, uint8_t* vis, hipStream_t s);
 int launch_traverse_single(const u32* dag, u32 words, int n, const float* ndc, u64 count, int try_leaf, int* out,
                            hipStream_t s);
 
 // gen.hip
 int launch_gen_depth(float* out, u32 fullN, u64 seed, u32 nboxes, int noise, u32 x0, u32 y0, u32 tileN, hipStream_t s);
-int launch_gen_queries_uniform(float* out, u64 seed, u64 count, u32 stride, hipStream_t s);
+int launch_gen_queries_uniform(float* out, u64 seed, u64 first, u64 step, u64 count, u32 stride, hipStream_t s);
 int launch_gen_queries_gbuffer(float* out, const float* depth, u32 N, u64 seed, u32 W, u32 H, hipStream_t s);
 
 }  // namespace cpvs

@@ -30,35 +30,41 @@ __device__ inline u32 quantise(float v, u32 R) {
     return (u32)v;
 }
 
-// traverse.cs:75-133 (sentinels as tested at :85-88).
+// traverse.cs:89-133: the walk below one grid cell whose DAG starts at word `base` of `dag`
+// (pointers inside a DAG are relative to its first word, combineDAGs keeps them so).
+__device__ inline u32 walk_dag(const u32* __restrict__ dag, u32 words, int dl, u32 base, u32 px, u32 py, u32 pz) {
+    u32 off = base;
+    for (int level = dl - 2; level >= 3; --level) {
+        const u32 ci = (((px >> level) & 1u) << 1) | (((py >> level) & 1u) << 2) | (((pz >> level) & 1u) << 3);
+        if (off >= words) return 0u;
+        const u32 m = dag[off];
+        const u32 v = (m >> ci) & 3u;
+        if (v != 2u) return v;  // 0 shadow, 1 visible (3 never occurs)
+        const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
+        if (off + 1 + co >= words) return 0u;
+        off = base + dag[off + 1 + co];
+    }
+    if (off >= words) return 0u;
+    const u32 ci = (pz & 7u) * 2u;
+    const u32 m = dag[off];
+    const u32 v = (m >> ci) & 3u;
+    if (v != 2u) return v;
+    const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
+    const u32 bit = (px & 7u) + 8u * (py & 7u);
+    const u32 idx = off + co * 2 + 1 + (bit >> 5);
+    if (idx >= words) return 0u;
+    const u32 w = dag[idx];
+    return (w >> (bit & 31u)) & 1u;
+}
+
+// traverse.cs:75-88: the grid cell, its sentinels, then the walk.
 __device__ inline u32 walk(const GridDag& g, u32 px, u32 py, u32 pz) {
     const u32 G = 1u << g.gl;
     const u32 sh = (u32)(g.dl - 1);
     const u32 base = g.grid[((pz >> sh) * G + (py >> sh)) * G + (px >> sh)];
     if (base == 0xFFFFFFFFu) return 0u;
     if (base == 0xFFFFFFFEu) return 1u;
-    u32 off = base;
-    for (int level = g.dl - 2; level >= 3; --level) {
-        const u32 ci = (((px >> level) & 1u) << 1) | (((py >> level) & 1u) << 2) | (((pz >> level) & 1u) << 3);
-        if (off >= g.words) return 0u;
-        const u32 m = g.dag[off];
-        const u32 v = (m >> ci) & 3u;
-        if (v != 2u) return v;  // 0 shadow, 1 visible (3 never occurs)
-        const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
-        if (off + 1 + co >= g.words) return 0u;
-        off = base + g.dag[off + 1 + co];
-    }
-    if (off >= g.words) return 0u;
-    const u32 ci = (pz & 7u) * 2u;
-    const u32 m = g.dag[off];
-    const u32 v = (m >> ci) & 3u;
-    if (v != 2u) return v;
-    const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
-    const u32 bit = (px & 7u) + 8u * (py & 7u);
-    const u32 idx = off + co * 2 + 1 + (bit >> 5);
-    if (idx >= g.words) return 0u;
-    const u32 w = g.dag[idx];
-    return (w >> (bit & 31u)) & 1u;
+    return walk_dag(g.dag, g.words, g.dl, base, px, py, pz);
 }
 
 // traverse.cs:141-144: p = lightViewProj * (pos, 1); p /= p.w; then the path.
@@ -190,6 +196,89 @@ int launch_traverse_single(const u32* dag, u32 words, int n, const float* ndc, u
     if (count == 0) return CPVS_OK;
     CPVS_LAUNCH("traverse_single", k_traverse_single, dim3((u32)((count + 255) / 256)), dim3(256), 0, s, dag, words, n, ndc, count,
                        try_leaf, out);
+    CPVS_HIP_CHECK(hipGetLastError());
+    return CPVS_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// DAG sets: many independent DAGs, each walked as a one-cell container
+// (CompressedShadowContainer(unique_ptr<CompressedShadow>), src/CompressedShadowContainer.h:27-32,
+// with gridLevels 0 in shader/traverse.cs), queried in one launch.  Query i names its DAG by
+// ids[i], or by i mod the set size when ids is null.  Queries are NDC (the product with an
+// identity matrix and the divide by w = 1 are exact, so this equals evaluate() with identity).
+// ------------------------------------------------------------------------------------------
+
+__device__ inline uint8_t dagset_lookup(const DagDesc& d, const float* __restrict__ q) {
+    const u32 R = 1u << (d.dl - 1);
+    const u32 px = quantise(q[0], R), py = quantise(q[1], R), pz = quantise(q[2], R);
+    if (d.base == 0xFFFFFFFFu) return 0;
+    if (d.base == 0xFFFFFFFEu) return 1;
+    return (uint8_t)walk_dag(d.words, d.nwords, (int)d.dl, 0u, px, py, pz);
+}
+
+// Query i names its DAG by ids[i] (or uses `one` when set is null).
+__global__ __launch_bounds__(256) void k_dagset_eval(const DagDesc* __restrict__ set, u32 nset, DagDesc one,
+                                                     const u32* __restrict__ ids, const float* __restrict__ ndc, u32 stride,
+                                                     u32 count, uint8_t* __restrict__ vis) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    DagDesc d = one;
+    if (set) {
+        const u32 id = ids[i];
+        if (id >= nset) {
+            vis[i] = kBadId;
+            return;
+        }
+        d = set[id];
+    }
+    vis[i] = dagset_lookup(d, ndc + (size_t)i * stride);
+}
+
+// Round-robin queries (query i -> DAG i mod nset), walked DAG by DAG.  Waves are ordered
+// (group of kRRGroup DAGs, block of 64 queries per DAG, DAG in the group): all lanes of a wave
+// share one DAG's upper levels, the waves resident at any time walk only the kRRGroup DAGs of one
+// group, and since blocks are dealt to the XCDs in turn, each XCD's L2 serves kRRGroup/4 of
+// them.  The waves of one group read neighbouring queries, so input lines are shared in L2.
+constexpr u32 kRRGroup = 16;
+
+__global__ __launch_bounds__(256) void k_dagset_eval_rr(const DagDesc* __restrict__ set, u32 nset, u32 group,
+                                                        u32 kblocks, const float* __restrict__ ndc, u32 stride,
+                                                        u32 count, uint8_t* __restrict__ vis) {
+    const u32 g = blockIdx.x * blockDim.x + threadIdx.x;
+    const u32 w = g >> 6, lane = g & 63;
+    const u32 din = w % group, rest = w / group;
+    const u32 kb = rest % kblocks, grp = rest / kblocks;
+    const u32 dag = grp * group + din;
+    if (dag >= nset) return;
+    const u64 i = ((u64)kb * 64 + lane) * nset + dag;
+    if (i >= count) return;
+    vis[i] = dagset_lookup(set[dag], ndc + (size_t)i * stride);
+}
+
+int launch_dagset_eval(const DagDesc* set, u32 nset, const DagDesc& one, const u32* ids, const float* ndc, u32 stride,
+                       u64 count, uint8_t* vis, hipStream_t s) {
+    if (count == 0) return CPVS_OK;
+    if (count > 0xFFFFFF00ull) {
+        set_error("dagset evaluate: %llu queries exceed one launch (2^32 - 256)", (unsigned long long)count);
+        return CPVS_EINVAL;
+    }
+    if (set && !ids) {
+        const u64 per = (count + nset - 1) / nset;  // queries of DAG 0 (the most of any DAG)
+        const u64 kblocks = (per + 63) / 64;
+        const u32 group = nset < kRRGroup ? nset : kRRGroup;
+        const u64 groups = (nset + group - 1) / group;
+        const u64 threads = groups * group * kblocks * 64;  // one wave per DAG per 64 queries
+        const u64 blocks = (threads + 255) / 256;
+        if (blocks * 256 > 0xFFFFFFFFull) {  // thread indices are 32-bit
+            set_error("dagset evaluate: %llu queries over %u DAGs exceed one launch", (unsigned long long)count, nset);
+            return CPVS_EINVAL;
+        }
+        CPVS_LAUNCH("dagset_eval_rr", k_dagset_eval_rr, dim3((u32)blocks), dim3(256), 0, s, set, nset, group, (u32)kblocks,
+                    ndc, stride, (u32)count, vis);
+    } else {
+        CPVS_LAUNCH("dagset_eval", k_dagset_eval, dim3((u32)((count + 255) / 256)), dim3(256), 0, s, set, nset, one, ids,
+                    ndc, stride, (u32)count, vis);
+    }
     CPVS_HIP_CHECK(hipGetLastError());
     return CPVS_OK;
 }

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_pyr_small(const float* __restrict__ in,
     }
 }
 
-int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s) {
+int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* into) {
     const u32 N = p->N;
     const int n = p->n;
     // Store levels 1-2 only when asked, or when the octree uses them (no leafmasks, N < 16).
@@ -142,7 +142,12 @@ int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s) {
         const size_t w = N >> L;
         total += w * w;
     }
-    if (hipMalloc(&p->levels, sizeof(float2) * (total ? total : 1)) != hipSuccess) {
+    if (into) {
+        const int rc = into->reserve(arena_bytes(total ? total : 1, sizeof(float2)), s);
+        if (rc) return rc;
+        p->levels = into->take<float2>(total ? total : 1);
+        p->levels_owned = false;
+    } else if (hipMalloc(&p->levels, sizeof(float2) * (total ? total : 1)) != hipSuccess) {
         p->levels = nullptr;
         set_error("pyramid: hipMalloc of %zu bytes failed", sizeof(float2) * total);
         return CPVS_ENOMEM;

@@ -147,6 +147,50 @@ int cpvs_container_evaluate_image(cpvs_container *c, const float *positions_rgba
                                   const float *light_view_proj, uint8_t *vis, void *stream);
 int cpvs_container_destroy(cpvs_container *c);
 
+/* ---- Batches: many maps built concurrently, many DAGs queried in one launch ------------------
+ * No counterpart in the reference, which builds tiles one after another
+ * (src/DeferredRenderer.cpp:150-187); each map is built exactly as cpvs_build builds it. */
+
+/* One map of a batch: N x N float depth in device memory (borrowed for the call). */
+typedef struct cpvs_batch_item {
+    const float *depth;
+    uint32_t N;          /* power of two in [8, 65536] */
+    uint32_t zTileIndex; /* as CompressedShadow::create (src/CompressedShadow.h:48-49) */
+    uint32_t zTileNum;
+    uint32_t reserved;   /* 0 */
+} cpvs_batch_item;
+
+/* MinMaxHierarchy + CompressedShadow::create for every item, on `lanes` concurrent streams
+ * (0 = the context's tuned or default count; at most 16).  out[i] receives item i's DAG (words
+ * identical to cpvs_build's).  done_ms (optional, host) receives each item's completion time in
+ * ms since the call began.  On error no DAG is returned and out[] is all NULL. */
+int cpvs_build_batch(cpvs_ctx *ctx, const cpvs_batch_item *items, uint32_t count, uint32_t lanes, cpvs_dag **out,
+                     double *done_ms);
+int cpvs_ctx_set_batch_lanes(cpvs_ctx *ctx, uint32_t lanes); /* 0 = default */
+int cpvs_ctx_batch_lanes(cpvs_ctx *ctx, uint32_t *lanes);    /* the count a batch with lanes = 0 uses */
+/* "<name> <arch> <CUs>CU" of the context's device (keys the lane-count tuning cache). */
+int cpvs_ctx_device_name(cpvs_ctx *ctx, char *buf, uint64_t cap);
+/* Times the batch once per candidate lane count (best of reps runs, DAGs discarded), stores the
+ * fastest as the context's batch lane count and returns it in *best; ms[k] (optional) receives
+ * candidate k's time. */
+int cpvs_tune_batch(cpvs_ctx *ctx, const cpvs_batch_item *items, uint32_t count, const uint32_t *candidates,
+                    uint32_t ncand, uint32_t reps, uint32_t *best, double *ms);
+
+/* A set of DAGs (numLevels >= 5), each looked up as a one-DAG container
+ * (CompressedShadowContainer(unique_ptr<CompressedShadow>), src/CompressedShadowContainer.h:27-32)
+ * with an identity light matrix: NDC in, visibility out.  The DAGs are borrowed and must outlive
+ * the set. */
+typedef struct cpvs_dagset cpvs_dagset;
+int cpvs_dagset_create(cpvs_ctx *ctx, const cpvs_dag *const *dags, uint32_t count, cpvs_dagset **out);
+/* count queries (device NDC, stride >= 3 floats apart); query i uses DAG ids[i] (device u32), or
+ * DAG i mod count when ids is NULL.  vis (device) gets 0/1, or 255 for an id outside the set. */
+int cpvs_dagset_evaluate(cpvs_dagset *s, const float *ndc, uint32_t stride, uint64_t count, const uint32_t *ids,
+                         uint8_t *vis, void *stream);
+int cpvs_dagset_destroy(cpvs_dagset *s);
+/* The same for one DAG. */
+int cpvs_dag_evaluate(const cpvs_dag *dag, const float *ndc, uint32_t stride, uint64_t count, uint8_t *vis,
+                      void *stream);
+
 /* Device memory and timing helpers for callers without their own HIP allocator. */
 int cpvs_device_alloc(cpvs_ctx *ctx, uint64_t bytes, void **dptr);
 int cpvs_device_free(cpvs_ctx *ctx, void *dptr);
@@ -165,6 +209,9 @@ int cpvs_gen_depth(cpvs_ctx *ctx, float *d_out, uint32_t fullN, uint64_t seed, u
 /* count uniform light-space points as NDC (stride 3 or 4; the 4th component is 1). */
 int cpvs_gen_queries_uniform(cpvs_ctx *ctx, float *d_out, uint64_t seed, uint64_t count, uint32_t stride,
                              void *stream);
+/* Points first, first+step, ... (count of them) of the same stream as cpvs_gen_queries_uniform. */
+int cpvs_gen_queries_uniform_range(cpvs_ctx *ctx, float *d_out, uint64_t seed, uint64_t first, uint64_t step,
+                                   uint64_t count, uint32_t stride, void *stream);
 /* 2*W*H G-buffer points (float4 NDC) over the N x N depth map d_depth: per pixel one point on
  * the surface and one jittered by U(-2,2) voxels in z. */
 int cpvs_gen_queries_gbuffer(cpvs_ctx *ctx, float *d_out, const float *d_depth, uint32_t N, uint64_t seed,
