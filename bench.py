@@ -272,6 +272,105 @@ def run_batched(cp, np, ctx, maps=256, kmin=8, kmax=13, qper=1 << 20, reps=3):
             "lookups_identical": same_lookup}
 
 
+def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_size=4, load=0.7,
+                budget=200 << 30, lanes=4):
+    """Serving: a trace of nreq requests, each a build (probability p_build) of a 2^k map, k uniform
+    in [kmin, kmax], or a lookup of 2^U(10,22) uniform points against one of a warm set of
+    resident DAGs (warm_per_size per size, built before timing).  Depth maps (one per size) and
+    points are generated on the device first.  The trace is served by the serial server (one
+    request at a time, the plain path) and by the concurrent server (build lanes + continuous
+    lookup batching), first all at once (throughput), then with Poisson arrivals at `load` times
+    the serial throughput (latency = submission to completion, per request kind)."""
+    import random
+    seed = SEED0 + 4
+    rng = random.Random(seed)
+    sizes = list(range(kmin, kmax + 1))
+    log(f"[bench] serving: generating maps 2^{kmin}..2^{kmax} and points")
+    depth = {k: cp.gen_depth(1 << k, seed + 1000 * k, max(1, (32 << (2 * k)) >> 20), True) for k in sizes}
+    qmax = 1 << 22
+    q = cp.gen_queries_uniform(seed + 17, qmax, 3)
+    trace = []  # (kind, k or warm index, count, point offset)
+    for _ in range(nreq):
+        if rng.random() < p_build:
+            trace.append((0, rng.choice(sizes), 0, 0))
+        else:
+            n = int(2 ** rng.uniform(10, 22))
+            trace.append((1, rng.randrange(warm_per_size * len(sizes)), n, rng.randrange(0, qmax - n + 1)))
+    out_bytes = sum(t[2] for t in trace)
+    vis = cp.DeviceArray((out_bytes,), np.uint8, ctx)
+    ctx.synchronize()
+
+    def serve(serial, arrivals):
+        srv = cp.Server(lanes=lanes, budget_bytes=budget, serial=serial, ctx=ctx)
+        try:
+            warm = []
+            for k in sizes:
+                for _ in range(warm_per_size):
+                    warm.append(srv.build(depth[k]))
+            for t in warm:
+                srv.wait(t)
+            tickets, kinds, lag = [], [], []
+            off = 0
+            t0 = time.perf_counter()
+            for (kind, a, n, qo), at in zip(trace, arrivals):
+                if at is not None:
+                    while True:
+                        dt = at - (time.perf_counter() - t0)
+                        if dt <= 0:
+                            break
+                        if dt > 2e-4:
+                            time.sleep(dt - 1e-4)
+                    lag.append((time.perf_counter() - t0) - at)
+                if kind == 0:
+                    tickets.append(srv.build(depth[a]))
+                else:
+                    tickets.append(srv.lookup(warm[a], q, vis, count=n, offset=qo, vis_offset=off))
+                    off += n
+                kinds.append(kind)
+            lat = [srv.wait(t) for t in tickets]
+            sec = time.perf_counter() - t0
+            st = srv.stats()
+        finally:
+            srv.close()
+        b = [x for x, k in zip(lat, kinds) if k == 0]
+        l = [x for x, k in zip(lat, kinds) if k == 1]
+        return {"s": sec, "req_per_s": len(trace) / sec,
+                "build_ms_p50": pct(b, 50), "build_ms_p99": pct(b, 99),
+                "lookup_ms_p50": pct(l, 50), "lookup_ms_p99": pct(l, 99),
+                "lookup_batches": st["batches"], "lookups": st["lookups"], "evicted": st["evicted"],
+                "submit_lag_ms_p99": pct(lag, 99) * 1e3 if lag else None}
+
+    none = [None] * len(trace)
+    thr = {m: serve(m == "serial", none) for m in ("serial", "served")}
+    log(f"[bench] serving throughput: {thr}")
+    rate = load * thr["serial"]["req_per_s"]
+    arr, t = [], 0.0
+    for _ in trace:
+        t += rng.expovariate(rate)
+        arr.append(t)
+    lat = {m: serve(m == "serial", arr) for m in ("serial", "served")}
+    log(f"[bench] serving latency at {rate:.0f} req/s: {lat}")
+    vis.free()
+    q.free()
+    for d in depth.values():
+        d.free()
+
+    def r(v):
+        return None if v is None else round(v, 3)
+    return {"workload": f"{nreq} requests: {p_build:.0%} builds of 2^k maps (k uniform in [{kmin},{kmax}], "
+                        f"boxes + noise), lookups of 2^U(10,22) uniform points on one of "
+                        f"{warm_per_size * len(sizes)} resident DAGs; HBM budget {budget >> 30} GiB",
+            "lanes": lanes,
+            "throughput_req_per_s": {m: round(v["req_per_s"], 1) for m, v in thr.items()},
+            "throughput_speedup": round(thr["served"]["req_per_s"] / thr["serial"]["req_per_s"], 2),
+            "poisson_rate_req_per_s": round(rate, 1),
+            "latency_ms": {m: {k: r(v[k]) for k in ("build_ms_p50", "build_ms_p99", "lookup_ms_p50", "lookup_ms_p99")}
+                           for m, v in lat.items()},
+            "lookup_launches": {m: v["lookup_batches"] for m, v in lat.items()},
+            "evicted": {m: v["evicted"] for m, v in lat.items()},
+            "submit_lag_ms_p99": {m: r(v["submit_lag_ms_p99"]) for m, v in lat.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,6 +383,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tiled", action="store_true", help="skip the 65536^2 tiled configuration")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-map batched configuration")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving configuration")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -415,6 +515,11 @@ def main():
         batched = run_batched(cp, np, ctx)
         log(f"[bench] batched: {batched}")
 
+    serving = None
+    if not args.no_serving:
+        serving = run_serving(cp, np, ctx)
+        log(f"[bench] serving: {serving}")
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle as O
@@ -472,6 +577,7 @@ def main():
         "kernel_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in prof.items()},
         "tiled_65536": tiled,
         "batched_256_maps": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

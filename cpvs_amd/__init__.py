@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import CpvsError, call
 
 __all__ = ["Context", "DeviceArray", "MinMaxHierarchy", "CompressedShadow", "CompressedShadowContainer",
-           "DagSet", "CpvsError", "default_context", "build_batch", "tune_batch", "gen_depth",
+           "DagSet", "Server", "CpvsError", "default_context", "build_batch", "tune_batch", "gen_depth",
            "gen_queries_uniform", "gen_queries_uniform_range", "gen_queries_gbuffer"]
 
 
@@ -430,6 +430,85 @@ class DagSet:
     def close(self):
         if getattr(self, "handle", None):
             _lib.load().cpvs_dagset_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop (cpvs_server): builds on concurrent stream lanes, lookups batched
+    continuously, DAGs resident under an HBM budget.  serial=True: one request at a time in
+    arrival order (the plain path).  Requests borrow their device buffers until waited for."""
+
+    STAT_NAMES = ("builds", "lookups", "batches", "queries", "evicted", "resident", "resident_bytes",
+                  "scratch_bytes", "reserved_bytes", "budget")
+
+    def __init__(self, lanes: int = 4, budget_bytes: int = 200 << 30, serial: bool = False,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.handle = None
+        h = C.c_void_p()
+        call("cpvs_server_create", self.ctx.handle, lanes, budget_bytes, int(serial), C.byref(h))
+        self.handle = h
+
+    def build(self, depth: DeviceArray, zTileIndex: int = 0, zTileNum: int = 1) -> int:
+        t = C.c_uint64()
+        call("cpvs_server_submit_build", self.handle, depth.ptr, depth.shape[0], zTileIndex, zTileNum, C.byref(t))
+        return t.value
+
+    def lookup(self, dag: int, ndc: DeviceArray, vis: DeviceArray, count: Optional[int] = None, offset: int = 0,
+               vis_offset: int = 0) -> int:
+        """Queries ndc[offset:offset+count] (stride = ndc's row length) against the resident DAG
+        of build ticket `dag`; results to vis[vis_offset:...]."""
+        stride = ndc.shape[1]
+        n = (ndc.shape[0] - offset) if count is None else count
+        t = C.c_uint64()
+        call("cpvs_server_submit_lookup", self.handle, dag, C.c_void_p(ndc.ptr.value + 4 * stride * offset), stride, n,
+             C.c_void_p(vis.ptr.value + vis_offset), C.byref(t))
+        return t.value
+
+    def wait(self, ticket: int) -> float:
+        """Latency (ms, submission to completion) of a finished request; raises on failure."""
+        ms = C.c_double()
+        call("cpvs_server_wait", self.handle, ticket, C.byref(ms))
+        return ms.value
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int()
+        call("cpvs_server_poll", self.handle, ticket, C.byref(d))
+        return bool(d.value)
+
+    def dag_words(self, dag: int) -> np.ndarray:
+        h = C.c_void_p()
+        call("cpvs_server_dag", self.handle, dag, C.byref(h))
+        nw = C.c_uint64()
+        call("cpvs_dag_info", h, None, C.byref(nw), None)
+        out = np.empty(max(nw.value, 1), np.uint32)
+        call("cpvs_dag_copy_words", h, out.ctypes.data_as(C.c_void_p), out.size)
+        return out[:nw.value]
+
+    def release(self, dag: int):
+        call("cpvs_server_release", self.handle, dag)
+
+    def resident(self):
+        n = C.c_uint32()
+        call("cpvs_server_resident", self.handle, None, 0, C.byref(n))
+        ids = np.zeros(max(n.value, 1), np.uint64)
+        call("cpvs_server_resident", self.handle, ids.ctypes.data_as(C.c_void_p), ids.size, C.byref(n))
+        return sorted(int(v) for v in ids[:min(n.value, ids.size)])
+
+    def stats(self) -> dict:
+        v = np.zeros(len(self.STAT_NAMES), np.uint64)
+        call("cpvs_server_stats", self.handle, v.ctypes.data_as(C.c_void_p), len(v))
+        return {k: int(x) for k, x in zip(self.STAT_NAMES, v)}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().cpvs_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

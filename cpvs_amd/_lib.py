@@ -84,6 +84,16 @@ _SIGS = {
     "cpvs_dagset_evaluate": [vp, vp, u32, u64, vp, vp, vp],
     "cpvs_dagset_destroy": [vp],
     "cpvs_dag_evaluate": [vp, vp, u32, u64, vp, vp],
+    "cpvs_server_create": [vp, u32, u64, i32, PP],
+    "cpvs_server_submit_build": [vp, vp, u32, u32, u32, C.POINTER(u64)],
+    "cpvs_server_submit_lookup": [vp, u64, vp, u32, u64, vp, C.POINTER(u64)],
+    "cpvs_server_wait": [vp, u64, C.POINTER(C.c_double)],
+    "cpvs_server_poll": [vp, u64, C.POINTER(i32)],
+    "cpvs_server_dag": [vp, u64, PP],
+    "cpvs_server_release": [vp, u64],
+    "cpvs_server_resident": [vp, vp, u32, C.POINTER(u32)],
+    "cpvs_server_stats": [vp, vp, u32],
+    "cpvs_server_destroy": [vp],
 }
 
 

@@ -183,16 +183,6 @@ uint32_t pick_lanes(const cpvs_ctx* ctx, uint32_t lanes) {
     return ctx->batch_workers ? ctx->batch_workers : kDefaultLanes;
 }
 
-DagDesc desc_of(const cpvs_dag* d) {
-    DagDesc r{};
-    r.words = d->d_words;
-    r.nwords = (u32)d->words;
-    // createTopLevelGrid (src/CompressedShadowContainer.cpp:71-91) sentinels for a one-cell grid.
-    r.base = (d->root_mask == 0u) ? 0xFFFFFFFFu : (d->root_mask == 0x5555u) ? 0xFFFFFFFEu : 0u;
-    r.dl = (u32)d->n;
-    return r;
-}
-
 }  // namespace
 }  // namespace cpvs
 
@@ -306,7 +296,7 @@ int cpvs_dagset_create(cpvs_ctx* ctx, const cpvs_dag* const* dags, uint32_t coun
         CHECK_ARG(d->ctx == ctx, "dagset_create: DAG %u belongs to another context", i);
         CHECK_ARG(d->n >= 5, "dagset_create: DAG %u has %d levels; the lookup walks leafmask DAGs (>= 5)", i, d->n);
         CHECK_ARG(d->words < 0xFFFFFFFEull, "dagset_create: DAG %u exceeds 32-bit offsets", i);
-        h[i] = desc_of(d);
+        h[i] = dag_desc(d);
     }
     GUARD_BEGIN
     CPVS_HIP_CHECK(hipSetDevice(ctx->device));
@@ -352,7 +342,7 @@ int cpvs_dag_evaluate(const cpvs_dag* d, const float* ndc, uint32_t stride, uint
     CHECK_ARG(d->n >= 5, "dag_evaluate: the lookup walks leafmask DAGs (numLevels >= 5), got %d", d->n);
     CHECK_ARG(d->words < 0xFFFFFFFEull, "dag_evaluate: DAG exceeds 32-bit offsets");
     CPVS_HIP_CHECK(hipSetDevice(d->ctx->device));
-    return launch_dagset_eval(nullptr, 0, desc_of(d), nullptr, ndc, stride, count, vis, pick_stream(d->ctx, stream));
+    return launch_dagset_eval(nullptr, 0, dag_desc(d), nullptr, ndc, stride, count, vis, pick_stream(d->ctx, stream));
 }
 
 }  // extern "C"

@@ -112,10 +112,19 @@ int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* 
 // col_count (may be null) receives the partial-cell count of every level-3 texel (leafmask mode).
 int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, u32* col_count, hipStream_t s);
 
+// Admission hook of a build: called once the exact node counts are known and before any scratch
+// or DAG memory is taken, with the scratch bytes the build needs (its arena grows only if it
+// holds less) and an upper bound of the DAG bytes.  May block; a non-zero return aborts the
+// build with that status.  Called again (replacing the first request) if the build retries.
+struct Admit {
+    int (*fn)(void* user, const Arena& arena, u64 scratch_bytes, u64 dag_bytes_max);
+    void* user;
+};
+
 // build.hip: one DAG from a pyramid on stream s, scratch from A, readbacks staged in pinned
 // (kPinnedWords words of pinned host memory).
 int build_dag(cpvs_ctx* ctx, Arena& A, u64* pinned, int merge_mode, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
-              cpvs_dag** out);
+              cpvs_dag** out, const Admit* admit = nullptr);
 
 // lookup.hip
 struct Mat4 {
@@ -134,9 +143,32 @@ struct DagDesc {
     u32 dl;  // numLevels
     u32 pad;
 };
+// The descriptor of a built DAG (createTopLevelGrid sentinels for a one-cell grid,
+// src/CompressedShadowContainer.cpp:71-91).
+inline DagDesc dag_desc(const cpvs_dag* d) {
+    DagDesc r{};
+    r.words = d->d_words;
+    r.nwords = (u32)d->words;
+    r.base = (d->root_mask == 0u) ? 0xFFFFFFFFu : (d->root_mask == 0x5555u) ? 0xFFFFFFFEu : 0u;
+    r.dl = (u32)d->n;
+    return r;
+}
 constexpr uint8_t kBadId = 0xFF;  // visibility written for a query whose DAG id is outside the set
 int launch_dagset_eval(const DagDesc* set, u32 nset, const DagDesc& one, const u32* ids, const float* ndc, u32 stride,
                        u64 count, uint8_t* vis, hipStream_t s);
+// Many lookup requests in one launch: request r's queries [0, count) use DAG d, read NDC from
+// ndc (stride floats apart) and write vis.  Requests take whole 256-query blocks, the first at
+// block0 (ascending).
+struct LookupSeg {
+    DagDesc d;
+    const float* ndc;
+    uint8_t* vis;
+    u32 stride;
+    u32 count;
+    u32 block0;
+    u32 pad;
+};
+int launch_segments_eval(const LookupSeg* segs, u32 nseg, u32 blocks, hipStream_t s);
 int launch_traverse_single(const u32* dag, u32 words, int n, const float* ndc, u64 count, int try_leaf, int* out,
                            hipStream_t s);
 

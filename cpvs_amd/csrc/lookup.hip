@@ -255,6 +255,29 @@ __global__ __launch_bounds__(256) void k_dagset_eval_rr(const DagDesc* __restric
     vis[i] = dagset_lookup(set[dag], ndc + (size_t)i * stride);
 }
 
+// One block of 256 queries of one request: the block finds its request by a binary search over
+// the (wave-uniform) block starts.
+__global__ __launch_bounds__(256) void k_segments_eval(const LookupSeg* __restrict__ segs, u32 nseg) {
+    const u32 b = blockIdx.x;
+    u32 lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const u32 mid = (lo + hi + 1) >> 1;
+        if (segs[mid].block0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const LookupSeg& g = segs[lo];
+    const u32 i = (b - g.block0) * 256 + threadIdx.x;
+    if (i >= g.count) return;
+    g.vis[i] = dagset_lookup(g.d, g.ndc + (size_t)i * g.stride);
+}
+
+int launch_segments_eval(const LookupSeg* segs, u32 nseg, u32 blocks, hipStream_t s) {
+    if (!nseg || !blocks) return CPVS_OK;
+    CPVS_LAUNCH("segments_eval", k_segments_eval, dim3(blocks), dim3(256), 0, s, segs, nseg);
+    CPVS_HIP_CHECK(hipGetLastError());
+    return CPVS_OK;
+}
+
 int launch_dagset_eval(const DagDesc* set, u32 nset, const DagDesc& one, const u32* ids, const float* ndc, u32 stride,
                        u64 count, uint8_t* vis, hipStream_t s) {
     if (count == 0) return CPVS_OK;

@@ -191,6 +191,35 @@ int cpvs_dagset_destroy(cpvs_dagset *s);
 int cpvs_dag_evaluate(const cpvs_dag *dag, const float *ndc, uint32_t stride, uint64_t count, uint8_t *vis,
                       void *stream);
 
+/* ---- Serving: a request loop over builds and lookups -----------------------------------------
+ * No counterpart in the reference (its renderer builds at start-up and evaluates once per frame,
+ * src/DeferredRenderer.cpp:221-230,333).  Requests carry device buffers, borrowed until the
+ * request completes.  Builds run on `lanes` concurrent streams; every lookup pending when the
+ * lookup stream frees up goes into one launch; DAGs stay resident, named by their build's ticket,
+ * within hbm_budget_bytes (resident DAGs + builds in flight, each reserving its scratch growth
+ * and an upper bound of its DAG from the exact node counts).  Least recently used DAGs without
+ * queued lookups are evicted to admit a build; a build that cannot fit fails with CPVS_EBUDGET.
+ * serial != 0: one request at a time in arrival order on one stream (the plain path). */
+typedef struct cpvs_server cpvs_server;
+int cpvs_server_create(cpvs_ctx *ctx, uint32_t lanes, uint64_t hbm_budget_bytes, int serial, cpvs_server **out);
+int cpvs_server_submit_build(cpvs_server *s, const float *depth, uint32_t N, uint32_t zTileIndex, uint32_t zTileNum,
+                             uint64_t *ticket);
+/* dag: the ticket of a completed build whose DAG is resident (else CPVS_ESTATE). */
+int cpvs_server_submit_lookup(cpvs_server *s, uint64_t dag, const float *ndc, uint32_t stride, uint64_t count,
+                              uint8_t *vis, uint64_t *ticket);
+/* Blocks until the request is done; returns its status; latency_ms = completion - submission. */
+int cpvs_server_wait(cpvs_server *s, uint64_t ticket, double *latency_ms);
+int cpvs_server_poll(cpvs_server *s, uint64_t ticket, int *done);
+/* The resident DAG of a build (valid until released or evicted). */
+int cpvs_server_dag(cpvs_server *s, uint64_t dag, const cpvs_dag **out);
+int cpvs_server_release(cpvs_server *s, uint64_t dag);
+int cpvs_server_resident(cpvs_server *s, uint64_t *ids, uint32_t cap, uint32_t *count);
+/* builds, lookups, lookup launches, queries, evictions, resident DAGs, resident bytes, lane scratch
+ * bytes, reserved bytes, budget (the first cap of them). */
+int cpvs_server_stats(cpvs_server *s, uint64_t *out, uint32_t cap);
+/* Finishes every queued request, then frees the resident DAGs. */
+int cpvs_server_destroy(cpvs_server *s);
+
 /* Device memory and timing helpers for callers without their own HIP allocator. */
 int cpvs_device_alloc(cpvs_ctx *ctx, uint64_t bytes, void **dptr);
 int cpvs_device_free(cpvs_ctx *ctx, void *dptr);
