@@ -50,12 +50,8 @@ __device__ inline u32 fmix32(u32 h) {
 // ---------------------------------------------------------------------------------------
 // 1. Inner classification: createChildmask (src/CompressedShadowUtil.cpp:20-54).
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord, u32 count, const float2* __restrict__ lv,
-                                                  const float* __restrict__ depth, u32 N, int L, u32 T,
-                                                  u32* __restrict__ mask_out) {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const u64 c = coord[i];
+__device__ inline u32 classify_node(u64 c, const float2* __restrict__ lv, const float* __restrict__ depth, u32 N, int L,
+                                    u32 T) {
     const u32 x = unpack_x(c), y = unpack_y(c), z = unpack_z(c);
     const float h = (float)((N >> L) * T);
     float mn[4], mx[4];  // texel (x + (q&1), y + (q>>1))
@@ -80,7 +76,15 @@ __global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord,
         else bits = classify_mid(cz, mn[q] * h);
         mask |= bits << (2 * ch);
     }
-    mask_out[i] = mask;
+    return mask;
+}
+
+__global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord, u32 count, const float2* __restrict__ lv,
+                                                  const float* __restrict__ depth, u32 N, int L, u32 T,
+                                                  u32* __restrict__ mask_out) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    mask_out[i] = classify_node(coord[i], lv, depth, N, L, T);
 }
 
 // Leaf columns: the partial z range [zs, ze) of a level-3 texel, as the node counts define it.
@@ -238,7 +242,9 @@ __device__ __forceinline__ void leaf_column(const LeafColumns& lc, u32 cnt, u32 
 // needs as many independent chains in flight as it can hold.
 constexpr u32 kLeafCols = 1;
 
-__global__ __launch_bounds__(256) void k_leaf_columns(LeafColumns lc, const u32* __restrict__ colcnt, u32 ncol,
+// One wave per workgroup: columns differ widely in length (one node to hundreds at box walls), and
+// with four waves per workgroup a long column held its group's slot (0.76 ms against 0.61 ms).
+__global__ __launch_bounds__(64) void k_leaf_columns(LeafColumns lc, const u32* __restrict__ colcnt, u32 ncol,
                                                       u32 nleaf, const float* __restrict__ depth, u32 N, u32 T,
                                                       u32* __restrict__ mask_out, u64* __restrict__ leafm,
                                                       u64* __restrict__ hkey, u32* __restrict__ runhead) {
@@ -281,13 +287,24 @@ __global__ __launch_bounds__(256) void k_leaf_columns(LeafColumns lc, const u32*
 // ---------------------------------------------------------------------------------------
 // 3. Merge.
 // ---------------------------------------------------------------------------------------
+// Children ranks of inner node i (slots j >= k are 0).  All eight loads are issued before any is
+// used, so a node costs one memory round trip, not one per child.
+__device__ inline void load_child_ranks(const u32* cstart, const u32* child_rank, u32 i, u32 k, u32 r[8]) {
+    const u32 s = k ? cstart[i] : 0u;
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j) r[j] = (j < k) ? child_rank[s + j] : 0u;
+}
+
 __device__ inline u64 inner_key_hash(const u32* mask, const u32* cstart, const u32* child_rank, u32 i) {
     const u32 m = mask[i];
     const u32 k = num_children(m);
     u64 h = hash_step(kHashSeed, m);
     if (k) {
-        const u32 s = cstart[i];
-        for (u32 j = 0; j < k; ++j) h = hash_step(h, child_rank[s + j]);
+        u32 r[8];
+        load_child_ranks(cstart, child_rank, i, k, r);
+#pragma unroll
+        for (u32 j = 0; j < 8; ++j)
+            if (j < k) h = hash_step(h, r[j]);
     }
     return h;
 }
@@ -307,9 +324,11 @@ struct LeafEq {
         const u32 k = num_children(m);
         const u64* pa = leafm + (size_t)a * 8;
         const u64* pb = leafm + (size_t)b * 8;
-        for (u32 j = 0; j < k; ++j)
-            if (pa[j] != pb[j]) return false;
-        return true;
+        bool eq = true;
+#pragma unroll
+        for (u32 j = 0; j < 8; ++j)
+            if (j < k) eq = eq && (pa[j] == pb[j]);
+        return eq;
     }
 };
 struct InnerEq {
@@ -321,10 +340,13 @@ struct InnerEq {
         if (m != mask[b]) return false;
         const u32 k = num_children(m);
         if (!k) return true;
-        const u32 sa = cstart[a], sb = cstart[b];
-        for (u32 j = 0; j < k; ++j)
-            if (child_rank[sa + j] != child_rank[sb + j]) return false;
-        return true;
+        u32 ra[8], rb[8];
+        load_child_ranks(cstart, child_rank, a, k, ra);
+        load_child_ranks(cstart, child_rank, b, k, rb);
+        bool eq = true;
+#pragma unroll
+        for (u32 j = 0; j < 8; ++j) eq = eq && (ra[j] == rb[j]);
+        return eq;
     }
 };
 
@@ -356,20 +378,38 @@ __device__ inline u32 table_insert(u64* th, u32* ti, u32 S, u64 h, u32 i) {
     }
 }
 
+// Neighbouring nodes in breadth-first order are often equal (siblings in flat regions), and a
+// table slot hit by many threads at once serialises their atomics.  So within a wave only the
+// first lane of each run of equal hashes in adjacent active lanes touches the table; the others
+// take its slot (the run's first lane has the smallest index).  Every lane of the wave calls this.
+__device__ inline u32 table_insert_runs(u64* th, u32* ti, u32 S, u64 h, u32 i, bool active) {
+    const u32 lane = threadIdx.x & (kWave - 1);
+    const u64 hp = __shfl_up(h, 1, kWave);
+    const int ap = __shfl_up(active ? 1 : 0, 1, kWave);
+    const bool head = active && (lane == 0 || !ap || hp != h);
+    u32 slot = head ? table_insert(th, ti, S, h, i) : 0u;
+    const u64 heads = __ballot(head);
+    const u64 upto = heads & ((lane == kWave - 1) ? ~0ull : ((2ull << lane) - 1ull));
+    const int src = upto ? 63 - __builtin_clzll(upto) : (int)lane;
+    return (u32)__shfl((int)slot, src, kWave);
+}
+
 __global__ __launch_bounds__(256) void k_insert_leaf(const u64* __restrict__ hkey, const u32* __restrict__ runhead, u32 n,
                                                      u64* __restrict__ th, u32* __restrict__ ti, u32 S,
                                                      u32* __restrict__ tslot) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || runhead[i] != i) return;  // a run's first node has the run's smallest index
-    tslot[i] = table_insert(th, ti, S, hkey[i], i);
+    const bool act = i < n && runhead[i] == i;  // a run's first node has the run's smallest index
+    const u32 slot = table_insert_runs(th, ti, S, act ? hkey[i] : 0ull, i, act);
+    if (act) tslot[i] = slot;
 }
 
 __global__ __launch_bounds__(256) void k_insert_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
                                                       const u32* __restrict__ child_rank, u32 n, u64* __restrict__ th,
                                                       u32* __restrict__ ti, u32 S, u32* __restrict__ tslot) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    tslot[i] = table_insert(th, ti, S, inner_key_hash(mask, cstart, child_rank, i), i);
+    const bool act = i < n;
+    const u32 slot = table_insert_runs(th, ti, S, act ? inner_key_hash(mask, cstart, child_rank, i) : 0ull, i, act);
+    if (act) tslot[i] = slot;
 }
 
 template <class Eq>
@@ -498,6 +538,16 @@ struct UniqueSizeIn {
     }
 };
 
+// Offsets of the unique nodes only (the scan runs over the level's node count, which the host
+// knows, while the unique count is on the device).
+struct UniqueOffOut {
+    u32* off;
+    const u32* n_uniq;
+    __device__ void operator()(u32 r, u32 ex, u32) const {
+        if (r < *n_uniq) off[r] = ex;
+    }
+};
+
 __global__ void k_level_bases(const u32* __restrict__ S, u32* __restrict__ B, u32* __restrict__ total, int R, int bottom) {
     // B[R] = 0, B[L-1] = B[L] + S[L]; total = B[bottom] + S[bottom] (levels absent have S = 0).
     unsigned long long acc = 0;
@@ -508,20 +558,34 @@ __global__ void k_level_bases(const u32* __restrict__ S, u32* __restrict__ B, u3
     *total = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
 }
 
-__global__ __launch_bounds__(256) void k_write_inner(const u32* __restrict__ uniq, u32 n_uniq, const u32* __restrict__ mask,
-                                                     const u32* __restrict__ cstart, const u32* __restrict__ off, u32 base,
-                                                     const u32* __restrict__ child_rank, const u32* __restrict__ child_off,
-                                                     u32 child_base, u32* __restrict__ out) {
-    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_uniq) return;
+// Unique node r of an inner level: [mask, absolute offsets of its partial children].
+__device__ inline void write_inner_node(u32 r, const u32* __restrict__ uniq, const u32* __restrict__ mask,
+                                        const u32* __restrict__ cstart, const u32* __restrict__ off, u32 base,
+                                        const u32* __restrict__ child_rank, const u32* __restrict__ child_off, u32 child_base,
+                                        u32* __restrict__ out) {
     const u32 i = uniq[r];
     const u32 o = base + off[r];
     const u32 m = mask[i];
     out[o] = m;
     const u32 k = num_children(m);
     if (!k) return;
-    const u32 s = cstart[i];
-    for (u32 j = 0; j < k; ++j) out[o + 1 + j] = child_base + child_off[child_rank[s + j]];
+    u32 cr[8];
+    load_child_ranks(cstart, child_rank, i, k, cr);
+    u32 co[8];
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j) co[j] = (j < k) ? child_off[cr[j]] : 0u;
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j)
+        if (j < k) out[o + 1 + j] = child_base + co[j];
+}
+
+__global__ __launch_bounds__(256) void k_write_inner(const u32* __restrict__ uniq, u32 n_uniq, const u32* __restrict__ mask,
+                                                     const u32* __restrict__ cstart, const u32* __restrict__ off, u32 base,
+                                                     const u32* __restrict__ child_rank, const u32* __restrict__ child_off,
+                                                     u32 child_base, u32* __restrict__ out) {
+    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_uniq) return;
+    write_inner_node(r, uniq, mask, cstart, off, base, child_rank, child_off, child_base, out);
 }
 
 __global__ __launch_bounds__(256) void k_write_leaf(const u32* __restrict__ uniq, u32 n_uniq, const u32* __restrict__ mask,
@@ -535,11 +599,15 @@ __global__ __launch_bounds__(256) void k_write_leaf(const u32* __restrict__ uniq
     out[o] = m;
     const u32 k = num_children(m);
     const u64* p = leafm + (size_t)i * 8;
-    for (u32 j = 0; j < k; ++j) {
-        const u64 v = p[j];
-        out[o + 1 + 2 * j] = (u32)v;
-        out[o + 2 + 2 * j] = (u32)(v >> 32);
-    }
+    u64 v[8];
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j) v[j] = (j < k) ? p[j] : 0ull;
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j)
+        if (j < k) {
+            out[o + 1 + 2 * j] = (u32)v[j];
+            out[o + 2 + 2 * j] = (u32)(v[j] >> 32);
+        }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -569,6 +637,224 @@ enum : int {
     kMetaSortColl = 4 * kMaxLevels + 3,  // 32-bit key collisions resolved by a scan (sort merge)
     kMetaWords = 4 * kMaxLevels + 4,
 };
+
+// ---------------------------------------------------------------------------------------
+// Small levels in one workgroup.  The upper octree levels hold few nodes (at 8192^2 the levels
+// above 6 hold a few thousand together), yet each costs about ten launches of the per-level
+// path at several microseconds apiece.  A level of at most kSmallMax nodes is handled by one
+// 1024-thread workgroup instead, all such levels in one launch per phase: top-down
+// classification and child emission, bottom-up merge (hash table in LDS), and compress
+// (size scans, level bases, words).  Results are those of the per-level path.
+// ---------------------------------------------------------------------------------------
+constexpr u32 kSmallMax = 2048;
+constexpr int kSmallThreads = 1024;
+constexpr u32 kSmallPer = kSmallMax / kSmallThreads;  // nodes per thread
+constexpr u32 kSmallSlots = 2 * kSmallMax;             // LDS table: 48 KB
+
+struct SmallArgs {
+    int R, Ls, deepest;  // the small levels are [Ls, R]; deepest = lowest level with nodes
+    u32 count[kMaxLevels];
+    u64* coord[kMaxLevels];
+    u32* mask[kMaxLevels];
+    u32* cstart[kMaxLevels];
+    u32* rank[kMaxLevels];
+    u32* uniq[kMaxLevels];
+    u32* off[kMaxLevels];
+    const float2* lv[kMaxLevels];
+    u32* meta;
+};
+
+// Exclusive sum over the workgroup, one value per thread; *total = the sum.
+__device__ inline u32 small_scan(u32 v, u32* s_wave, u32* total) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const u32 inc = wave_inclusive_scan(v, OpSum{});
+    if (lane == kWave - 1) s_wave[wid] = inc;
+    __syncthreads();
+    u32 wpre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kSmallThreads / kWave; ++w) {
+        const u32 x = s_wave[w];
+        if (w < wid) wpre += x;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return wpre + inc - v;
+}
+
+// Thread t owns nodes [t*E, t*E + E) of a level of c nodes (contiguous, so scans keep order).
+__device__ inline u32 small_per_thread(u32 c) { return (c + kSmallThreads - 1) / kSmallThreads; }
+
+__global__ __launch_bounds__(kSmallThreads) void k_small_topdown(SmallArgs a, const float* __restrict__ depth, u32 N, u32 T,
+                                                                 u64 root, LeafColumns cols) {
+    __shared__ u32 s_wave[kSmallThreads / kWave];
+    if (threadIdx.x == 0) a.coord[a.R][0] = root;  // create(): root offset (0, 0, 2*zTileIndex)
+    __syncthreads();
+    for (int L = a.R; L >= a.Ls; --L) {
+        const u32 c = a.count[L];
+        const u32 E = small_per_thread(c), i0 = threadIdx.x * E;
+        u32 k[kSmallPer], sum = 0;
+#pragma unroll
+        for (u32 e = 0; e < kSmallPer; ++e) {
+            k[e] = 0;
+            const u32 i = i0 + e;
+            if (e < E && i < c) {
+                const u32 m = classify_node(a.coord[L][i], a.lv[L], depth, N, L, T);
+                a.mask[L][i] = m;
+                k[e] = num_children(m);
+                sum += k[e];
+            }
+        }
+        u32 total;
+        u32 first = small_scan(sum, s_wave, &total);
+        EmitChildrenOut emit{a.mask[L], a.coord[L], a.cstart[L], a.coord[L - 1], a.count[L - 1],
+                             (L == 3) ? cols : LeafColumns{}};
+#pragma unroll
+        for (u32 e = 0; e < kSmallPer; ++e) {
+            const u32 i = i0 + e;
+            if (e < E && i < c) {
+                emit(i, first, k[e]);
+                first += k[e];
+            }
+        }
+        if (threadIdx.x == 0) a.meta[kMetaC + L] = total;
+        __syncthreads();  // the child level's coordinates are read next
+    }
+}
+
+__device__ inline u32 lds_insert(u64* th, u32* ti, u32 S, u64 h, u32 i) {
+    if (h == 0ull) h = 1ull;  // 0 marks an empty slot
+    u32 slot = (u32)(h ^ (h >> 31)) & (S - 1);
+    for (;;) {
+        const u64 cur = atomicCAS(reinterpret_cast<unsigned long long*>(th + slot), 0ull, (unsigned long long)h);
+        if (cur == 0ull || cur == h) {
+            atomicMin(ti + slot, i);
+            return slot;
+        }
+        slot = (slot + 1) & (S - 1);
+    }
+}
+
+// Merge of the small levels Ls .. R-1 (the root is never merged), then the root's rank.  A node
+// takes the smallest index with its 64-bit key hash; if that node's key differs (a hash
+// collision), it looks for its first occurrence by comparing keys with every lower index.
+__global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
+    __shared__ u64 th[kSmallSlots];
+    __shared__ u32 ti[kSmallSlots];
+    __shared__ u32 s_wave[kSmallThreads / kWave];
+    for (int L = a.Ls; L < a.R; ++L) {
+        const u32 c = a.count[L];
+        u32 S = 64;
+        while (S < 2 * c) S <<= 1;
+        for (u32 j = threadIdx.x; j < S; j += kSmallThreads) {
+            th[j] = 0ull;
+            ti[j] = kNoRep;
+        }
+        __syncthreads();
+        const u32* child_rank = (L > a.deepest) ? a.rank[L - 1] : nullptr;
+        const u32 E = small_per_thread(c), i0 = threadIdx.x * E;
+        u32 slot[kSmallPer];
+#pragma unroll
+        for (u32 e = 0; e < kSmallPer; ++e) {
+            const u32 i = i0 + e;
+            slot[e] = 0;
+            if (e < E && i < c) slot[e] = lds_insert(th, ti, S, inner_key_hash(a.mask[L], a.cstart[L], child_rank, i), i);
+        }
+        __syncthreads();
+        const InnerEq eq{a.mask[L], a.cstart[L], child_rank};
+        u32 rep[kSmallPer], nfirst = 0;
+#pragma unroll
+        for (u32 e = 0; e < kSmallPer; ++e) {
+            const u32 i = i0 + e;
+            rep[e] = i;
+            if (e < E && i < c) {
+                u32 r = ti[slot[e]];
+                if (r != i && !eq(i, r)) {
+                    r = i;
+                    for (u32 j = 0; j < i; ++j)
+                        if (eq(i, j)) {
+                            r = j;
+                            break;
+                        }
+                }
+                rep[e] = r;
+                nfirst += (r == i) ? 1u : 0u;
+            }
+        }
+        u32 total;
+        u32 ex = small_scan(nfirst, s_wave, &total);
+#pragma unroll
+        for (u32 e = 0; e < kSmallPer; ++e) {
+            const u32 i = i0 + e;
+            if (e < E && i < c && rep[e] == i) {
+                a.rank[L][i] = ex;
+                a.uniq[L][ex] = i;
+                ++ex;
+            }
+        }
+        if (threadIdx.x == 0) a.meta[kMetaU + L] = total;
+        __syncthreads();  // first occurrences ranked
+#pragma unroll
+        for (u32 e = 0; e < kSmallPer; ++e) {
+            const u32 i = i0 + e;
+            if (e < E && i < c && rep[e] != i) a.rank[L][i] = a.rank[L][rep[e]];
+        }
+        __syncthreads();  // the next level reads these ranks
+    }
+    if (threadIdx.x == 0) {
+        a.rank[a.R][0] = 0;
+        a.uniq[a.R][0] = 0;
+        a.meta[kMetaU + a.R] = 1;
+    }
+}
+
+// Size scans of the small levels (1 + k words per unique node), then the level bases of all
+// levels (the per-level scans of the larger levels ran before).
+__global__ __launch_bounds__(kSmallThreads) void k_small_compress(SmallArgs a) {
+    __shared__ u32 s_wave[kSmallThreads / kWave];
+    for (int L = a.R; L >= a.Ls; --L) {
+        const u32 U = a.meta[kMetaU + L];
+        const u32 E = small_per_thread(U), i0 = threadIdx.x * E;
+        u32 sz[kSmallPer], sum = 0;
+#pragma unroll
+        for (u32 e = 0; e < kSmallPer; ++e) {
+            const u32 r = i0 + e;
+            sz[e] = (e < E && r < U) ? 1u + num_children(a.mask[L][a.uniq[L][r]]) : 0u;
+            sum += sz[e];
+        }
+        u32 total;
+        u32 ex = small_scan(sum, s_wave, &total);
+#pragma unroll
+        for (u32 e = 0; e < kSmallPer; ++e) {
+            const u32 r = i0 + e;
+            if (e < E && r < U) {
+                a.off[L][r] = ex;
+                ex += sz[e];
+            }
+        }
+        if (threadIdx.x == 0) a.meta[kMetaS + L] = total;
+    }
+    if (threadIdx.x == 0) {
+        unsigned long long acc = 0;
+        for (int L = a.R; L >= a.deepest; --L) {
+            a.meta[kMetaB + L] = (u32)acc;
+            acc += a.meta[kMetaS + L];
+        }
+        a.meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
+    }
+}
+
+__global__ __launch_bounds__(kSmallThreads) void k_small_write(SmallArgs a, u32* __restrict__ out) {
+    for (int L = a.R; L >= a.Ls; --L) {
+        const u32 U = a.meta[kMetaU + L], base = a.meta[kMetaB + L];
+        const bool kids = L > a.deepest;
+        const u32* child_rank = kids ? a.rank[L - 1] : nullptr;
+        const u32* child_off = kids ? a.off[L - 1] : nullptr;
+        const u32 child_base = kids ? a.meta[kMetaB + L - 1] : 0u;
+        for (u32 r = threadIdx.x; r < U; r += kSmallThreads)
+            write_inner_node(r, a.uniq[L], a.mask[L], a.cstart[L], a.off[L], base, child_rank, child_off, child_base, out);
+    }
+}
 
 inline dim3 grid1d(u32 n, u32 block = 256) { return dim3(ceil_div<u32>(n ? n : 1, block)); }
 
@@ -698,14 +984,36 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
 
     // --- 1/2. BFS classification -------------------------------------------------------------
     const u64 root = pack_xyz(0, 0, 2 * t);  // create(): root offset (0,0,2*zTileIndex)
-    CPVS_LAUNCH("set_u64", k_set_u64, dim3(1), dim3(1), 0, s, lv[R].coord, root);
-    int deepest = R;  // lowest level with nodes
-    for (int L = R; L >= bottom; --L) {
+    int deepest = R;  // lowest level with nodes (the counts are exact)
+    while (deepest > bottom && cnt[deepest - 1] > 0) --deepest;
+    // Small levels [Ls, R] go through the one-workgroup kernels (with the hash-table merge).
+    int Ls = R + 1;
+    if (use_table && leaf)
+        for (int L = R; L >= std::max(deepest, 3) && cnt[L] <= kSmallMax; --L) Ls = L;
+    SmallArgs sa{};
+    sa.R = R;
+    sa.Ls = Ls;
+    sa.deepest = deepest;
+    sa.meta = meta;
+    for (int L = bottom; L <= R; ++L) {
+        sa.count[L] = lv[L].count;
+        sa.coord[L] = lv[L].coord;
+        sa.mask[L] = lv[L].mask;
+        sa.cstart[L] = lv[L].cstart;
+        sa.rank[L] = lv[L].rank;
+        sa.uniq[L] = lv[L].uniq;
+        sa.off[L] = lv[L].off;
+        sa.lv[L] = p->lv[L];
+    }
+    const int top_big = std::min(R, Ls - 1);  // highest level of the per-level path
+    if (Ls <= R)
+        CPVS_LAUNCH("small_topdown", k_small_topdown, dim3(1), dim3(kSmallThreads), 0, s, sa, p->depth, N, T, root, lc);
+    else
+        CPVS_LAUNCH("set_u64", k_set_u64, dim3(1), dim3(1), 0, s, lv[R].coord, root);
+    for (int L = top_big; L >= deepest; --L) {
         const u32 c = lv[L].count;
-        if (c == 0) break;
-        deepest = L;
         if (leaf && L == 2) {
-            CPVS_LAUNCH("leaf_columns", k_leaf_columns, grid1d(ceil_div<u32>(ncol, kLeafCols), 4), dim3(256), 0, s, lc,
+            CPVS_LAUNCH("leaf_columns", k_leaf_columns, grid1d(ceil_div<u32>(ncol, kLeafCols), 1), dim3(64), 0, s, lc,
                         colcnt, ncol, c, p->depth, N,
                         T, lv[2].mask, leafm, hkey, runhead);
             break;
@@ -721,7 +1029,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     CPVS_HIP_CHECK(hipGetLastError());
 
     // --- 3. merge, bottom-up (the root is never merged) --------------------------------------
-    for (int L = deepest; L < R; ++L) {
+    for (int L = deepest; L < std::min(R, Ls); ++L) {
         const u32 c = lv[L].count;
         const bool is_leaf = leaf && L == 2;
         const u32* child_rank = (L > deepest) ? lv[L - 1].rank : nullptr;
@@ -758,17 +1066,24 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         device_scan(FirstFlagIn{rep}, FirstRankOut{lv[L].rank, lv[L].uniq}, c, OpSum{}, meta + kMetaU + L, scan_scratch, s);
         CPVS_LAUNCH("rank_copies", k_rank_copies, grid1d(c), dim3(256), 0, s, rep, rh, c, lv[L].rank);
     }
-    CPVS_LAUNCH("root_init", k_root_init, dim3(1), dim3(1), 0, s, lv[R].rank, lv[R].uniq, meta + kMetaU + R);
+    if (Ls <= R)
+        CPVS_LAUNCH("small_merge", k_small_merge, dim3(1), dim3(kSmallThreads), 0, s, sa);
+    else
+        CPVS_LAUNCH("root_init", k_root_init, dim3(1), dim3(1), 0, s, lv[R].rank, lv[R].uniq, meta + kMetaU + R);
     CPVS_HIP_CHECK(hipGetLastError());
 
     // --- 4. compress: sizes, bases, one readback, write -------------------------------------
-    for (int L = R; L >= deepest; --L) {
+    for (int L = top_big; L >= deepest; --L) {
         const u32 c = lv[L].count;
         const u32 is_leaf = (leaf && L == 2) ? 1u : 0u;
-        device_scan(UniqueSizeIn{lv[L].uniq, lv[L].mask, meta + kMetaU + L, is_leaf}, ArrOutExclusive{lv[L].off}, c, OpSum{},
-                    meta + kMetaS + L, scan_scratch, s);
+        device_scan(UniqueSizeIn{lv[L].uniq, lv[L].mask, meta + kMetaU + L, is_leaf},
+                    UniqueOffOut{lv[L].off, meta + kMetaU + L}, c, OpSum{}, meta + kMetaS + L, scan_scratch, s);
     }
-    CPVS_LAUNCH("level_bases", k_level_bases, dim3(1), dim3(1), 0, s, meta + kMetaS, meta + kMetaB, meta + kMetaW, R, deepest);
+    if (Ls <= R)
+        CPVS_LAUNCH("small_compress", k_small_compress, dim3(1), dim3(kSmallThreads), 0, s, sa);
+    else
+        CPVS_LAUNCH("level_bases", k_level_bases, dim3(1), dim3(1), 0, s, meta + kMetaS, meta + kMetaB, meta + kMetaW, R,
+                    deepest);
     CPVS_HIP_CHECK(hipMemcpyAsync(meta + kMetaRoot, lv[R].mask, sizeof(u32), hipMemcpyDeviceToDevice, s));
     CPVS_HIP_CHECK(hipGetLastError());
     CPVS_HIP_CHECK(hipMemcpyAsync(pinned, meta, sizeof(u32) * kMetaWords, hipMemcpyDeviceToHost, s));
@@ -800,7 +1115,8 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         set_error("build: hipMalloc of %llu DAG words failed", (unsigned long long)total);
         return CPVS_ENOMEM;
     }
-    for (int L = R; L >= deepest; --L) {
+    if (Ls <= R) CPVS_LAUNCH("small_write", k_small_write, dim3(1), dim3(kSmallThreads), 0, s, sa, d_words);
+    for (int L = top_big; L >= deepest; --L) {
         const u32 U = hm[kMetaU + L];
         const u32 base = hm[kMetaB + L];
         if (leaf && L == 2) {
