@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import CpvsError, call
 
 __all__ = ["Context", "DeviceArray", "MinMaxHierarchy", "CompressedShadow", "CompressedShadowContainer",
-           "DagSet", "Server", "CpvsError", "default_context", "build_batch", "tune_batch", "gen_depth",
+           "DagSet", "Server", "CpvsError", "default_context", "build_batch", "tune_batch", "dag_check", "gen_depth",
            "gen_queries_uniform", "gen_queries_uniform_range", "gen_queries_gbuffer"]
 
 
@@ -292,6 +292,31 @@ class CompressedShadow:
         call("cpvs_dag_evaluate", self._h(), C.c_void_p(base), st, n, visibilities.ptr, stream)
         return visibilities
 
+    def serialize(self) -> bytes:
+        """The DAG in the library's byte format (64-byte header + words)."""
+        n = C.c_uint64()
+        call("cpvs_dag_serialize", self._h(), None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value)
+        call("cpvs_dag_serialize", self._h(), buf, n.value, C.byref(n))
+        return buf.raw[:n.value]
+
+    @staticmethod
+    def deserialize(data: bytes, ctx: Optional[Context] = None) -> "CompressedShadow":
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        call("cpvs_dag_deserialize", ctx.handle, data, len(data), C.byref(h))
+        return CompressedShadow(h, ctx)
+
+    def save(self, path: str):
+        call("cpvs_dag_save", self._h(), os.fsencode(path))
+
+    @staticmethod
+    def load(path: str, ctx: Optional[Context] = None) -> "CompressedShadow":
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        call("cpvs_dag_load", ctx.handle, os.fsencode(path), C.byref(h))
+        return CompressedShadow(h, ctx)
+
     def _release(self):
         h, self.handle = self.handle, None
         return h
@@ -375,6 +400,20 @@ class CompressedShadowContainer:
         if lightViewProj is not None:
             m = np.ascontiguousarray(np.asarray(lightViewProj, np.float32).T.reshape(16))  # column-major
         call("cpvs_container_evaluate", self.handle, positions.ptr, stride, count,
+             None if m is None else m.ctypes.data_as(C.c_void_p), visibilities.ptr, stream)
+        return visibilities
+
+    def evaluate_pcf(self, positions: DeviceArray, lightViewProj=None, visibilities: Optional[DeviceArray] = None,
+                     stream=None) -> DeviceArray:
+        """Percentage-closer filtered visibility (float32 in [0, 1]) with the filter size of
+        setFilterSize: the fraction of lit leaf cells over the f x f neighbourhood of each path."""
+        n = positions.shape[0]
+        if visibilities is None:
+            visibilities = DeviceArray((n,), np.float32, self.ctx)
+        m = None
+        if lightViewProj is not None:
+            m = np.ascontiguousarray(np.asarray(lightViewProj, np.float32).T.reshape(16))
+        call("cpvs_container_evaluate_pcf", self.handle, positions.ptr, positions.shape[1], n,
              None if m is None else m.ctypes.data_as(C.c_void_p), visibilities.ptr, stream)
         return visibilities
 
@@ -516,6 +555,14 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def dag_check(data: bytes):
+    """Checks a serialized DAG without a GPU; returns (numLevels, words, nodes per level)."""
+    nl, nw = C.c_uint32(), C.c_uint64()
+    nodes = np.zeros(24, np.uint64)
+    call("cpvs_dag_check", data, len(data), C.byref(nl), C.byref(nw), nodes.ctypes.data_as(C.c_void_p), 24)
+    return nl.value, nw.value, nodes[:nl.value]
 
 
 def _batch_items(depths: Sequence[DeviceArray], tiles=None):

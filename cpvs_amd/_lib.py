@@ -84,6 +84,12 @@ _SIGS = {
     "cpvs_dagset_evaluate": [vp, vp, u32, u64, vp, vp, vp],
     "cpvs_dagset_destroy": [vp],
     "cpvs_dag_evaluate": [vp, vp, u32, u64, vp, vp],
+    "cpvs_dag_serialize": [vp, vp, u64, C.POINTER(u64)],
+    "cpvs_dag_check": [vp, u64, C.POINTER(u32), C.POINTER(u64), vp, i32],
+    "cpvs_dag_deserialize": [vp, vp, u64, PP],
+    "cpvs_dag_save": [vp, C.c_char_p],
+    "cpvs_dag_load": [vp, C.c_char_p, PP],
+    "cpvs_container_evaluate_pcf": [vp, vp, u32, u64, vp, vp, vp],
     "cpvs_server_create": [vp, u32, u64, i32, PP],
     "cpvs_server_submit_build": [vp, vp, u32, u32, u32, C.POINTER(u64)],
     "cpvs_server_submit_lookup": [vp, u64, vp, u32, u64, vp, C.POINTER(u64)],

@@ -615,6 +615,22 @@ int cpvs_container_evaluate(cpvs_container* c, const float* pos, uint32_t stride
                            to_mat(m), vis, pick_stream(c->ctx, stream));
 }
 
+int cpvs_container_evaluate_pcf(cpvs_container* c, const float* pos, uint32_t stride, uint64_t count, const float* m,
+                                float* vis, void* stream) {
+    CHECK_ARG(c && (pos || !count) && (vis || !count), "container_evaluate_pcf: NULL argument");
+    CHECK_ARG(stride == 3 || stride == 4, "container_evaluate_pcf: stride %u must be 3 or 4", stride);
+    CHECK_ARG(c->filter_size >= 1 && c->filter_size <= kMaxFilterSize, "container_evaluate_pcf: filter size %u (1..%u)",
+              c->filter_size, kMaxFilterSize);
+    if (!c->on_gpu) {
+        set_error("container_evaluate_pcf: copyToGPU first");
+        return CPVS_ESTATE;
+    }
+    CPVS_HIP_CHECK(hipSetDevice(c->ctx->device));
+    ProfInstall prof(c->ctx->prof_on ? &c->ctx->prof : nullptr);
+    return launch_evaluate_pcf(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, pos, stride, count,
+                               to_mat(m), c->filter_size, vis, pick_stream(c->ctx, stream));
+}
+
 int cpvs_container_evaluate_image(cpvs_container* c, const float* pos, uint32_t width, uint32_t height, const float* m,
                                   uint8_t* vis, void* stream) {
     CHECK_ARG(c && pos && vis, "container_evaluate_image: NULL argument");

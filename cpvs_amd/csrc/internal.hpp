@@ -12,6 +12,7 @@
 namespace cpvs {
 
 constexpr int kMaxLevels = 24;  // N <= 2^21 texels per side
+constexpr uint32_t kMaxFilterSize = 64;  // PCF kernel width limit (64 x 64 lookups per point)
 
 void set_error(const char* fmt, ...);
 
@@ -132,6 +133,8 @@ struct Mat4 {
 };
 int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 stride, u64 count,
                     const Mat4& m, uint8_t* vis, hipStream_t s);
+int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 stride, u64 count,
+                        const Mat4& m, u32 f, float* vis, hipStream_t s);
 int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 width,
                           u32 height, const Mat4& m, uint8_t* vis, hipStream_t s);
 // One DAG of a set, walked as a one-cell container.  base is 0, or the grid sentinel

@@ -68,7 +68,7 @@ __device__ inline u32 walk(const GridDag& g, u32 px, u32 py, u32 pz) {
 }
 
 // traverse.cs:141-144: p = lightViewProj * (pos, 1); p /= p.w; then the path.
-__device__ inline u32 lookup_point(const GridDag& g, const Mat4& M, float x, float y, float z) {
+__device__ inline void project(const GridDag& g, const Mat4& M, float x, float y, float z, u32& px, u32& py, u32& pz) {
     float r[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -79,10 +79,53 @@ __device__ inline u32 lookup_point(const GridDag& g, const Mat4& M, float x, flo
         r[c] = acc;
     }
     const u32 R = 1u << (g.dl + g.gl - 1);
-    const u32 px = quantise(r[0] / r[3], R);
-    const u32 py = quantise(r[1] / r[3], R);
-    const u32 pz = quantise(r[2] / r[3], R);
+    px = quantise(r[0] / r[3], R);
+    py = quantise(r[1] / r[3], R);
+    pz = quantise(r[2] / r[3], R);
+}
+
+__device__ inline u32 lookup_point(const GridDag& g, const Mat4& M, float x, float y, float z) {
+    u32 px, py, pz;
+    project(g, M, x, y, z, px, py, pz);
     return walk(g, px, py, pz);
+}
+
+// Percentage-closer filtering (the reference declares the kernel width, filterSize, and leaves it
+// unused: shader/traverse.cs:16-17): the fraction of lit leaf cells among the f x f paths
+// (px + dx, py + dy, pz), dx and dy in [-(f-1)/2, f/2] (integer halves), each clamped to the grid,
+// as float(lit) / float(f*f).  f = 1 is the plain lookup.
+__global__ __launch_bounds__(256) void k_evaluate_pcf(GridDag g, Mat4 M, const float* __restrict__ pos, u32 stride,
+                                                      u64 count, u32 f, float* __restrict__ vis) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float* q = pos + i * stride;
+    u32 px, py, pz;
+    project(g, M, q[0], q[1], q[2], px, py, pz);
+    const int R = 1 << (g.dl + g.gl - 1);
+    const int lo = -(int)((f - 1) / 2);
+    u32 lit = 0;
+    for (u32 a = 0; a < f; ++a) {
+        const int y = min(max((int)py + lo + (int)a, 0), R - 1);
+        for (u32 b = 0; b < f; ++b) {
+            const int x = min(max((int)px + lo + (int)b, 0), R - 1);
+            lit += walk(g, (u32)x, (u32)y, pz);
+        }
+    }
+    vis[i] = (float)lit / (float)(f * f);
+}
+
+int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 stride, u64 count,
+                        const Mat4& m, u32 f, float* vis, hipStream_t s) {
+    if (count == 0) return CPVS_OK;
+    GridDag g{dag, grid, dl, gl, words};
+    const u64 blocks = (count + 255) / 256;
+    if (blocks > 0x7FFFFFFFull) {
+        set_error("evaluate_pcf: %llu points exceed one launch", (unsigned long long)count);
+        return CPVS_EINVAL;
+    }
+    CPVS_LAUNCH("evaluate_pcf", k_evaluate_pcf, dim3((u32)blocks), dim3(256), 0, s, g, m, pos, stride, count, f, vis);
+    CPVS_HIP_CHECK(hipGetLastError());
+    return CPVS_OK;
 }
 
 __global__ __launch_bounds__(256) void k_evaluate4(GridDag g, Mat4 M, const float4* __restrict__ pos, u64 count,

@@ -141,11 +141,32 @@ int cpvs_container_copy_words(const cpvs_container *c, uint32_t *host_words, uin
  * identity), divided by w, quantised and looked up; vis (device) gets 0 or 1 per point. */
 int cpvs_container_evaluate(cpvs_container *c, const float *positions, uint32_t stride_floats, uint64_t count,
                             const float *light_view_proj, uint8_t *vis, void *stream);
+/* Percentage-closer filtering with the container's filter size f (setFilterSize; the reference
+ * stores it and leaves it unused, shader/traverse.cs:16-17): vis (device, one float per point)
+ * gets the fraction of lit leaf cells among the f x f paths (px+dx, py+dy, pz), dx, dy in
+ * [-(f-1)/2, f/2], clamped to the grid.  f = 1 gives 0.0 / 1.0 as evaluate.  f in 1..64. */
+int cpvs_container_evaluate_pcf(cpvs_container *c, const float *positions, uint32_t stride_floats, uint64_t count,
+                                const float *light_view_proj, float *vis, void *stream);
 /* The same over a width x height rgba32f position image (row-major, 4 floats per texel), as the
  * reference's Texture2D arguments; vis is a width x height byte image. */
 int cpvs_container_evaluate_image(cpvs_container *c, const float *positions_rgba, uint32_t width, uint32_t height,
                                   const float *light_view_proj, uint8_t *vis, void *stream);
 int cpvs_container_destroy(cpvs_container *c);
+
+/* ---- DAG byte format ----------------------------------------------------------------------------
+ * CompressedShadow::getDAG's word array (src/CompressedShadow.h:75-77) behind a 64-byte versioned
+ * header (magic "CPVSDAG", version 1, numLevels, N, zTileIndex, zTileNum, root mask, word count,
+ * checksum); loading checks the header, the checksum and the DAG structure before any upload.
+ * serialize with buf NULL returns the size in *size. */
+int cpvs_dag_serialize(const cpvs_dag *dag, void *buf, uint64_t cap, uint64_t *size);
+/* Checks a serialized DAG without a GPU: header, checksum and structure (root at word 0, levels
+ * contiguous, every pointer the start of a next-level node, every node referenced, no trailing
+ * words).  Optional outputs: numLevels, word count, nodes per level (cap entries). */
+int cpvs_dag_check(const void *buf, uint64_t size, uint32_t *num_levels, uint64_t *words, uint64_t *level_nodes,
+                   int cap);
+int cpvs_dag_deserialize(cpvs_ctx *ctx, const void *buf, uint64_t size, cpvs_dag **out);
+int cpvs_dag_save(const cpvs_dag *dag, const char *path);
+int cpvs_dag_load(cpvs_ctx *ctx, const char *path, cpvs_dag **out);
 
 /* ---- Batches: many maps built concurrently, many DAGs queried in one launch ------------------
  * No counterpart in the reference, which builds tiles one after another
