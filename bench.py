@@ -64,8 +64,8 @@ def load_traffic(N):
         for line in open(path):
             if line.startswith("#") or not line.strip():
                 continue
-            name, fb, wb = line.split()
-            out[name] = int(fb) + int(wb)
+            name, fb, wb = line.rsplit(None, 2)  # kernel names may contain spaces
+            out[name.strip()] = int(fb) + int(wb)
     return out
 
 
@@ -414,7 +414,7 @@ def main():
     copy_ms = []
     for _ in range(5):
         c0 = ctx.event(stream)
-        cp._lib.call("cpvs_memcpy_d2d", ctx.handle, dst.ptr, src.ptr, copy_bytes, stream)
+        cp._lib.call("cpvs_device_copy", ctx.handle, dst.ptr, src.ptr, copy_bytes, stream)
         c1 = ctx.event(stream)
         copy_ms.append(ctx.elapsed_ms(c0, c1))
         ctx.destroy_event(c0)

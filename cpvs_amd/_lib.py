@@ -68,6 +68,7 @@ _SIGS = {
     "cpvs_memcpy_h2d": [vp, vp, vp, u64],
     "cpvs_memcpy_d2h": [vp, vp, vp, u64],
     "cpvs_memcpy_d2d": [vp, vp, vp, u64, vp],
+    "cpvs_device_copy": [vp, vp, vp, u64, vp],
     "cpvs_event_record": [vp, vp, PP],
     "cpvs_event_elapsed_ms": [vp, vp, C.POINTER(C.c_float)],
     "cpvs_event_destroy": [vp],

@@ -690,6 +690,14 @@ int cpvs_memcpy_d2d(cpvs_ctx* ctx, void* dst, const void* src, uint64_t bytes, v
     return CPVS_OK;
 }
 
+int cpvs_device_copy(cpvs_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream) {
+    CHECK_ARG(ctx && dst && src, "device_copy: NULL argument");
+    CHECK_ARG(bytes % 16 == 0, "device_copy: %llu bytes is not a multiple of 16", (unsigned long long)bytes);
+    CHECK_ARG(((uintptr_t)dst | (uintptr_t)src) % 16 == 0, "device_copy: pointers must be 16-byte aligned");
+    CPVS_HIP_CHECK(hipSetDevice(ctx->device));
+    return launch_device_copy(dst, src, bytes, pick_stream(ctx, stream));
+}
+
 int cpvs_memcpy_d2h(cpvs_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
     CHECK_ARG(ctx && dst && src, "memcpy_d2h: NULL argument");
     CPVS_HIP_CHECK(hipSetDevice(ctx->device));

@@ -172,6 +172,27 @@ __global__ __launch_bounds__(256) void k_gen_gbuffer(float4* __restrict__ out, c
     out[2 * pix + 1] = make_float4(nx, ny, 2.0f * z1 - 1.0f, 1.0f);
 }
 
+// Plain device copy, one 16-byte element per thread: the reference kernel for bandwidth and for
+// scaling HBM traffic counters (its grid spreads over all XCDs as the path's kernels do).
+__global__ __launch_bounds__(256) void k_device_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, u64 n) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+int launch_device_copy(void* dst, const void* src, u64 bytes, hipStream_t s) {
+    const u64 n = bytes / 16;
+    if (!n) return CPVS_OK;
+    const u64 blocks = (n + 255) / 256;
+    if (blocks > 0x7FFFFFFFull) {
+        set_error("device_copy: %llu bytes exceed one launch", (unsigned long long)bytes);
+        return CPVS_EINVAL;
+    }
+    CPVS_LAUNCH("device_copy", k_device_copy, dim3((u32)blocks), dim3(256), 0, s, static_cast<const uint4*>(src),
+                static_cast<uint4*>(dst), n);
+    CPVS_HIP_CHECK(hipGetLastError());
+    return CPVS_OK;
+}
+
 int launch_gen_queries_gbuffer(float* out, const float* depth, u32 N, u64 seed, u32 W, u32 H, hipStream_t s) {
     const u64 n = (u64)W * H;
     if (!n) return CPVS_OK;

@@ -178,6 +178,7 @@ int launch_traverse_single(const u32* dag, u32 words, int n, const float* ndc, u
 // gen.hip
 int launch_gen_depth(float* out, u32 fullN, u64 seed, u32 nboxes, int noise, u32 x0, u32 y0, u32 tileN, hipStream_t s);
 int launch_gen_queries_uniform(float* out, u64 seed, u64 first, u64 step, u64 count, u32 stride, hipStream_t s);
+int launch_device_copy(void* dst, const void* src, u64 bytes, hipStream_t s);
 int launch_gen_queries_gbuffer(float* out, const float* depth, u32 N, u64 seed, u32 W, u32 H, hipStream_t s);
 
 }  // namespace cpvs

@@ -247,6 +247,9 @@ int cpvs_device_free(cpvs_ctx *ctx, void *dptr);
 int cpvs_memcpy_h2d(cpvs_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int cpvs_memcpy_d2h(cpvs_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int cpvs_memcpy_d2d(cpvs_ctx *ctx, void *dst, const void *src, uint64_t bytes, void *stream); /* asynchronous */
+/* A plain copy kernel (16-byte aligned, bytes a multiple of 16), asynchronous: the measured HBM
+ * bandwidth and the scale of HBM traffic counters come from it. */
+int cpvs_device_copy(cpvs_ctx *ctx, void *dst, const void *src, uint64_t bytes, void *stream);
 int cpvs_event_record(cpvs_ctx *ctx, void *stream, void **event);
 int cpvs_event_elapsed_ms(void *start, void *stop, float *ms); /* synchronises on stop */
 int cpvs_event_destroy(void *event);
