@@ -171,6 +171,13 @@ public:
         d_ = nullptr;
         return d;
     }
+    // The DAG in the library's byte format (64-byte header + words), and back.
+    void save(const std::string& path) const { check(cpvs_dag_save(d_, path.c_str()), "cpvs_dag_save"); }
+    static std::unique_ptr<CompressedShadow> load(const std::string& path) {
+        cpvs_dag* d = nullptr;
+        check(cpvs_dag_load(Context::instance().get(), path.c_str(), &d), "cpvs_dag_load");
+        return std::unique_ptr<CompressedShadow>(new CompressedShadow(d));
+    }
 
 private:
     explicit CompressedShadow(cpvs_dag* d) : d_(d) {}
@@ -206,6 +213,14 @@ public:
                   uint8_t* deviceVisibilities, void* stream = nullptr) {
         check(cpvs_container_evaluate(c_, devicePositions, strideFloats, count, lightViewProj, deviceVisibilities, stream),
               "cpvs_container_evaluate");
+    }
+    // PCF with the filter size of setFilterSize (the reference declares it, shader/traverse.cs:16-17):
+    // one float per point, the fraction of lit leaf cells over the f x f neighbourhood.
+    void evaluatePCF(const float* devicePositions, uint32_t strideFloats, uint64_t count, const float lightViewProj[16],
+                     float* deviceVisibilities, void* stream = nullptr) {
+        check(cpvs_container_evaluate_pcf(c_, devicePositions, strideFloats, count, lightViewProj, deviceVisibilities,
+                                          stream),
+              "cpvs_container_evaluate_pcf");
     }
     void freeOnCPU() { check(cpvs_container_free_on_cpu(c_), "cpvs_container_free_on_cpu"); }
     void copyToGPU() { check(cpvs_container_copy_to_gpu(c_), "cpvs_container_copy_to_gpu"); }

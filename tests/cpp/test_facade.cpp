@@ -9,6 +9,7 @@
 // Exit status 0 when every case passes.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <map>
 #include <memory>
@@ -119,6 +120,24 @@ int main(int argc, char** argv) {
         }
     } catch (const Error& e) {
         std::printf("ERROR container: %s\n", e.what());
+        ++fails;
+    }
+    // DAG file round trip through the facade.
+    try {
+        auto& d = dag("depths32x32");
+        const char* tmp = std::getenv("TMPDIR");
+        const std::string path = std::string(tmp ? tmp : "/tmp") + "/cpvs_facade_dag.bin";
+        d.save(path);
+        std::unique_ptr<CompressedShadow> back = CompressedShadow::load(path);
+        std::remove(path.c_str());
+        ++cases;
+        if (back->getDAG() != d.getDAG() || back->getNumLevels() != d.getNumLevels() ||
+            back->getTotalVisibility() != d.getTotalVisibility()) {
+            std::printf("FAIL dag file round trip\n");
+            ++fails;
+        }
+    } catch (const Error& e) {
+        std::printf("ERROR dag file: %s\n", e.what());
         ++fails;
     }
     std::printf("%d cases, %d failures\n", cases, fails);
