@@ -87,19 +87,24 @@ __global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord,
     mask_out[i] = classify_node(coord[i], lv, depth, N, L, T);
 }
 
-// Leaf columns: the partial z range [zs, ze) of a level-3 texel, as the node counts define it.
+// Leaf columns: per level-3 texel one record {partial cells, first leaf's index in column order,
+// first partial z cell, 0}, so a column's scalars come in one load.  The counts (x, z) come from
+// the node-count pass, the starts (y) from a scan of the counts.
 struct LeafColumns {
-    const float2* lv3;  // pyramid level 3
-    const u32* start;   // exclusive scan of leaves per column
-    u32* bfs;           // (column start + z - zs) -> breadth-first index of the leaf node
-    u32 w3;             // N >> 3
-    float h3;           // (N >> 3) * zTileNum
-    int zlo;            // first level-3 cell of the z-tile
+    uint4* info;
+    u32* bfs;       // (column start + z - zs) -> breadth-first index of the leaf node
+    u32 w3;         // N >> 3
+    u32 w3_shift;   // log2(w3)
 };
 
-__device__ inline int column_zs(const LeafColumns& lc, u32 col) {
-    return max(lc.zlo, (int)floorf(lc.lv3[col].x * lc.h3));
-}
+struct ColCountIn {
+    const uint4* info;
+    __device__ u32 operator()(u32 i) const { return info[i].x; }
+};
+struct ColStartOut {
+    uint4* info;
+    __device__ void operator()(u32 i, u32 ex, u32) const { info[i].y = ex; }
+};
 
 // Scan functors: partial-child count in, (first child index, children coordinates) out.
 struct ChildCountIn {
@@ -129,7 +134,8 @@ struct EmitChildrenOut {
             if (j < child_cap) {
                 if (cols.bfs) {
                     const u32 col = cy * cols.w3 + cx;
-                    const u32 slot = cols.start[col] + (u32)((int)cz - column_zs(cols, col));
+                    const uint4 ci = cols.info[col];
+                    const u32 slot = ci.y + (cz - ci.z);
                     if (slot < child_cap) cols.bfs[slot] = j;
                 } else {
                     child_coord[j] = pack_xyz(2 * cx, 2 * cy, 2 * cz);
@@ -177,28 +183,27 @@ struct LeafHash {
 // The nodes of one column, one texel per lane, in z order.  Only a run's first node stores its
 // mask, leafmasks and key hash: the merge and the final write read those of first occurrences
 // only, and a key's first occurrence is always the first node of a run (breadth-first order
-// grows with z inside a column).  The other nodes of a run store just the index of the run's
-// first node.  first_chunk holds the breadth-first indices of the column's first 64 nodes.
+// grows with z inside a column).  Every node stores the breadth-first index of its run's first
+// node and the slot of that node's leafmasks.
+// Leafmasks go to the column's own slots (start + q: a column's nodes are contiguous in z), an
+// address that needs no breadth-first index, so the load of those indices overlaps the node loop;
+// the per-node words are stored by breadth-first index once per 64-node chunk.
 // f is this lane's floor(d*N*T - 0.5); fmin and fmax are its minimum and maximum over the column:
 // min(c) and max(c) of node z3 are clamp(fmin - 8*z3 + 1, 0, 8) and the same for fmax.
 __device__ __forceinline__ void leaf_column(const LeafColumns& lc, u32 cnt, u32 base, int zs, int f, int fmin, int fmax,
-                                            u32 first_chunk, u32 lane, u32 nleaf, u32* __restrict__ mask_out,
-                                            u64* __restrict__ leafm, u64* __restrict__ hkey, u32* __restrict__ runhead) {
-    u32 head = 0;
+                                            u32 lane, u32 nleaf, u32* __restrict__ mask_out, u64* __restrict__ leafm,
+                                            u64* __restrict__ hkey, u32* __restrict__ runhead,
+                                            u32* __restrict__ leafslot) {
+    u32 headq = 0;     // column position of the current run's first node
+    u32 head_bfs = 0;  // its breadth-first index (a run can span chunks)
     for (u32 q0 = 0; q0 < cnt; q0 += 64) {
-        // Breadth-first indices of the next 64 nodes, one per lane.  Reading lane 0 here makes the
-        // wait for a load happen once, not in every iteration of the node loop below (whose
-        // stores would otherwise be waited for too).
-        const u32 bfs_chunk = q0 ? ((q0 + lane < cnt) ? lc.bfs[base + q0 + lane] : 0u) : first_chunk;
-        u32 b = (u32)__builtin_amdgcn_readfirstlane((int)bfs_chunk);
         const u32 qn = min(cnt - q0, 64u);
-        // Lane j collects node q0 + j's run head, and its mask and hash when it heads a run; they
-        // are stored once per chunk.
-        u32 my_head = 0, my_mask = 0;
+        const u32 bfs_chunk = (lane < qn) ? lc.bfs[base + q0 + lane] : 0u;  // first read after the node loop
+        // Lane j collects node q0 + j's run head, and its mask and hash when it heads a run.
+        u32 my_headq = 0, my_mask = 0;
         u64 my_hash = 0;
         bool my_first = false;
         for (u32 qq = 0; qq < qn; ++qq) {
-            if (qq) b = (u32)__builtin_amdgcn_readlane((int)bfs_chunk, (int)qq);
             const u32 q = q0 + qq;
             const int z8 = 8 * (zs + (int)q);
             const int u = f - z8;
@@ -216,18 +221,25 @@ __device__ __forceinline__ void leaf_column(const LeafColumns& lc, u32 cnt, u32 
                     h.add(m);
                     mine = (lane == j) ? m : mine;
                 }
-                head = b;
-                if (lane < k && b < nleaf) leafm[(size_t)b * 8 + lane] = mine;
+                headq = q;
+                const u32 slot = base + q;  // < nleaf: the column counts sum to the leaf count
+                if (lane < k && slot < nleaf) leafm[(size_t)slot * 8 + lane] = mine;
                 const u64 hv = h.value();
                 my_mask = (lane == qq) ? mask : my_mask;
                 my_hash = (lane == qq) ? hv : my_hash;
                 my_first = (lane == qq) ? true : my_first;
             }
-            my_head = (lane == qq) ? head : my_head;
+            my_headq = (lane == qq) ? headq : my_headq;
         }
+        // The run head's breadth-first index: in this chunk, or carried from the previous one.
+        const int src = (int)my_headq - (int)q0;
+        const u32 in_chunk = (u32)__shfl((int)bfs_chunk, src >= 0 ? src : 0, kWave);
+        const u32 my_head = (src >= 0) ? in_chunk : head_bfs;
+        head_bfs = (u32)__shfl((int)my_head, (int)(qn - 1), kWave);
         // nleaf bounds every index (only exceeded if the column map were inconsistent).
         if (lane < qn && bfs_chunk < nleaf) {
             runhead[bfs_chunk] = my_head;
+            leafslot[bfs_chunk] = base + my_headq;
             if (my_first) {
                 mask_out[bfs_chunk] = my_mask;
                 hkey[bfs_chunk] = my_hash;
@@ -236,58 +248,34 @@ __device__ __forceinline__ void leaf_column(const LeafColumns& lc, u32 cnt, u32 
     }
 }
 
-// kLeafCols neighbouring columns per wave, all loads issued before any column is processed.
-// One column per wave measured fastest on MI355X (0.71 ms for the 8192^2 map against 0.76 ms
-// with two and 0.90 ms with four): the node loop of a column is a serial chain, and the chip
-// needs as many independent chains in flight as it can hold.
-constexpr u32 kLeafCols = 1;
-
-// One wave per workgroup: columns differ widely in length (one node to hundreds at box walls), and
-// with four waves per workgroup a long column held its group's slot (0.76 ms against 0.61 ms).
+// One column per 64-thread workgroup.  Measured on MI355X: one column per wave beats two and four
+// (the node loop of a column is a serial chain, and the chip needs as many independent chains in
+// flight as it can hold), and one-wave workgroups beat four-wave ones (columns differ widely in
+// length, from one node to hundreds at box walls, and a long column held its group's slot).
 // Workgroups are dealt to the XCDs in turn (block b to XCD b mod kXcds), each XCD with its own L2.
-// XCD x takes the contiguous workgroups [x*chunk, (x+1)*chunk): neighbouring columns, whose 8-texel
+// XCD x takes the contiguous columns [x*chunk, (x+1)*chunk): neighbouring columns, whose 8-texel
 // depth rows share 128-byte lines, are then read through one L2 instead of four.
 constexpr u32 kXcds = 8;
 
-__global__ __launch_bounds__(64) void k_leaf_columns(LeafColumns lc, const u32* __restrict__ colcnt, u32 ncol,
-                                                      u32 nleaf, const float* __restrict__ depth, u32 N, u32 T,
-                                                      u32* __restrict__ mask_out, u64* __restrict__ leafm,
-                                                      u64* __restrict__ hkey, u32* __restrict__ runhead, u32 xcd_chunk) {
+__global__ __launch_bounds__(64) void k_leaf_columns(LeafColumns lc, u32 ncol, u32 nleaf, const float* __restrict__ depth,
+                                                      u32 N, u32 T, u32* __restrict__ mask_out, u64* __restrict__ leafm,
+                                                      u64* __restrict__ hkey, u32* __restrict__ runhead,
+                                                      u32* __restrict__ leafslot, u32 xcd_chunk) {
     const u32 lane = threadIdx.x & 63;
-    const u32 wg = (blockIdx.x % kXcds) * xcd_chunk + blockIdx.x / kXcds;
-    const u32 col0 = __builtin_amdgcn_readfirstlane((wg * (blockDim.x / 64) + (threadIdx.x >> 6)) * kLeafCols);
-    if (col0 >= ncol) return;  // uniform per wave
-    u32 cnt[kLeafCols], base[kLeafCols], chunk[kLeafCols];
-    int zs[kLeafCols];
-    float d[kLeafCols];
-#pragma unroll
-    for (u32 j = 0; j < kLeafCols; ++j) {
-        const u32 col = col0 + j;
-        cnt[j] = (col < ncol) ? colcnt[col] : 0u;
-        base[j] = (col < ncol) ? lc.start[col] : 0u;
-        zs[j] = (col < ncol) ? column_zs(lc, col) : 0;
-    }
-#pragma unroll
-    for (u32 j = 0; j < kLeafCols; ++j) {
-        const u32 col = col0 + j;
-        const u32 x3 = col % lc.w3, y3 = col / lc.w3;
-        d[j] = cnt[j] ? depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)] : 0.0f;
-        chunk[j] = (lane < cnt[j]) ? lc.bfs[base[j] + lane] : 0u;
-    }
-    int f[kLeafCols], fmin[kLeafCols], fmax[kLeafCols];
-#pragma unroll
-    for (u32 j = 0; j < kLeafCols; ++j) {
-        const float D = d[j] * (float)(N * T);
-        f[j] = -(1 << 30);  // NaN is never lit
-        if (D == D) f[j] = (int)floorf(fminf(fmaxf(D, -2.0f), 4194304.0f) - 0.5f);  // exact for |D| < 2^22
-        fmin[j] = __ockl_wfred_min_i32(f[j]);
-        fmax[j] = __ockl_wfred_max_i32(f[j]);
-    }
-#pragma unroll
-    for (u32 j = 0; j < kLeafCols; ++j)
-        if (cnt[j])
-            leaf_column(lc, cnt[j], base[j], zs[j], f[j], fmin[j], fmax[j], chunk[j], lane, nleaf, mask_out, leafm, hkey,
-                        runhead);
+    const u32 col = __builtin_amdgcn_readfirstlane((blockIdx.x % kXcds) * xcd_chunk + blockIdx.x / kXcds);
+    if (col >= ncol) return;  // uniform per wave
+    const u32 x3 = col & (lc.w3 - 1), y3 = col >> lc.w3_shift;
+    // The column's two loads are independent and issued together (no early exit for an empty
+    // column: its node loop simply does not run).
+    const float d = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
+    const uint4 ci = lc.info[col];
+    const u32 cnt = ci.x, base = ci.y;
+    const int zs = (int)ci.z;
+    const float D = d * (float)(N * T);
+    int f = -(1 << 30);  // NaN is never lit
+    if (D == D) f = (int)floorf(fminf(fmaxf(D, -2.0f), 4194304.0f) - 0.5f);  // exact for |D| < 2^22
+    const int fmin = __ockl_wfred_min_i32(f), fmax = __ockl_wfred_max_i32(f);
+    leaf_column(lc, cnt, base, zs, f, fmin, fmax, lane, nleaf, mask_out, leafm, hkey, runhead, leafslot);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -317,8 +305,9 @@ __device__ inline u64 inner_key_hash(const u32* mask, const u32* cstart, const u
 
 struct LeafEq {
     const u32* mask;
-    const u64* leafm;
-    const u32* runhead;  // when set, a node's data is read from its run's first node
+    const u64* leafm;     // indexed by leafmask slot
+    const u32* leafslot;  // node -> leafmask slot of its run's first node
+    const u32* runhead;   // when set, a node's mask is read from its run's first node
     __device__ bool operator()(u32 a, u32 b) const {
         if (runhead) {
             a = runhead[a];
@@ -328,8 +317,8 @@ struct LeafEq {
         const u32 m = mask[a];
         if (m != mask[b]) return false;
         const u32 k = num_children(m);
-        const u64* pa = leafm + (size_t)a * 8;
-        const u64* pb = leafm + (size_t)b * 8;
+        const u64* pa = leafm + (size_t)leafslot[a] * 8;
+        const u64* pb = leafm + (size_t)leafslot[b] * 8;
         bool eq = true;
 #pragma unroll
         for (u32 j = 0; j < 8; ++j)
@@ -595,8 +584,8 @@ __global__ __launch_bounds__(256) void k_write_inner(const u32* __restrict__ uni
 }
 
 __global__ __launch_bounds__(256) void k_write_leaf(const u32* __restrict__ uniq, u32 n_uniq, const u32* __restrict__ mask,
-                                                    const u64* __restrict__ leafm, const u32* __restrict__ off, u32 base,
-                                                    u32* __restrict__ out) {
+                                                    const u64* __restrict__ leafm, const u32* __restrict__ leafslot,
+                                                    const u32* __restrict__ off, u32 base, u32* __restrict__ out) {
     const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_uniq) return;
     const u32 i = uniq[r];
@@ -604,7 +593,7 @@ __global__ __launch_bounds__(256) void k_write_leaf(const u32* __restrict__ uniq
     const u32 m = mask[i];
     out[o] = m;
     const u32 k = num_children(m);
-    const u64* p = leafm + (size_t)i * 8;
+    const u64* p = leafm + (size_t)leafslot[i] * 8;
     u64 v[8];
 #pragma unroll
     for (u32 j = 0; j < 8; ++j) v[j] = (j < k) ? p[j] : 0ull;
@@ -885,12 +874,12 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     const u32 w3 = leaf ? (N >> 3) : 0;
     const u32 ncol = w3 * w3;
     const size_t meta_bytes = arena_bytes(kMaxLevels, sizeof(unsigned long long)) + arena_bytes(kMetaWords, sizeof(u32));
-    const size_t col_bytes = leaf ? arena_bytes(ncol, 4) : 0;
+    const size_t col_bytes = leaf ? arena_bytes(ncol, sizeof(uint4)) : 0;
     int rc = A.reserve(meta_bytes + col_bytes, s);
     if (rc) return rc;
     unsigned long long* d_counts = A.take<unsigned long long>(kMaxLevels);
-    u32* colcnt = leaf ? A.take<u32>(ncol) : nullptr;
-    rc = pyramid_node_counts(p, t, T, d_counts, colcnt, s);
+    uint4* colinfo = leaf ? A.take<uint4>(ncol) : nullptr;
+    rc = pyramid_node_counts(p, t, T, d_counts, colinfo, s);
     if (rc) return rc;
     CPVS_HIP_CHECK(hipMemcpyAsync(pinned, d_counts, sizeof(u64) * kMaxLevels, hipMemcpyDeviceToHost, s));
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
@@ -915,7 +904,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         need += arena_bytes(c, 8) + 5 * arena_bytes(c, 4);
     }
     const size_t c2 = leaf ? (cnt[2] ? cnt[2] : 1) : 1;
-    if (leaf) need += arena_bytes(c2 * 8, 8) + arena_bytes(c2, 8) + 2 * arena_bytes(c2, 4) + arena_bytes(ncol, 4);
+    if (leaf) need += arena_bytes(c2 * 8, 8) + arena_bytes(c2, 8) + 3 * arena_bytes(c2, 4);
     need += 2 * arena_bytes(cmax, 4);
     if (use_table) need += arena_bytes(S, 8) + arena_bytes(S, 4);
     else need += 4 * arena_bytes(cmax, 4) + arena_bytes(sort_scratch_words(cmax), 4);
@@ -934,10 +923,10 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     rc = A.reserve(need, s, &moved);  // keeps the contents when large enough
     if (rc) return rc;
     (void)A.take<unsigned long long>(kMaxLevels);
-    colcnt = leaf ? A.take<u32>(ncol) : nullptr;
+    colinfo = leaf ? A.take<uint4>(ncol) : nullptr;
     u32* meta = A.take<u32>(kMetaWords);
     if (leaf && moved) {  // the arena was reallocated: count the columns again
-        rc = pyramid_node_counts(p, t, T, reinterpret_cast<unsigned long long*>(A.base), colcnt, s);
+        rc = pyramid_node_counts(p, t, T, reinterpret_cast<unsigned long long*>(A.base), colinfo, s);
         if (rc) return rc;
     }
     CPVS_HIP_CHECK(hipMemsetAsync(meta, 0, sizeof(u32) * kMetaWords, s));
@@ -956,8 +945,8 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     u64* leafm = leaf ? A.take<u64>(c2 * 8) : nullptr;
     u64* hkey = leaf ? A.take<u64>(c2) : nullptr;
     u32* runhead = leaf ? A.take<u32>(c2) : nullptr;
+    u32* leafslot = leaf ? A.take<u32>(c2) : nullptr;
     u32* colbfs = leaf ? A.take<u32>(c2) : nullptr;
-    u32* colstart = leaf ? A.take<u32>(ncol) : nullptr;
     u32* rep = A.take<u32>(cmax);
     u32* tslot = A.take<u32>(cmax);  // table slot of each node, or segment starts when sorting
     u32* seg = tslot;
@@ -979,13 +968,11 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
 
     LeafColumns lc{};
     if (leaf) {
-        lc.lv3 = p->lv[3];
-        lc.start = colstart;
+        lc.info = colinfo;
         lc.bfs = colbfs;
         lc.w3 = w3;
-        lc.h3 = (float)(w3 * T);
-        lc.zlo = (int)(t * w3);
-        device_scan(ArrIn{colcnt}, ArrOutExclusive{colstart}, ncol, OpSum{}, nullptr, scan_scratch, s);
+        lc.w3_shift = (u32)__builtin_ctz(w3);
+        device_scan(ColCountIn{colinfo}, ColStartOut{colinfo}, ncol, OpSum{}, nullptr, scan_scratch, s);
     }
 
     // --- 1/2. BFS classification -------------------------------------------------------------
@@ -1019,10 +1006,9 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     for (int L = top_big; L >= deepest; --L) {
         const u32 c = lv[L].count;
         if (leaf && L == 2) {
-            const u32 groups = ceil_div<u32>(ncol, kLeafCols);  // one wave (workgroup) per kLeafCols columns
-            const u32 chunk = ceil_div<u32>(groups, kXcds);
-            CPVS_LAUNCH("leaf_columns", k_leaf_columns, dim3(chunk * kXcds), dim3(64), 0, s, lc, colcnt, ncol, c, p->depth,
-                        N, T, lv[2].mask, leafm, hkey, runhead, chunk);
+            const u32 chunk = ceil_div<u32>(ncol, kXcds);  // one column per workgroup
+            CPVS_LAUNCH("leaf_columns", k_leaf_columns, dim3(chunk * kXcds), dim3(64), 0, s, lc, ncol, c, p->depth, N, T,
+                        lv[2].mask, leafm, hkey, runhead, leafslot, chunk);
             break;
         }
         CPVS_LAUNCH("classify", k_classify, grid1d(c), dim3(256), 0, s, lv[L].coord, c, p->lv[L], p->depth, N, L, T,
@@ -1047,7 +1033,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             if (is_leaf) {
                 CPVS_LAUNCH("insert_leaf", k_insert_leaf, grid1d(c), dim3(256), 0, s, hkey, runhead, c, th, ti, Sl, tslot);
                 CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c), dim3(256), 0, s, tslot, ti, runhead, c,
-                            LeafEq{lv[2].mask, leafm, nullptr}, rep, meta + kMetaColl);
+                            LeafEq{lv[2].mask, leafm, leafslot, nullptr}, rep, meta + kMetaColl);
             } else {
                 CPVS_LAUNCH("insert_inner", k_insert_inner, grid1d(c), dim3(256), 0, s, lv[L].mask, lv[L].cstart, child_rank, c,
                             th, ti, Sl, tslot);
@@ -1064,7 +1050,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             device_scan(SegHeadIn{skey}, SegStartOut{seg}, c, OpMax{}, nullptr, scan_scratch, s);
             if (is_leaf)
                 CPVS_LAUNCH("representative", k_representative<LeafEq>, grid1d(c), dim3(256), 0, s, sval, seg, c,
-                            LeafEq{lv[2].mask, leafm, runhead}, rep, meta + kMetaSortColl);
+                            LeafEq{lv[2].mask, leafm, leafslot, runhead}, rep, meta + kMetaSortColl);
             else
                 CPVS_LAUNCH("representative", k_representative<InnerEq>, grid1d(c), dim3(256), 0, s, sval, seg, c,
                             InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaSortColl);
@@ -1127,7 +1113,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         const u32 U = hm[kMetaU + L];
         const u32 base = hm[kMetaB + L];
         if (leaf && L == 2) {
-            CPVS_LAUNCH("write_leaf", k_write_leaf, grid1d(U), dim3(256), 0, s, lv[2].uniq, U, lv[2].mask, leafm, lv[2].off,
+            CPVS_LAUNCH("write_leaf", k_write_leaf, grid1d(U), dim3(256), 0, s, lv[2].uniq, U, lv[2].mask, leafm, leafslot, lv[2].off,
                         base, d_words);
         } else {
             const bool has_children = L > deepest;

@@ -110,8 +110,9 @@ namespace cpvs {
 // arena (p then does not own them; the arena is reset).
 int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* into = nullptr);
 // Exact node counts per level (counts[L] = nodes at octree level L) for z-tile t of T.
-// col_count (may be null) receives the partial-cell count of every level-3 texel (leafmask mode).
-int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, u32* col_count, hipStream_t s);
+// col_info (may be null) receives, per level-3 texel (leafmask mode), its partial-cell count (.x)
+// and first partial z cell within the z-tile (.z).
+int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, uint4* col_info, hipStream_t s);
 
 // Admission hook of a build: called once the exact node counts are known and before any scratch
 // or DAG memory is taken, with the scratch bytes the build needs (its arena grows only if it

@@ -205,7 +205,8 @@ struct CountArgs {
     const float2* lv[kMaxLevels];
     u32 N, T, t;
     unsigned long long* counts;  // counts[L-1] += partial cells of level L
-    u32* col_count;              // per-texel partial count of level 3 (leaf columns), may be null
+    uint4* col_info;             // per level-3 texel (leaf column): x = partial cells, z = first partial z
+                                 // (y is left for the column's start); may be null
 };
 
 constexpr u32 kCountPerThread = 8;
@@ -219,7 +220,7 @@ __global__ __launch_bounds__(256) void k_node_counts(CountArgs a) {
     const int L = a.level[k];
     const float h = (float)((a.N >> L) * a.T);
     const int zlo = (int)(a.t * (a.N >> L)), zhi = (int)((a.t + 1) * (a.N >> L));
-    u32* colc = (L == 3) ? a.col_count : nullptr;
+    uint4* colc = (L == 3) ? a.col_info : nullptr;
     const u32 base = (blockIdx.x - a.block_start[k]) * kCountTile;
     u32 acc = 0;
 #pragma unroll
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(256) void k_node_counts(CountArgs a) {
             const float lo = floorf(v.x * h), hi = ceilf(v.y * h);
             const int s = max(zlo, (int)lo), e = min(zhi, (int)hi);
             const u32 c = (e > s) ? (u32)(e - s) : 0u;
-            if (colc) colc[j] = c;
+            if (colc) colc[j] = make_uint4(c, 0u, (u32)max(s, 0), 0u);
             acc += c;
         }
     }
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(256) void k_node_counts(CountArgs a) {
     }
 }
 
-int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, u32* col_count, hipStream_t s) {
+int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, uint4* col_info, hipStream_t s) {
     CountArgs a = {};
     const int n = p->n;
     const int R = n - 2;
@@ -254,7 +255,7 @@ int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long*
     a.T = T;
     a.t = t;
     a.counts = d_counts;
-    a.col_count = (n >= 5) ? col_count : nullptr;
+    a.col_info = (n >= 5) ? col_info : nullptr;
     u32 blocks = 0;
     for (int L = Lmin; L <= R; ++L) {
         const u32 w = p->N >> L;
