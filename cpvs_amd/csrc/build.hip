@@ -373,20 +373,38 @@ __device__ inline u32 table_insert(u64* th, u32* ti, u32 S, u64 h, u32 i) {
     }
 }
 
-// Neighbouring nodes in breadth-first order are often equal (siblings in flat regions), and a
-// table slot hit by many threads at once serialises their atomics.  So within a wave only the
-// first lane of each run of equal hashes in adjacent active lanes touches the table; the others
-// take its slot (the run's first lane has the smallest index).  Every lane of the wave calls this.
-__device__ inline u32 table_insert_runs(u64* th, u32* ti, u32 S, u64 h, u32 i, bool active) {
-    const u32 lane = threadIdx.x & (kWave - 1);
-    const u64 hp = __shfl_up(h, 1, kWave);
-    const int ap = __shfl_up(active ? 1 : 0, 1, kWave);
-    const bool head = active && (lane == 0 || !ap || hp != h);
-    u32 slot = head ? table_insert(th, ti, S, h, i) : 0u;
-    const u64 heads = __ballot(head);
-    const u64 upto = heads & ((lane == kWave - 1) ? ~0ull : ((2ull << lane) - 1ull));
-    const int src = upto ? 63 - __builtin_clzll(upto) : (int)lane;
-    return (u32)__shfl((int)slot, src, kWave);
+// A table slot hit by many threads at once serialises their atomics, and equal keys are common
+// (flat regions repeat the same subtrees all over a level; a level of a few hundred thousand nodes
+// runs almost all at once, before any first occurrence has landed).  So each 256-thread workgroup
+// first merges its own equal hashes in an LDS table: only the smallest index of each distinct
+// hash in the group touches the global table, and the others take its slot.  Every thread of the
+// workgroup calls this (it has barriers).
+constexpr u32 kGroupSlots = 512;
+
+__device__ inline u32 table_insert_group(u64* th, u32* ti, u32 S, u64 h, u32 i, bool active) {
+    __shared__ u64 gh[kGroupSlots];
+    __shared__ u32 gi[kGroupSlots];
+    __shared__ u32 gs[kGroupSlots];
+    for (u32 j = threadIdx.x; j < kGroupSlots; j += blockDim.x) {
+        gh[j] = 0ull;
+        gi[j] = kNoRep;
+    }
+    __syncthreads();
+    if (h == 0ull) h = 1ull;  // 0 marks an empty slot (as in the global table)
+    u32 ls = 0;
+    if (active) {
+        ls = (u32)(h ^ (h >> 29)) & (kGroupSlots - 1);
+        for (;;) {
+            const u64 cur = atomicCAS(reinterpret_cast<unsigned long long*>(gh + ls), 0ull, (unsigned long long)h);
+            if (cur == 0ull || cur == h) break;
+            ls = (ls + 1) & (kGroupSlots - 1);
+        }
+        atomicMin(gi + ls, i);
+    }
+    __syncthreads();
+    if (active && gi[ls] == i) gs[ls] = table_insert(th, ti, S, h, i);
+    __syncthreads();
+    return active ? gs[ls] : 0u;
 }
 
 __global__ __launch_bounds__(256) void k_insert_leaf(const u64* __restrict__ hkey, const u32* __restrict__ runhead, u32 n,
@@ -394,7 +412,7 @@ __global__ __launch_bounds__(256) void k_insert_leaf(const u64* __restrict__ hke
                                                      u32* __restrict__ tslot) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < n && runhead[i] == i;  // a run's first node has the run's smallest index
-    const u32 slot = table_insert_runs(th, ti, S, act ? hkey[i] : 0ull, i, act);
+    const u32 slot = table_insert_group(th, ti, S, act ? hkey[i] : 0ull, i, act);
     if (act) tslot[i] = slot;
 }
 
@@ -403,7 +421,7 @@ __global__ __launch_bounds__(256) void k_insert_inner(const u32* __restrict__ ma
                                                       u32* __restrict__ ti, u32 S, u32* __restrict__ tslot) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < n;
-    const u32 slot = table_insert_runs(th, ti, S, act ? inner_key_hash(mask, cstart, child_rank, i) : 0ull, i, act);
+    const u32 slot = table_insert_group(th, ti, S, act ? inner_key_hash(mask, cstart, child_rank, i) : 0ull, i, act);
     if (act) tslot[i] = slot;
 }
 
