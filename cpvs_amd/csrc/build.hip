@@ -516,6 +516,40 @@ struct FirstRankOut {
     }
 };
 
+// Ranks and word offsets in one scan: first occurrences in index order are the uniques in rank
+// order, so a scan of (first ? 1 : 0, first ? node size : 0) -- packed as a u64 whose halves
+// cannot carry (size sums stay below 2^32) -- gives each first occurrence its rank and its offset.
+struct FirstSizeIn {
+    const u32* rep;
+    const u32* mask;
+    u32 leaf;  // 1 at the leafmask level (two words per partial child)
+    __device__ u64 operator()(u32 i) const {
+        if (rep[i] != i) return 0ull;
+        return (1ull << 32) | (u64)(1u + (num_children(mask[i]) << leaf));
+    }
+};
+struct FirstSizeOut {
+    u32* rank;
+    u32* uniq;
+    u32* off;
+    u32* n_uniq;  // device: the level's unique count
+    u32* words;   // device: the level's word count
+    u32 n;
+    __device__ void operator()(u32 i, u64 ex, u64 v) const {
+        if (v) {
+            const u32 r = (u32)(ex >> 32);
+            rank[i] = r;
+            uniq[r] = i;
+            off[r] = (u32)ex;
+        }
+        if (i == n - 1) {
+            const u64 t = ex + v;
+            *n_uniq = (u32)(t >> 32);
+            *words = (u32)t;
+        }
+    }
+};
+
 __global__ __launch_bounds__(256) void k_rank_copies(const u32* __restrict__ rep, const u32* __restrict__ runhead, u32 count,
                                                      u32* __restrict__ rank) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1039,6 +1073,13 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     }
     CPVS_HIP_CHECK(hipGetLastError());
 
+    // Levels merged below (not the root, not the small levels) whose word count surely fits 32
+    // bits get ranks and offsets from one 64-bit scan.
+    auto fused_sizes = [&](int L) {
+        const u64 per_node = (leaf && L == 2) ? 17ull : 9ull;
+        return L >= deepest && L < std::min(R, Ls) && (u64)lv[L].count * per_node < 0xFFFFFFFFull;
+    };
+
     // --- 3. merge, bottom-up (the root is never merged) --------------------------------------
     for (int L = deepest; L < std::min(R, Ls); ++L) {
         const u32 c = lv[L].count;
@@ -1074,7 +1115,12 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
                             InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaSortColl);
             rh = nullptr;  // every node got its representative directly
         }
-        device_scan(FirstFlagIn{rep}, FirstRankOut{lv[L].rank, lv[L].uniq}, c, OpSum{}, meta + kMetaU + L, scan_scratch, s);
+        if (fused_sizes(L))
+            device_scan(FirstSizeIn{rep, lv[L].mask, is_leaf ? 1u : 0u},
+                        FirstSizeOut{lv[L].rank, lv[L].uniq, lv[L].off, meta + kMetaU + L, meta + kMetaS + L, c}, c, OpSum64{},
+                        nullptr, scan_scratch, s);
+        else
+            device_scan(FirstFlagIn{rep}, FirstRankOut{lv[L].rank, lv[L].uniq}, c, OpSum{}, meta + kMetaU + L, scan_scratch, s);
         CPVS_LAUNCH("rank_copies", k_rank_copies, grid1d(c), dim3(256), 0, s, rep, rh, c, lv[L].rank);
     }
     if (Ls <= R)
@@ -1085,6 +1131,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
 
     // --- 4. compress: sizes, bases, one readback, write -------------------------------------
     for (int L = top_big; L >= deepest; --L) {
+        if (fused_sizes(L)) continue;  // offsets came with the ranks
         const u32 c = lv[L].count;
         const u32 is_leaf = (leaf && L == 2) ? 1u : 0u;
         device_scan(UniqueSizeIn{lv[L].uniq, lv[L].mask, meta + kMetaU + L, is_leaf},

@@ -1,4 +1,4 @@
-// scan.hpp -- device-wide scans over u32 with fused input/output functors.
+// scan.hpp -- device-wide scans (u32, or u64 pairs of sums) with fused input/output functors.
 //
 // Reduce-then-scan in three launches: per-tile reduction, a single-block scan of the tile
 // partials, and a per-tile scan that hands (index, exclusive prefix, value) to an output
@@ -14,28 +14,36 @@ constexpr int kScanThreads = 256;
 constexpr int kScanRounds = 8;
 constexpr u32 kScanTile = kScanThreads * kScanRounds;  // 2048 elements per block
 
+// Scan operators; T is the value type (u32, or u64 for a pair of 32-bit sums that cannot carry).
 struct OpSum {
-    static constexpr u32 identity = 0;
-    __device__ u32 operator()(u32 a, u32 b) const { return a + b; }
+    using T = u32;
+    static constexpr T identity = 0;
+    __device__ T operator()(T a, T b) const { return a + b; }
 };
 struct OpMax {
-    static constexpr u32 identity = 0;
-    __device__ u32 operator()(u32 a, u32 b) const { return a > b ? a : b; }
+    using T = u32;
+    static constexpr T identity = 0;
+    __device__ T operator()(T a, T b) const { return a > b ? a : b; }
+};
+struct OpSum64 {
+    using T = u64;
+    static constexpr T identity = 0;
+    __device__ T operator()(T a, T b) const { return a + b; }
 };
 
 template <class Op>
-__device__ inline u32 wave_inclusive_scan(u32 v, Op op) {
+__device__ inline typename Op::T wave_inclusive_scan(typename Op::T v, Op op) {
     const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
-        const u32 o = __shfl_up(v, d, kWave);
+        const typename Op::T o = __shfl_up(v, d, kWave);
         if (lane >= d) v = op(v, o);
     }
     return v;
 }
 
 template <class Op>
-__device__ inline u32 wave_reduce(u32 v, Op op) {
+__device__ inline typename Op::T wave_reduce(typename Op::T v, Op op) {
 #pragma unroll
     for (int d = kWave / 2; d >= 1; d >>= 1) v = op(v, __shfl_xor(v, d, kWave));
     return v;
@@ -44,32 +52,35 @@ __device__ inline u32 wave_reduce(u32 v, Op op) {
 // Exclusive scan of one value per thread over a 256-thread block; *total gets the block sum.
 // s_wave must hold kScanThreads/64 entries of shared memory.
 template <class Op>
-__device__ inline u32 block_exclusive_scan(u32 v, u32* s_wave, u32* total, Op op) {
+__device__ inline typename Op::T block_exclusive_scan(typename Op::T v, typename Op::T* s_wave, typename Op::T* total,
+                                                      Op op) {
+    using T = typename Op::T;
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    const u32 inc = wave_inclusive_scan(v, op);
+    const T inc = wave_inclusive_scan(v, op);
     if (lane == kWave - 1) s_wave[wid] = inc;
     __syncthreads();
-    u32 wpre = Op::identity, tot = Op::identity;
+    T wpre = Op::identity, tot = Op::identity;
 #pragma unroll
     for (int w = 0; w < kScanThreads / kWave; ++w) {
-        const u32 x = s_wave[w];
+        const T x = s_wave[w];
         if (w < wid) wpre = op(wpre, x);
         tot = op(tot, x);
     }
     __syncthreads();
-    u32 ex = __shfl_up(inc, 1, kWave);
+    T ex = __shfl_up(inc, 1, kWave);
     if (lane == 0) ex = Op::identity;
     *total = tot;
     return op(wpre, ex);
 }
 
 template <class Op>
-__device__ inline u32 block_reduce(u32 v, u32* s_wave, Op op) {
+__device__ inline typename Op::T block_reduce(typename Op::T v, typename Op::T* s_wave, Op op) {
+    using T = typename Op::T;
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     v = wave_reduce(v, op);
     if (lane == 0) s_wave[wid] = v;
     __syncthreads();
-    u32 tot = Op::identity;
+    T tot = Op::identity;
 #pragma unroll
     for (int w = 0; w < kScanThreads / kWave; ++w) tot = op(tot, s_wave[w]);
     __syncthreads();
@@ -77,29 +88,32 @@ __device__ inline u32 block_reduce(u32 v, u32* s_wave, Op op) {
 }
 
 template <class In, class Op>
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(In in, u32 n, u32* partials, Op op) {
-    __shared__ u32 s_wave[kScanThreads / kWave];
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(In in, u32 n, typename Op::T* partials, Op op) {
+    using T = typename Op::T;
+    __shared__ T s_wave[kScanThreads / kWave];
     const u32 base = blockIdx.x * kScanTile;
-    u32 acc = Op::identity;
+    T acc = Op::identity;
 #pragma unroll
     for (int r = 0; r < kScanRounds; ++r) {
         const u32 i = base + r * kScanThreads + threadIdx.x;
         if (i < n) acc = op(acc, in(i));
     }
-    const u32 tot = block_reduce(acc, s_wave, op);
+    const T tot = block_reduce(acc, s_wave, op);
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
 // Exclusive scan of the tile partials in place (one block); *total = overall reduction.
 template <class Op>
-__global__ __launch_bounds__(kScanThreads) void k_scan_partials(u32* partials, u32 nparts, u32* total, Op op) {
-    __shared__ u32 s_wave[kScanThreads / kWave];
-    u32 carry = Op::identity;
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(typename Op::T* partials, u32 nparts, typename Op::T* total,
+                                                                Op op) {
+    using T = typename Op::T;
+    __shared__ T s_wave[kScanThreads / kWave];
+    T carry = Op::identity;
     for (u32 base = 0; base < nparts; base += kScanThreads) {
         const u32 i = base + threadIdx.x;
-        const u32 v = (i < nparts) ? partials[i] : Op::identity;
-        u32 tot;
-        const u32 ex = block_exclusive_scan(v, s_wave, &tot, op);
+        const T v = (i < nparts) ? partials[i] : Op::identity;
+        T tot;
+        const T ex = block_exclusive_scan(v, s_wave, &tot, op);
         if (i < nparts) partials[i] = op(carry, ex);
         carry = op(carry, tot);
     }
@@ -107,17 +121,18 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(u32* partials, u
 }
 
 template <class In, class Out, class Op>
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(In in, Out out, u32 n, const u32* partials, Op op) {
-    __shared__ u32 s_wave[kScanThreads / kWave];
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(In in, Out out, u32 n, const typename Op::T* partials, Op op) {
+    using T = typename Op::T;
+    __shared__ T s_wave[kScanThreads / kWave];
     const u32 base = blockIdx.x * kScanTile;
-    u32 carry = partials[blockIdx.x];
+    T carry = partials[blockIdx.x];
     for (int r = 0; r < kScanRounds; ++r) {
         const u32 r0 = base + r * kScanThreads;
         if (r0 >= n) break;  // uniform across the block
         const u32 i = r0 + threadIdx.x;
-        const u32 v = (i < n) ? in(i) : Op::identity;
-        u32 tot;
-        const u32 ex = block_exclusive_scan(v, s_wave, &tot, op);
+        const T v = (i < n) ? in(i) : Op::identity;
+        T tot;
+        const T ex = block_exclusive_scan(v, s_wave, &tot, op);
         if (i < n) out(i, op(carry, ex), v);
         carry = op(carry, tot);
     }
@@ -125,16 +140,17 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(In in, Out out, u32 
 
 // Single-block scan for n <= kScanTile: one launch.
 template <class In, class Out, class Op>
-__global__ __launch_bounds__(kScanThreads) void k_scan_single(In in, Out out, u32 n, u32* total, Op op) {
-    __shared__ u32 s_wave[kScanThreads / kWave];
-    u32 carry = Op::identity;
+__global__ __launch_bounds__(kScanThreads) void k_scan_single(In in, Out out, u32 n, typename Op::T* total, Op op) {
+    using T = typename Op::T;
+    __shared__ T s_wave[kScanThreads / kWave];
+    T carry = Op::identity;
     for (int r = 0; r < kScanRounds; ++r) {
         const u32 r0 = r * kScanThreads;
         if (r0 >= n) break;  // uniform across the block
         const u32 i = r0 + threadIdx.x;
-        const u32 v = (i < n) ? in(i) : Op::identity;
-        u32 tot;
-        const u32 ex = block_exclusive_scan(v, s_wave, &tot, op);
+        const T v = (i < n) ? in(i) : Op::identity;
+        T tot;
+        const T ex = block_exclusive_scan(v, s_wave, &tot, op);
         if (i < n) out(i, op(carry, ex), v);
         carry = op(carry, tot);
     }
@@ -146,26 +162,27 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_single(In in, Out out, u3
 constexpr u32 kScanPartialsInline = 4 * kScanThreads;
 
 template <class In, class Out, class Op>
-__global__ __launch_bounds__(kScanThreads) void k_scan_down_inline(In in, Out out, u32 n, const u32* partials, u32 nparts,
-                                                                   u32* total, Op op) {
-    __shared__ u32 s_wave[kScanThreads / kWave];
-    u32 pre = Op::identity, all = Op::identity;
+__global__ __launch_bounds__(kScanThreads) void k_scan_down_inline(In in, Out out, u32 n, const typename Op::T* partials,
+                                                                   u32 nparts, typename Op::T* total, Op op) {
+    using T = typename Op::T;
+    __shared__ T s_wave[kScanThreads / kWave];
+    T pre = Op::identity, all = Op::identity;
     for (u32 k = threadIdx.x; k < nparts; k += kScanThreads) {
-        const u32 v = partials[k];
+        const T v = partials[k];
         if (k < blockIdx.x) pre = op(pre, v);
         all = op(all, v);
     }
-    u32 carry = block_reduce(pre, s_wave, op);
-    const u32 grand = block_reduce(all, s_wave, op);
+    T carry = block_reduce(pre, s_wave, op);
+    const T grand = block_reduce(all, s_wave, op);
     if (blockIdx.x == 0 && threadIdx.x == 0 && total) *total = grand;
     const u32 base = blockIdx.x * kScanTile;
     for (int r = 0; r < kScanRounds; ++r) {
         const u32 r0 = base + r * kScanThreads;
         if (r0 >= n) break;  // uniform across the block
         const u32 i = r0 + threadIdx.x;
-        const u32 v = (i < n) ? in(i) : Op::identity;
-        u32 tot;
-        const u32 ex = block_exclusive_scan(v, s_wave, &tot, op);
+        const T v = (i < n) ? in(i) : Op::identity;
+        T tot;
+        const T ex = block_exclusive_scan(v, s_wave, &tot, op);
         if (i < n) out(i, op(carry, ex), v);
         carry = op(carry, tot);
     }
@@ -181,27 +198,29 @@ struct ArrOutExclusive {
     __device__ void operator()(u32 i, u32 ex, u32) const { a[i] = ex; }
 };
 
-// Scratch words needed by device_scan for n elements.
-inline size_t scan_scratch_words(u32 n) { return (size_t)ceil_div<u32>(n ? n : 1, kScanTile) + 1; }
+// Scratch words (u32) needed by device_scan for n elements (room for 64-bit partials).
+inline size_t scan_scratch_words(u32 n) { return 2 * ((size_t)ceil_div<u32>(n ? n : 1, kScanTile) + 1); }
 
 // out(i, exclusive_prefix_i, value_i) for all i < n; *d_total (device, may be null) gets the
-// reduction of all values.  scratch: scan_scratch_words(n) u32 of device memory.
+// reduction of all values.  scratch: scan_scratch_words(n) u32 of device memory (8-byte aligned).
 // One launch for n <= 2048, two up to 2M elements, three beyond.
 template <class In, class Out, class Op>
-inline void device_scan(In in, Out out, u32 n, Op op, u32* d_total, u32* scratch, hipStream_t s) {
+inline void device_scan(In in, Out out, u32 n, Op op, typename Op::T* d_total, u32* scratch, hipStream_t s) {
+    using T = typename Op::T;
+    T* part = reinterpret_cast<T*>(scratch);
     if (n <= kScanTile) {
         CPVS_LAUNCH("scan_single", (k_scan_single<In, Out, Op>), dim3(1), dim3(kScanThreads), 0, s, in, out, n, d_total, op);
         return;
     }
     const u32 nb = ceil_div<u32>(n, kScanTile);
-    CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, n, scratch, op);
+    CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, n, part, op);
     if (nb <= kScanPartialsInline) {
-        CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, scratch,
-                    nb, d_total, op);
+        CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, part, nb,
+                    d_total, op);
         return;
     }
-    CPVS_LAUNCH("scan_partials", k_scan_partials<Op>, dim3(1), dim3(kScanThreads), 0, s, scratch, nb, d_total, op);
-    CPVS_LAUNCH("scan_down", (k_scan_down<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, scratch, op);
+    CPVS_LAUNCH("scan_partials", k_scan_partials<Op>, dim3(1), dim3(kScanThreads), 0, s, part, nb, d_total, op);
+    CPVS_LAUNCH("scan_down", (k_scan_down<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, part, op);
 }
 
 // ------------------------------------------------------------------------------------------
