@@ -42,12 +42,12 @@ def pct(xs, p):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-def leaf_columns_bytes(N, nodes_leaf):
-    """Compulsory traffic of k_leaf_columns: the 8x8 depth block of every level-3 column (256 B),
-    and per leaf node its breadth-first index read and its run-head index written (8 B).  The
-    first node of each run also writes mask, hash and leafmasks; those bytes are left out, so
-    the figure is a lower bound."""
-    return 256 * (N // 8) ** 2 + 8 * nodes_leaf
+def leaf_waves_bytes(N, nodes_leaf):
+    """Compulsory traffic of k_leaf_waves: the 8x8 depth block of every level-3 column (256 B),
+    and per leaf node its breadth-first index read and its run-head index and leafmask slot
+    written (12 B).  The first node of each run also writes mask, hash and leafmasks; those bytes
+    are left out, so the figure is a lower bound."""
+    return 256 * (N // 8) ** 2 + 12 * nodes_leaf
 
 
 def pyramid_root_bytes(N):
@@ -482,7 +482,7 @@ def main():
 
     nodes, uniq = levels
     algo = {  # algorithmic bytes per profiled step of the kernels with a closed form
-        "leaf_columns": leaf_columns_bytes(N, int(nodes[2])),
+        "leaf_waves": leaf_waves_bytes(N, int(nodes[2])),
         "pyr_tile_root": pyramid_root_bytes(N),
         "evaluate4": 17 * nq,
     }
@@ -499,7 +499,7 @@ def main():
                 "avg_launch_us": round(per_launch_ms * 1e3, 2),
                 "bytes_per_launch": int(bytes_per_launch)}
 
-    rname = dominant[0] if dominant[0] in algo else "leaf_columns"
+    rname = dominant[0] if dominant[0] in algo else "leaf_waves"
     roofline = roof(rname)
     pyramid_roofline = roof("pyr_tile_root")
     lookup_roofline = roof("evaluate4")

@@ -180,102 +180,146 @@ struct LeafHash {
     }
 };
 
-// The nodes of one column, one texel per lane, in z order.  Only a run's first node stores its
-// mask, leafmasks and key hash: the merge and the final write read those of first occurrences
-// only, and a key's first occurrence is always the first node of a run (breadth-first order
-// grows with z inside a column).  Every node stores the breadth-first index of its run's first
-// node and the slot of that node's leafmasks.
-// Leafmasks go to the column's own slots (start + q: a column's nodes are contiguous in z), an
-// address that needs no breadth-first index, so the load of those indices overlaps the node loop;
-// the per-node words are stored by breadth-first index once per 64-node chunk.
-// f is this lane's floor(d*N*T - 0.5); fmin and fmax are its minimum and maximum over the column:
-// min(c) and max(c) of node z3 are clamp(fmin - 8*z3 + 1, 0, 8) and the same for fmax.
-__device__ __forceinline__ void leaf_column(const LeafColumns& lc, u32 cnt, u32 base, int zs, int f, int fmin, int fmax,
-                                            u32 lane, u32 nleaf, u32* __restrict__ mask_out, u64* __restrict__ leafm,
-                                            u64* __restrict__ hkey, u32* __restrict__ runhead,
-                                            u32* __restrict__ leafslot) {
-    u32 headq = 0;     // column position of the current run's first node
-    u32 head_bfs = 0;  // its breadth-first index (a run can span chunks)
-    for (u32 q0 = 0; q0 < cnt; q0 += 64) {
-        const u32 qn = min(cnt - q0, 64u);
-        const u32 bfs_chunk = (lane < qn) ? lc.bfs[base + q0 + lane] : 0u;  // first read after the node loop
-        // Lane j collects node q0 + j's run head, and its mask and hash when it heads a run.
-        u32 my_headq = 0, my_mask = 0;
-        u64 my_hash = 0;
-        bool my_first = false;
-        for (u32 qq = 0; qq < qn; ++qq) {
-            const u32 q = q0 + qq;
-            const int z8 = 8 * (zs + (int)q);
-            const int u = f - z8;
-            const bool changed = (q == 0) || (__ballot((u32)(u + 8) <= 14u) != 0ull);
-            if (changed) {
+// ---------------------------------------------------------------------------------------
+// The leaf kernel: kLeafCols level-3 columns per wave, one texel per lane.
+// * All loads of a wave's columns are issued together (column info, then the depth blocks and
+//   the first 64 breadth-first indices of each column); the columns are then walked one after the
+//   other with no memory wait.  A column of more than 64 nodes (box walls) loads the indices of
+//   its later 64-node chunks in the loop.
+// * Which nodes differ from their predecessor follows from the texels alone.  With
+//   h_t = f_t + 8 - 8*zs, node q changes iff some h_t - 8q lies in [0, 14]: q = floor(h_t/8), or
+//   q = floor(h_t/8) - 1 when h_t mod 8 != 7.  Each texel marks at most two nodes, so a wave
+//   OR-reduction of the marks gives a chunk's changed nodes at once, and only those (the run
+//   heads, 1.3 M of 6.7 M leaves at 8192^2) get compares.
+// * Only a run's first node stores its mask, leafmasks and key hash: the merge and the final write
+//   read those of first occurrences only, and a key's first occurrence is always the first node
+//   of a run (breadth-first order grows with z inside a column).  Every node stores the
+//   breadth-first index of its run's first node and the slot of that node's leafmasks.
+// * Leafmasks go to the column's own slots (start + q: a column's nodes are contiguous in z), so
+//   their stores need no breadth-first index.
+// * Measured on MI355X: 8 columns per wave 0.31 ms, 4 columns 0.32 ms, 16 columns 0.42 ms; one
+//   column per one-wave workgroup (loads issued per column, run heads found by a ballot per node)
+//   0.55 ms; one lane per node over LDS-staged columns 0.51 ms (64 texels of work per node).
+// Workgroups are dealt to the XCDs in turn (block b to XCD b mod kXcds), each XCD with its own L2.
+// XCD x takes the contiguous units [x*unit_chunk, (x+1)*unit_chunk): neighbouring columns, whose
+// 8-texel depth rows share 128-byte lines, are then read through one L2.
+// ---------------------------------------------------------------------------------------
+constexpr u32 kXcds = 8;
+constexpr u32 kLeafCols = 8;
+constexpr u32 kLeafWaves = 4;
+
+__global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, u32 ncol, u32 nleaf,
+                                                                const float* __restrict__ depth, u32 N, u32 T,
+                                                                u32* __restrict__ mask_out, u64* __restrict__ leafm,
+                                                                u64* __restrict__ hkey, u32* __restrict__ runhead,
+                                                                u32* __restrict__ leafslot, u32 unit_chunk) {
+    // Per lane: f_t and the breadth-first index of node `lane` of each of the wave's columns (a lane
+    // only ever reads back its own entries, so the arrays need no barrier).
+    __shared__ int s_f[kLeafWaves][kLeafCols][64];
+    __shared__ u32 s_b[kLeafWaves][kLeafCols][64];
+    const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // Workgroups are dealt to the XCDs in turn; XCD x takes the contiguous units [x*unit_chunk, ...).
+    const u32 unit = (blockIdx.x % kXcds) * unit_chunk + blockIdx.x / kXcds;
+    const u32 c0 = (unit * kLeafWaves + wave) * kLeafCols;
+    // Every load below is unconditional (addresses clamped into range, values masked afterwards), so
+    // the compiler counts them exactly and waits only for what each step uses.
+    const u32 clast = ncol - 1;
+    uint4 ci = lc.info[min(c0 + (lane & (kLeafCols - 1)), clast)];  // lane k < kLeafCols: column c0 + k
+    if (lane >= kLeafCols || c0 + lane >= ncol) ci = make_uint4(0u, 0u, 0u, 0u);
+    float d[kLeafCols];
+#pragma unroll
+    for (u32 k = 0; k < kLeafCols; ++k) {
+        const u32 c = min(c0 + k, clast);
+        const u32 x3 = c & (lc.w3 - 1), y3 = c >> lc.w3_shift;
+        d[k] = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
+    }
+    u32 b[kLeafCols];
+#pragma unroll
+    for (u32 k = 0; k < kLeafCols; ++k) {
+        const u32 cnt = __builtin_amdgcn_readlane(ci.x, k), st = __builtin_amdgcn_readlane(ci.y, k);
+        b[k] = lc.bfs[(lane < cnt) ? st + lane : 0u];  // st + lane < nleaf: the column counts sum to it
+    }
+    const float HT = (float)(N * T);
+#pragma unroll
+    for (u32 k = 0; k < kLeafCols; ++k) {
+        const float D = d[k] * HT;
+        int f = -(1 << 30);  // NaN is never lit
+        if (D == D) f = (int)floorf(fminf(fmaxf(D, -2.0f), 4194304.0f) - 0.5f);  // exact for |D| < 2^22
+        s_f[wave][k][lane] = f;
+        s_b[wave][k][lane] = b[k];
+    }
+
+    for (u32 k = 0; k < kLeafCols; ++k) {
+        if (c0 + k >= ncol) break;
+        const u32 cnt = __builtin_amdgcn_readlane(ci.x, k);
+        if (cnt == 0) continue;
+        const u32 base = __builtin_amdgcn_readlane(ci.y, k);
+        const int zs = (int)__builtin_amdgcn_readlane(ci.z, k);
+        const int f = s_f[wave][k][lane];
+        const int fmin = __ockl_wfred_min_i32(f), fmax = __ockl_wfred_max_i32(f);
+        const int h = f + 8 - 8 * zs;
+        const int qa = h >> 3;  // floor(h / 8)
+        const bool below = (h & 7) != 7;
+        u32 head_q = 0, head_bfs = 0;  // the run head carried into the next chunk
+        // Nodes [q0, q0 + 64) of the column; bfs = breadth-first index of node q0 + lane.
+        auto chunk = [&](u32 q0, u32 bfs) {
+            const u32 qn = min(cnt - q0, 64u);
+            const int ra = qa - (int)q0;
+            u64 marks = 0;
+            if ((u32)ra < 64u) marks |= 1ull << ra;
+            if (below && (u32)(ra - 1) < 64u) marks |= 1ull << (ra - 1);
+            const u32 chg_hi = __builtin_amdgcn_readfirstlane(__ockl_wfred_or_u32((u32)(marks >> 32)));
+            const u32 chg_lo = __builtin_amdgcn_readfirstlane(__ockl_wfred_or_u32((u32)marks));
+            u64 chg = ((u64)chg_hi << 32) | chg_lo;  // wave-uniform
+            if (q0 == 0) chg |= 1ull;                // a column's first node starts a run
+            if (qn < 64) chg &= (1ull << qn) - 1ull;
+            // The run heads of this chunk, one after the other (wave-uniform): mask, leafmasks, hash.
+            u32 my_mask = 0;
+            u64 my_hash = 0;
+            for (u64 pend = chg; pend; pend &= pend - 1ull) {
+                const u32 j = (u32)__builtin_ctzll(pend);
+                const int z8 = 8 * (zs + (int)(q0 + j));
+                const int u = f - z8;
                 const u32 cmin = (u32)min(max(fmin - z8 + 1, 0), 8);
                 const u32 cmax = (u32)min(max(fmax - z8 + 1, 0), 8);
                 const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
                 const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min);
-                LeafHash h(mask);
-                u64 mine = 0;  // lane j gets the leafmask of partial slice cmin + j
-                const u32 k = cmax - cmin;
-                for (u32 j = 0; j < k; ++j) {
-                    const u64 m = __ballot(u >= (int)(cmin + j));
-                    h.add(m);
-                    mine = (lane == j) ? m : mine;
+                LeafHash hs(mask);
+                u64 mine = 0;  // lane s gets the leafmask of partial slice cmin + s
+                const u32 kp = cmax - cmin;
+                for (u32 sl = 0; sl < kp; ++sl) {
+                    const u64 m = __ballot(u >= (int)(cmin + sl));
+                    hs.add(m);
+                    mine = (lane == sl) ? m : mine;
                 }
-                headq = q;
-                const u32 slot = base + q;  // < nleaf: the column counts sum to the leaf count
-                if (lane < k && slot < nleaf) leafm[(size_t)slot * 8 + lane] = mine;
-                const u64 hv = h.value();
-                my_mask = (lane == qq) ? mask : my_mask;
-                my_hash = (lane == qq) ? hv : my_hash;
-                my_first = (lane == qq) ? true : my_first;
+                const u32 slot = base + q0 + j;  // < nleaf
+                if (lane < kp && slot < nleaf) leafm[(size_t)slot * 8 + lane] = mine;
+                const u64 hv = hs.value();
+                my_mask = (lane == j) ? mask : my_mask;
+                my_hash = (lane == j) ? hv : my_hash;
             }
-            my_headq = (lane == qq) ? headq : my_headq;
-        }
-        // The run head's breadth-first index: in this chunk, or carried from the previous one.
-        const int src = (int)my_headq - (int)q0;
-        const u32 in_chunk = (u32)__shfl((int)bfs_chunk, src >= 0 ? src : 0, kWave);
-        const u32 my_head = (src >= 0) ? in_chunk : head_bfs;
-        head_bfs = (u32)__shfl((int)my_head, (int)(qn - 1), kWave);
-        // nleaf bounds every index (only exceeded if the column map were inconsistent).
-        if (lane < qn && bfs_chunk < nleaf) {
-            runhead[bfs_chunk] = my_head;
-            leafslot[bfs_chunk] = base + my_headq;
-            if (my_first) {
-                mask_out[bfs_chunk] = my_mask;
-                hkey[bfs_chunk] = my_hash;
+            // Node q0 + lane's run head: the last changed node at or before it.
+            const u64 upto = chg & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+            const int src = upto ? 63 - __builtin_clzll(upto) : 0;
+            const u32 hb = (u32)__shfl((int)bfs, src, kWave);
+            const u32 my_head_bfs = upto ? hb : head_bfs;
+            const u32 my_head_q = upto ? q0 + (u32)src : head_q;
+            head_bfs = (u32)__shfl((int)my_head_bfs, (int)(qn - 1), kWave);
+            head_q = (u32)__shfl((int)my_head_q, (int)(qn - 1), kWave);
+            // nleaf bounds every index (only exceeded if the column map were inconsistent).
+            if (lane < qn && bfs < nleaf) {
+                runhead[bfs] = my_head_bfs;
+                leafslot[bfs] = base + my_head_q;
+                if ((chg >> lane) & 1ull) {
+                    mask_out[bfs] = my_mask;
+                    hkey[bfs] = my_hash;
+                }
             }
-        }
+        };
+        // The first chunk's indices are staged; a longer column (box walls) loads the rest here.
+        chunk(0u, s_b[wave][k][lane]);
+        for (u32 q0 = 64; q0 < cnt; q0 += 64) chunk(q0, lc.bfs[base + min(q0 + lane, cnt - 1)]);
     }
-}
-
-// One column per 64-thread workgroup.  Measured on MI355X: one column per wave beats two and four
-// (the node loop of a column is a serial chain, and the chip needs as many independent chains in
-// flight as it can hold), and one-wave workgroups beat four-wave ones (columns differ widely in
-// length, from one node to hundreds at box walls, and a long column held its group's slot).
-// Workgroups are dealt to the XCDs in turn (block b to XCD b mod kXcds), each XCD with its own L2.
-// XCD x takes the contiguous columns [x*chunk, (x+1)*chunk): neighbouring columns, whose 8-texel
-// depth rows share 128-byte lines, are then read through one L2 instead of four.
-constexpr u32 kXcds = 8;
-
-__global__ __launch_bounds__(64) void k_leaf_columns(LeafColumns lc, u32 ncol, u32 nleaf, const float* __restrict__ depth,
-                                                      u32 N, u32 T, u32* __restrict__ mask_out, u64* __restrict__ leafm,
-                                                      u64* __restrict__ hkey, u32* __restrict__ runhead,
-                                                      u32* __restrict__ leafslot, u32 xcd_chunk) {
-    const u32 lane = threadIdx.x & 63;
-    const u32 col = __builtin_amdgcn_readfirstlane((blockIdx.x % kXcds) * xcd_chunk + blockIdx.x / kXcds);
-    if (col >= ncol) return;  // uniform per wave
-    const u32 x3 = col & (lc.w3 - 1), y3 = col >> lc.w3_shift;
-    // The column's two loads are independent and issued together (no early exit for an empty
-    // column: its node loop simply does not run).
-    const float d = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
-    const uint4 ci = lc.info[col];
-    const u32 cnt = ci.x, base = ci.y;
-    const int zs = (int)ci.z;
-    const float D = d * (float)(N * T);
-    int f = -(1 << 30);  // NaN is never lit
-    if (D == D) f = (int)floorf(fminf(fmaxf(D, -2.0f), 4194304.0f) - 0.5f);  // exact for |D| < 2^22
-    const int fmin = __ockl_wfred_min_i32(f), fmax = __ockl_wfred_max_i32(f);
-    leaf_column(lc, cnt, base, zs, f, fmin, fmax, lane, nleaf, mask_out, leafm, hkey, runhead, leafslot);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1058,9 +1102,10 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     for (int L = top_big; L >= deepest; --L) {
         const u32 c = lv[L].count;
         if (leaf && L == 2) {
-            const u32 chunk = ceil_div<u32>(ncol, kXcds);  // one column per workgroup
-            CPVS_LAUNCH("leaf_columns", k_leaf_columns, dim3(chunk * kXcds), dim3(64), 0, s, lc, ncol, c, p->depth, N, T,
-                        lv[2].mask, leafm, hkey, runhead, leafslot, chunk);
+            const u32 units = ceil_div<u32>(ncol, kLeafWaves * kLeafCols);
+            const u32 unit_chunk = ceil_div<u32>(units, kXcds);
+            CPVS_LAUNCH("leaf_waves", k_leaf_waves, dim3(unit_chunk * kXcds), dim3(64 * kLeafWaves), 0, s, lc, ncol, c,
+                        p->depth, N, T, lv[2].mask, leafm, hkey, runhead, leafslot, unit_chunk);
             break;
         }
         CPVS_LAUNCH("classify", k_classify, grid1d(c), dim3(256), 0, s, lv[L].coord, c, p->lv[L], p->depth, N, L, T,
