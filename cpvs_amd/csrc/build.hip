@@ -391,26 +391,35 @@ struct InnerEq {
 
 // --- (a) hash table --------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ th, u32* __restrict__ ti, u32 S) {
+// One 64-bit word per slot: the key hash's high 32 bits (its tag) above the smallest node index
+// with that hash, so a probe touches one cache line and a new key costs one atomic.  ~0 marks an
+// empty slot (node indices stay below 2^32 - 1).  Two keys whose tags are equal and whose probe
+// chains meet share a slot; the full-key compare in k_resolve then sends the build down the
+// sorting path, as for equal 64-bit hashes.
+constexpr u64 kEmptySlot = ~0ull;
+
+__global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ tab, u32 S) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= S) return;
-    th[i] = 0ull;
-    ti[i] = kNoRep;
+    if (i < S) tab[i] = kEmptySlot;
 }
 
 // Open addressing, linear probing; the table has more slots than keys, so every probe ends.
-// Large groups of equal keys hit one slot, so the slot is read first with plain loads: a slot
-// only ever goes from empty to one hash, and its index only decreases, so a stale read merely
+// Large groups of equal keys hit one slot, so the slot is read first with a plain load: a slot
+// only ever goes from empty to one tag, and its index only decreases, so a stale read merely
 // leads to an atomic that was not needed.
-__device__ inline u32 table_insert(u64* th, u32* ti, u32 S, u64 h, u32 i) {
-    if (h == 0ull) h = 1ull;  // 0 marks an empty slot
+__device__ inline u32 table_insert(u64* tab, u32 S, u64 h, u32 i) {
+    const u32 tag = (u32)(h >> 32);
+    const u64 mine = ((u64)tag << 32) | i;
     u32 slot = (u32)(h ^ (h >> 31)) & (S - 1);
     for (;;) {
-        u64 cur = __builtin_nontemporal_load(th + slot);
-        if (cur == 0ull)
-            cur = atomicCAS(reinterpret_cast<unsigned long long*>(th + slot), 0ull, (unsigned long long)h);
-        if (cur == 0ull || cur == h) {
-            if (__builtin_nontemporal_load(ti + slot) > i) atomicMin(ti + slot, i);
+        u64 cur = __builtin_nontemporal_load(tab + slot);
+        if (cur == kEmptySlot) {
+            cur = atomicCAS(reinterpret_cast<unsigned long long*>(tab + slot), (unsigned long long)kEmptySlot,
+                            (unsigned long long)mine);
+            if (cur == kEmptySlot) return slot;
+        }
+        if ((u32)(cur >> 32) == tag) {
+            if ((u32)cur > i) atomicMin(reinterpret_cast<unsigned long long*>(tab + slot), (unsigned long long)mine);
             return slot;
         }
         slot = (slot + 1) & (S - 1);
@@ -425,7 +434,7 @@ __device__ inline u32 table_insert(u64* th, u32* ti, u32 S, u64 h, u32 i) {
 // workgroup calls this (it has barriers).
 constexpr u32 kGroupSlots = 512;
 
-__device__ inline u32 table_insert_group(u64* th, u32* ti, u32 S, u64 h, u32 i, bool active) {
+__device__ inline u32 table_insert_group(u64* tab, u32 S, u64 h, u32 i, bool active) {
     __shared__ u64 gh[kGroupSlots];
     __shared__ u32 gi[kGroupSlots];
     __shared__ u32 gs[kGroupSlots];
@@ -434,7 +443,7 @@ __device__ inline u32 table_insert_group(u64* th, u32* ti, u32 S, u64 h, u32 i, 
         gi[j] = kNoRep;
     }
     __syncthreads();
-    if (h == 0ull) h = 1ull;  // 0 marks an empty slot (as in the global table)
+    if (h == 0ull) h = 1ull;  // 0 marks an empty LDS slot
     u32 ls = 0;
     if (active) {
         ls = (u32)(h ^ (h >> 29)) & (kGroupSlots - 1);
@@ -446,31 +455,30 @@ __device__ inline u32 table_insert_group(u64* th, u32* ti, u32 S, u64 h, u32 i, 
         atomicMin(gi + ls, i);
     }
     __syncthreads();
-    if (active && gi[ls] == i) gs[ls] = table_insert(th, ti, S, h, i);
+    if (active && gi[ls] == i) gs[ls] = table_insert(tab, S, h, i);
     __syncthreads();
     return active ? gs[ls] : 0u;
 }
 
 __global__ __launch_bounds__(256) void k_insert_leaf(const u64* __restrict__ hkey, const u32* __restrict__ runhead, u32 n,
-                                                     u64* __restrict__ th, u32* __restrict__ ti, u32 S,
-                                                     u32* __restrict__ tslot) {
+                                                     u64* __restrict__ tab, u32 S, u32* __restrict__ tslot) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < n && runhead[i] == i;  // a run's first node has the run's smallest index
-    const u32 slot = table_insert_group(th, ti, S, act ? hkey[i] : 0ull, i, act);
+    const u32 slot = table_insert_group(tab, S, act ? hkey[i] : 0ull, i, act);
     if (act) tslot[i] = slot;
 }
 
 __global__ __launch_bounds__(256) void k_insert_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
-                                                      const u32* __restrict__ child_rank, u32 n, u64* __restrict__ th,
-                                                      u32* __restrict__ ti, u32 S, u32* __restrict__ tslot) {
+                                                      const u32* __restrict__ child_rank, u32 n, u64* __restrict__ tab,
+                                                      u32 S, u32* __restrict__ tslot) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < n;
-    const u32 slot = table_insert_group(th, ti, S, act ? inner_key_hash(mask, cstart, child_rank, i) : 0ull, i, act);
+    const u32 slot = table_insert_group(tab, S, act ? inner_key_hash(mask, cstart, child_rank, i) : 0ull, i, act);
     if (act) tslot[i] = slot;
 }
 
 template <class Eq>
-__global__ __launch_bounds__(256) void k_resolve(const u32* __restrict__ tslot, const u32* __restrict__ ti,
+__global__ __launch_bounds__(256) void k_resolve(const u32* __restrict__ tslot, const u64* __restrict__ tab,
                                                  const u32* __restrict__ runhead, u32 n, Eq eq, u32* __restrict__ rep,
                                                  u32* __restrict__ collisions) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -479,7 +487,7 @@ __global__ __launch_bounds__(256) void k_resolve(const u32* __restrict__ tslot, 
         rep[i] = kNoRep;  // resolved through its run's first node
         return;
     }
-    u32 r = ti[tslot[i]];
+    u32 r = (u32)tab[tslot[i]];  // the smallest index with this node's tag
     if (r >= n) r = i;
     if (r != i && !eq(i, r)) {
         atomicAdd(collisions, 1u);  // two keys share a 64-bit hash: the build is redone by sorting
@@ -1002,7 +1010,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     const size_t c2 = leaf ? (cnt[2] ? cnt[2] : 1) : 1;
     if (leaf) need += arena_bytes(c2 * 8, 8) + arena_bytes(c2, 8) + 3 * arena_bytes(c2, 4);
     need += 2 * arena_bytes(cmax, 4);
-    if (use_table) need += arena_bytes(S, 8) + arena_bytes(S, 4);
+    if (use_table) need += arena_bytes(S, 8);
     else need += 4 * arena_bytes(cmax, 4) + arena_bytes(sort_scratch_words(cmax), 4);
     need += arena_bytes(scan_scratch_words(scan_n), 4);
     need += 1 << 20;
@@ -1046,8 +1054,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     u32* rep = A.take<u32>(cmax);
     u32* tslot = A.take<u32>(cmax);  // table slot of each node, or segment starts when sorting
     u32* seg = tslot;
-    u64* th = use_table ? A.take<u64>(S) : nullptr;
-    u32* ti = use_table ? A.take<u32>(S) : nullptr;
+    u64* tab = use_table ? A.take<u64>(S) : nullptr;
     u32 *skey = nullptr, *sval = nullptr, *ktmp = nullptr, *vtmp = nullptr, *sort_scratch = nullptr;
     if (!use_table) {
         skey = A.take<u32>(cmax);
@@ -1133,15 +1140,15 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         const u32* rh = is_leaf ? runhead : nullptr;
         if (use_table) {
             const u32 Sl = table_slots(c);
-            CPVS_LAUNCH("table_clear", k_table_clear, grid1d(Sl), dim3(256), 0, s, th, ti, Sl);
+            CPVS_LAUNCH("table_clear", k_table_clear, grid1d(Sl), dim3(256), 0, s, tab, Sl);
             if (is_leaf) {
-                CPVS_LAUNCH("insert_leaf", k_insert_leaf, grid1d(c), dim3(256), 0, s, hkey, runhead, c, th, ti, Sl, tslot);
-                CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c), dim3(256), 0, s, tslot, ti, runhead, c,
+                CPVS_LAUNCH("insert_leaf", k_insert_leaf, grid1d(c), dim3(256), 0, s, hkey, runhead, c, tab, Sl, tslot);
+                CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c), dim3(256), 0, s, tslot, tab, runhead, c,
                             LeafEq{lv[2].mask, leafm, leafslot, nullptr}, rep, meta + kMetaColl);
             } else {
                 CPVS_LAUNCH("insert_inner", k_insert_inner, grid1d(c), dim3(256), 0, s, lv[L].mask, lv[L].cstart, child_rank, c,
-                            th, ti, Sl, tslot);
-                CPVS_LAUNCH("resolve", k_resolve<InnerEq>, grid1d(c), dim3(256), 0, s, tslot, ti, nullptr, c,
+                            tab, Sl, tslot);
+                CPVS_LAUNCH("resolve", k_resolve<InnerEq>, grid1d(c), dim3(256), 0, s, tslot, tab, nullptr, c,
                             InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaColl);
             }
         } else {

@@ -344,7 +344,7 @@ int cpvs_estimate_build(cpvs_ctx* ctx, const cpvs_pyramid* p, uint32_t t, uint32
     u64 slots = 1024;
     while (slots < cmax + cmax / 2 + 1) slots <<= 1;
     const u64 merge = (ctx->merge_mode == CPVS_MERGE_TABLE)
-                          ? arena_bytes(slots, 8) + arena_bytes(slots, 4)
+                          ? arena_bytes(slots, 8)
                           : 4 * arena_bytes(cmax, 4) + arena_bytes(sort_scratch_words((u32)cmax), 4);
     const u64 temps = 2 * arena_bytes(cmax, 4) + merge + arena_bytes(scan_scratch_words((u32)(cmax > ncol ? cmax : ncol)), 4) +
                       (1 << 20);
