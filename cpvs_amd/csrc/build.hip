@@ -745,10 +745,12 @@ enum : int {
 // classification and child emission, bottom-up merge (hash table in LDS), and compress
 // (size scans, level bases, words).  Results are those of the per-level path.
 // ---------------------------------------------------------------------------------------
-constexpr u32 kSmallMax = 2048;
+// Measured at 8192^2: a limit of 4096 nodes beats 2048 (1.29 against 1.31 ms per build) and 8192
+// (1.34 ms: one workgroup then spends more on the 7.9 K-node level than its per-level launches do).
+constexpr u32 kSmallMax = 4096;
 constexpr int kSmallThreads = 1024;
 constexpr u32 kSmallPer = kSmallMax / kSmallThreads;  // nodes per thread
-constexpr u32 kSmallSlots = 2 * kSmallMax;             // LDS table: 48 KB
+constexpr u32 kSmallSlots = 2 * kSmallMax;             // LDS table of 8-byte slots: 64 KB
 
 struct SmallArgs {
     int R, Ls, deepest;  // the small levels are [Ls, R]; deepest = lowest level with nodes
@@ -821,13 +823,17 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_topdown(SmallArgs a, co
     }
 }
 
-__device__ inline u32 lds_insert(u64* th, u32* ti, u32 S, u64 h, u32 i) {
-    if (h == 0ull) h = 1ull;  // 0 marks an empty slot
+// The LDS table of the small levels: one 64-bit word per slot, as in the global table.
+__device__ inline u32 lds_insert(u64* tab, u32 S, u64 h, u32 i) {
+    const u32 tag = (u32)(h >> 32);
+    const u64 mine = ((u64)tag << 32) | i;
     u32 slot = (u32)(h ^ (h >> 31)) & (S - 1);
     for (;;) {
-        const u64 cur = atomicCAS(reinterpret_cast<unsigned long long*>(th + slot), 0ull, (unsigned long long)h);
-        if (cur == 0ull || cur == h) {
-            atomicMin(ti + slot, i);
+        const u64 cur = atomicCAS(reinterpret_cast<unsigned long long*>(tab + slot), (unsigned long long)kEmptySlot,
+                                  (unsigned long long)mine);
+        if (cur == kEmptySlot) return slot;
+        if ((u32)(cur >> 32) == tag) {
+            atomicMin(reinterpret_cast<unsigned long long*>(tab + slot), (unsigned long long)mine);
             return slot;
         }
         slot = (slot + 1) & (S - 1);
@@ -835,20 +841,16 @@ __device__ inline u32 lds_insert(u64* th, u32* ti, u32 S, u64 h, u32 i) {
 }
 
 // Merge of the small levels Ls .. R-1 (the root is never merged), then the root's rank.  A node
-// takes the smallest index with its 64-bit key hash; if that node's key differs (a hash
-// collision), it looks for its first occurrence by comparing keys with every lower index.
+// takes the smallest index with its slot's tag; if that node's key differs (a collision of tags
+// or of 64-bit hashes), it looks for its first occurrence by comparing keys with every lower index.
 __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
-    __shared__ u64 th[kSmallSlots];
-    __shared__ u32 ti[kSmallSlots];
+    __shared__ u64 tab[kSmallSlots];
     __shared__ u32 s_wave[kSmallThreads / kWave];
     for (int L = a.Ls; L < a.R; ++L) {
         const u32 c = a.count[L];
         u32 S = 64;
         while (S < 2 * c) S <<= 1;
-        for (u32 j = threadIdx.x; j < S; j += kSmallThreads) {
-            th[j] = 0ull;
-            ti[j] = kNoRep;
-        }
+        for (u32 j = threadIdx.x; j < S; j += kSmallThreads) tab[j] = kEmptySlot;
         __syncthreads();
         const u32* child_rank = (L > a.deepest) ? a.rank[L - 1] : nullptr;
         const u32 E = small_per_thread(c), i0 = threadIdx.x * E;
@@ -857,7 +859,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
         for (u32 e = 0; e < kSmallPer; ++e) {
             const u32 i = i0 + e;
             slot[e] = 0;
-            if (e < E && i < c) slot[e] = lds_insert(th, ti, S, inner_key_hash(a.mask[L], a.cstart[L], child_rank, i), i);
+            if (e < E && i < c) slot[e] = lds_insert(tab, S, inner_key_hash(a.mask[L], a.cstart[L], child_rank, i), i);
         }
         __syncthreads();
         const InnerEq eq{a.mask[L], a.cstart[L], child_rank};
@@ -867,7 +869,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
             const u32 i = i0 + e;
             rep[e] = i;
             if (e < E && i < c) {
-                u32 r = ti[slot[e]];
+                u32 r = (u32)tab[slot[e]];
                 if (r != i && !eq(i, r)) {
                     r = i;
                     for (u32 j = 0; j < i; ++j)
