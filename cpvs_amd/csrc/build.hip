@@ -159,6 +159,15 @@ struct EmitChildrenOut {
 // Node z3 equals node z3-1 of the same column unless some u_t lies in [-8, 6].
 // ---------------------------------------------------------------------------------------
 
+// Empty slot of the merge hash tables (see table_insert).
+constexpr u64 kEmptySlot = ~0ull;
+
+// Grid-stride clear of a merge table, folded into a kernel that runs before the table is used.
+__device__ inline void clear_table(u64* tab, u32 slots) {
+    const u32 stride = gridDim.x * blockDim.x;
+    for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < slots; j += stride) tab[j] = kEmptySlot;
+}
+
 // Key hash of a leaf node (wave-uniform): two 32-bit multiply-xorshift streams over the low and
 // high halves of the partial leafmasks, seeded by the childmask and coupled at every step, then
 // a finaliser that is a bijection on the two streams (so it adds no collisions).
@@ -212,7 +221,8 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
                                                                 const float* __restrict__ depth, u32 N, u32 T,
                                                                 u32* __restrict__ mask_out, u64* __restrict__ leafm,
                                                                 u64* __restrict__ hkey, u32* __restrict__ runhead,
-                                                                u32* __restrict__ leafslot, u32 unit_chunk) {
+                                                                u32* __restrict__ leafslot, u32 unit_chunk,
+                                                                u64* __restrict__ tab, u32 tab_slots) {
     // Per lane: f_t and the breadth-first index of node `lane` of each of the wave's columns (a lane
     // only ever reads back its own entries, so the arrays need no barrier).
     __shared__ int s_f[kLeafWaves][kLeafCols][64];
@@ -320,6 +330,9 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
         chunk(0u, s_b[wave][k][lane]);
         for (u32 q0 = 64; q0 < cnt; q0 += 64) chunk(q0, lc.bfs[base + min(q0 + lane, cnt - 1)]);
     }
+    // The leaf level's merge table is cleared here, after all the work (stores issued earlier
+    // would be waited for with the loads): this kernel leaves HBM bandwidth to spare.
+    clear_table(tab, tab_slots);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -396,7 +409,6 @@ struct InnerEq {
 // empty slot (node indices stay below 2^32 - 1).  Two keys whose tags are equal and whose probe
 // chains meet share a slot; the full-key compare in k_resolve then sends the build down the
 // sorting path, as for equal 64-bit hashes.
-constexpr u64 kEmptySlot = ~0ull;
 
 __global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ tab, u32 S) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -602,17 +614,20 @@ struct FirstSizeOut {
     }
 };
 
+// Copies take their first occurrence's rank.  The merge table of the next level (tab_slots > 0)
+// is cleared here too: this level's last table read (k_resolve) ran before.
 __global__ __launch_bounds__(256) void k_rank_copies(const u32* __restrict__ rep, const u32* __restrict__ runhead, u32 count,
-                                                     u32* __restrict__ rank) {
+                                                     u32* __restrict__ rank, u64* __restrict__ tab, u32 tab_slots) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    u32 r = rep[i];
-    if (r == i) return;
-    if (r == kNoRep) {
-        const u32 h = runhead[i];
-        r = (h < count) ? rep[h] : kNoRep;
+    if (i < count) {
+        u32 r = rep[i];
+        if (r == kNoRep) {
+            const u32 h = runhead[i];
+            r = (h < count) ? rep[h] : kNoRep;
+        }
+        if (r != i && r < count) rank[i] = rank[r];
     }
-    if (r < count) rank[i] = rank[r];
+    clear_table(tab, tab_slots);
 }
 
 __global__ void k_root_init(u32* rank, u32* uniq, u32* n_uniq) {
@@ -1113,8 +1128,10 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         if (leaf && L == 2) {
             const u32 units = ceil_div<u32>(ncol, kLeafWaves * kLeafCols);
             const u32 unit_chunk = ceil_div<u32>(units, kXcds);
+            // The leaf level is the first merged: its table is cleared by this kernel.
             CPVS_LAUNCH("leaf_waves", k_leaf_waves, dim3(unit_chunk * kXcds), dim3(64 * kLeafWaves), 0, s, lc, ncol, c,
-                        p->depth, N, T, lv[2].mask, leafm, hkey, runhead, leafslot, unit_chunk);
+                        p->depth, N, T, lv[2].mask, leafm, hkey, runhead, leafslot, unit_chunk, tab,
+                        use_table ? table_slots(c) : 0u);
             break;
         }
         CPVS_LAUNCH("classify", k_classify, grid1d(c), dim3(256), 0, s, lv[L].coord, c, p->lv[L], p->depth, N, L, T,
@@ -1142,7 +1159,8 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         const u32* rh = is_leaf ? runhead : nullptr;
         if (use_table) {
             const u32 Sl = table_slots(c);
-            CPVS_LAUNCH("table_clear", k_table_clear, grid1d(Sl), dim3(256), 0, s, tab, Sl);
+            // Cleared by the leaf kernel (leaf level) or the previous level's k_rank_copies.
+            if (L == deepest && !is_leaf) CPVS_LAUNCH("table_clear", k_table_clear, grid1d(Sl), dim3(256), 0, s, tab, Sl);
             if (is_leaf) {
                 CPVS_LAUNCH("insert_leaf", k_insert_leaf, grid1d(c), dim3(256), 0, s, hkey, runhead, c, tab, Sl, tslot);
                 CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c), dim3(256), 0, s, tslot, tab, runhead, c,
@@ -1175,7 +1193,8 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
                         nullptr, scan_scratch, s);
         else
             device_scan(FirstFlagIn{rep}, FirstRankOut{lv[L].rank, lv[L].uniq}, c, OpSum{}, meta + kMetaU + L, scan_scratch, s);
-        CPVS_LAUNCH("rank_copies", k_rank_copies, grid1d(c), dim3(256), 0, s, rep, rh, c, lv[L].rank);
+        const u32 S_next = (use_table && L + 1 < std::min(R, Ls)) ? table_slots(lv[L + 1].count) : 0u;
+        CPVS_LAUNCH("rank_copies", k_rank_copies, grid1d(c), dim3(256), 0, s, rep, rh, c, lv[L].rank, tab, S_next);
     }
     if (Ls <= R)
         CPVS_LAUNCH("small_merge", k_small_merge, dim3(1), dim3(kSmallThreads), 0, s, sa);
