@@ -42,12 +42,21 @@ def pct(xs, p):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
+def table_slots(n):
+    """Slots of a merge hash table for n keys: the power of two >= max(1024, n + n/2 + 1)."""
+    S = 1024
+    while S < n + n // 2 + 1:
+        S <<= 1
+    return S
+
+
 def leaf_waves_bytes(N, nodes_leaf):
-    """Compulsory traffic of k_leaf_waves: the 8x8 depth block of every level-3 column (256 B),
-    and per leaf node its breadth-first index read and its run-head index and leafmask slot
-    written (12 B).  The first node of each run also writes mask, hash and leafmasks; those bytes
-    are left out, so the figure is a lower bound."""
-    return 256 * (N // 8) ** 2 + 12 * nodes_leaf
+    """Compulsory traffic of k_leaf_waves: the 8x8 depth block of every level-3 column (256 B), per
+    leaf node its breadth-first index read and its run-head index and leafmask slot written
+    (12 B), and the clear of the leaf level's merge table (8 B per slot), which this kernel does.
+    The first node of each run also writes mask, hash and leafmasks; those bytes are left out, so
+    the figure is a lower bound."""
+    return 256 * (N // 8) ** 2 + 12 * nodes_leaf + 8 * table_slots(nodes_leaf)
 
 
 def pyramid_root_bytes(N):
