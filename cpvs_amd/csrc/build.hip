@@ -118,11 +118,17 @@ struct EmitChildrenOut {
     u64* child_coord;   // used when the child level is classified node by node
     u32 child_cap;      // nodes of the child level (exact count); writes beyond it are dropped
     LeafColumns cols;   // used when cols.bfs != null (the children are leaves)
-    __device__ void operator()(u32 i, u32 first, u32 k) const {
+    // The node's mask and cell, loaded by the scan before any output is written.
+    struct Pre {
+        u32 m;
+        u64 c;
+    };
+    __device__ Pre load(u32 i) const { return Pre{mask[i], coord[i]}; }
+    __device__ void operator()(u32 i, u32 first, u32 k, Pre pre) const {
         cstart[i] = first;
         if (!k) return;
-        const u32 m = mask[i];
-        const u64 c = coord[i];
+        const u32 m = pre.m;
+        const u64 c = pre.c;
         const u32 x = unpack_x(c), y = unpack_y(c), z = unpack_z(c);
         u32 j = first;
         // getChildCoordinates (src/CompressedShadowUtil.cpp:101-117): partial children in
@@ -588,8 +594,9 @@ struct FirstSizeIn {
     const u32* mask;
     u32 leaf;  // 1 at the leafmask level (two words per partial child)
     __device__ u64 operator()(u32 i) const {
-        if (rep[i] != i) return 0ull;
-        return (1ull << 32) | (u64)(1u + (num_children(mask[i]) << leaf));
+        const u32 r = rep[i], m = mask[i];  // both loads issued together (a select, not a branch)
+        const u64 v = (1ull << 32) | (u64)(1u + (num_children(m) << leaf));
+        return (r == i) ? v : 0ull;
     }
 };
 struct FirstSizeOut {
@@ -829,7 +836,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_topdown(SmallArgs a, co
         for (u32 e = 0; e < kSmallPer; ++e) {
             const u32 i = i0 + e;
             if (e < E && i < c) {
-                emit(i, first, k[e]);
+                emit(i, first, k[e], emit.load(i));
                 first += k[e];
             }
         }

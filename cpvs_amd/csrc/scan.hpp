@@ -6,6 +6,8 @@
 // no inter-block waiting.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace cpvs {
@@ -87,14 +89,112 @@ __device__ inline typename Op::T block_reduce(typename Op::T v, typename Op::T* 
     return tot;
 }
 
-template <class In, class Op>
+// Exclusive scans of a tile held striped across a 256-thread block (thread t holds elements
+// r*256 + t, r < R), all R rounds at once: R independent wave scans and one LDS exchange, so a
+// tile costs two barriers instead of two per round.  ex[r] = the element's prefix within the tile;
+// returns the tile total.
+template <int R, class Op>
+__device__ inline typename Op::T block_exclusive_scan_rounds(const typename Op::T (&v)[R], typename Op::T (&ex)[R], Op op) {
+    using T = typename Op::T;
+    __shared__ T s_w[R][kScanThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    T inc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) inc[r] = v[r];
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const T o = __shfl_up(inc[r], d, kWave);
+            if (lane >= d) inc[r] = op(inc[r], o);
+        }
+    }
+    if (lane == kWave - 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) s_w[r][wid] = inc[r];
+    }
+    __syncthreads();
+    T run = Op::identity;  // total of the earlier rounds
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        T wpre = Op::identity, tot = Op::identity;
+#pragma unroll
+        for (int w = 0; w < kScanThreads / kWave; ++w) {
+            const T x = s_w[r][w];
+            if (w < wid) wpre = op(wpre, x);
+            tot = op(tot, x);
+        }
+        T e = __shfl_up(inc[r], 1, kWave);
+        if (lane == 0) e = Op::identity;
+        ex[r] = op(run, op(wpre, e));
+        run = op(run, tot);
+    }
+    __syncthreads();
+    return run;
+}
+
+// An output functor may declare `Pre` and `Pre load(u32 i) const`: what out(i, ...) reads, loaded
+// by the latency path below together with the inputs, and handed over as a fourth argument.
+template <class Out, class = void>
+struct HasPre : std::false_type {};
+template <class Out>
+struct HasPre<Out, std::void_t<typename Out::Pre>> : std::true_type {};
+
+// One tile for launches of few blocks, where each block's chain of memory round trips is the cost:
+// every input (and prefetch) of the tile's 8 rounds is loaded first, one multi-round block scan
+// follows, then the outputs, with no load after the scan.  Returns the tile total.
+template <int R, class In, class Out, class Op>
+__device__ inline typename Op::T scan_tile_latency(In in, Out out, u32 n, u32 base, typename Op::T carry, Op op) {
+    using T = typename Op::T;
+    T v[R], ex[R];
+    if constexpr (HasPre<Out>::value) {
+        typename Out::Pre pre[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const u32 i = base + r * kScanThreads + threadIdx.x;
+            v[r] = (i < n) ? in(i) : Op::identity;
+            pre[r] = (i < n) ? out.load(i) : typename Out::Pre{};
+        }
+        const T tot = block_exclusive_scan_rounds<R>(v, ex, op);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const u32 i = base + r * kScanThreads + threadIdx.x;
+            if (i < n) out(i, op(carry, ex[r]), v[r], pre[r]);
+        }
+        return tot;
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const u32 i = base + r * kScanThreads + threadIdx.x;
+            v[r] = (i < n) ? in(i) : Op::identity;
+        }
+        const T tot = block_exclusive_scan_rounds<R>(v, ex, op);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const u32 i = base + r * kScanThreads + threadIdx.x;
+            if (i < n) out(i, op(carry, ex[r]), v[r]);
+        }
+        return tot;
+    }
+}
+
+// Calls out(i, ex, v), loading the prefetched part itself where the functor has one.
+template <class Out, class T>
+__device__ inline void call_out(const Out& out, u32 i, T ex, T v) {
+    if constexpr (HasPre<Out>::value)
+        out(i, ex, v, out.load(i));
+    else
+        out(i, ex, v);
+}
+
+template <class In, class Op, int R = kScanRounds>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(In in, u32 n, typename Op::T* partials, Op op) {
     using T = typename Op::T;
     __shared__ T s_wave[kScanThreads / kWave];
-    const u32 base = blockIdx.x * kScanTile;
+    const u32 base = blockIdx.x * (R * kScanThreads);
     T acc = Op::identity;
 #pragma unroll
-    for (int r = 0; r < kScanRounds; ++r) {
+    for (int r = 0; r < R; ++r) {
         const u32 i = base + r * kScanThreads + threadIdx.x;
         if (i < n) acc = op(acc, in(i));
     }
@@ -133,7 +233,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(In in, Out out, u32 
         const T v = (i < n) ? in(i) : Op::identity;
         T tot;
         const T ex = block_exclusive_scan(v, s_wave, &tot, op);
-        if (i < n) out(i, op(carry, ex), v);
+        if (i < n) call_out(out, i, op(carry, ex), v);
         carry = op(carry, tot);
     }
 }
@@ -142,26 +242,20 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(In in, Out out, u32 
 template <class In, class Out, class Op>
 __global__ __launch_bounds__(kScanThreads) void k_scan_single(In in, Out out, u32 n, typename Op::T* total, Op op) {
     using T = typename Op::T;
-    __shared__ T s_wave[kScanThreads / kWave];
-    T carry = Op::identity;
-    for (int r = 0; r < kScanRounds; ++r) {
-        const u32 r0 = r * kScanThreads;
-        if (r0 >= n) break;  // uniform across the block
-        const u32 i = r0 + threadIdx.x;
-        const T v = (i < n) ? in(i) : Op::identity;
-        T tot;
-        const T ex = block_exclusive_scan(v, s_wave, &tot, op);
-        if (i < n) out(i, op(carry, ex), v);
-        carry = op(carry, tot);
-    }
-    if (threadIdx.x == 0 && total) *total = carry;
+    const T all = scan_tile_latency<kScanRounds>(in, out, n, 0u, Op::identity, op);
+    if (threadIdx.x == 0 && total) *total = all;
 }
 
 // Down-sweep that derives its own carry from the tile partials (nparts <= kScanPartialsInline):
 // two launches in total instead of three.
 constexpr u32 kScanPartialsInline = 4 * kScanThreads;
 
-template <class In, class Out, class Op>
+// Up to kScanThreads * kScanPartialsInline elements, tiles are one round (256 elements): a level
+// of a few thousand to a few hundred thousand nodes then spreads over up to 1024 workgroups rather
+// than a handful, and each tile loads its inputs (and the outputs' prefetch) before its scan.
+constexpr u32 kScanSmallTileMax = kScanThreads * kScanPartialsInline;
+
+template <class In, class Out, class Op, int R>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down_inline(In in, Out out, u32 n, const typename Op::T* partials,
                                                                    u32 nparts, typename Op::T* total, Op op) {
     using T = typename Op::T;
@@ -175,16 +269,20 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down_inline(In in, Out ou
     T carry = block_reduce(pre, s_wave, op);
     const T grand = block_reduce(all, s_wave, op);
     if (blockIdx.x == 0 && threadIdx.x == 0 && total) *total = grand;
-    const u32 base = blockIdx.x * kScanTile;
-    for (int r = 0; r < kScanRounds; ++r) {
-        const u32 r0 = base + r * kScanThreads;
-        if (r0 >= n) break;  // uniform across the block
-        const u32 i = r0 + threadIdx.x;
-        const T v = (i < n) ? in(i) : Op::identity;
-        T tot;
-        const T ex = block_exclusive_scan(v, s_wave, &tot, op);
-        if (i < n) out(i, op(carry, ex), v);
-        carry = op(carry, tot);
+    const u32 base = blockIdx.x * (R * kScanThreads);
+    if constexpr (R == 1) {
+        scan_tile_latency<1>(in, out, n, base, carry, op);
+    } else {
+        for (int r = 0; r < R; ++r) {
+            const u32 r0 = base + r * kScanThreads;
+            if (r0 >= n) break;  // uniform across the block
+            const u32 i = r0 + threadIdx.x;
+            const T v = (i < n) ? in(i) : Op::identity;
+            T tot;
+            const T ex = block_exclusive_scan(v, s_wave, &tot, op);
+            if (i < n) call_out(out, i, op(carry, ex), v);
+            carry = op(carry, tot);
+        }
     }
 }
 
@@ -199,7 +297,14 @@ struct ArrOutExclusive {
 };
 
 // Scratch words (u32) needed by device_scan for n elements (room for 64-bit partials).
-inline size_t scan_scratch_words(u32 n) { return 2 * ((size_t)ceil_div<u32>(n ? n : 1, kScanTile) + 1); }
+// Covers every scan of at most n elements (callers size scratch for their largest scan): one-round
+// tiles make a scan of 262,144 elements use 1024 partials, more than one of 300,000.
+inline size_t scan_scratch_words(u32 n) {
+    size_t nb = ceil_div<u32>(n ? n : 1, kScanTile);
+    const u32 small = n < kScanSmallTileMax ? n : kScanSmallTileMax;
+    if (small > kScanTile) nb = std::max(nb, (size_t)ceil_div<u32>(small, kScanThreads));
+    return 2 * (nb + 1);
+}
 
 // out(i, exclusive_prefix_i, value_i) for all i < n; *d_total (device, may be null) gets the
 // reduction of all values.  scratch: scan_scratch_words(n) u32 of device memory (8-byte aligned).
@@ -212,11 +317,18 @@ inline void device_scan(In in, Out out, u32 n, Op op, typename Op::T* d_total, u
         CPVS_LAUNCH("scan_single", (k_scan_single<In, Out, Op>), dim3(1), dim3(kScanThreads), 0, s, in, out, n, d_total, op);
         return;
     }
+    if (n <= kScanSmallTileMax) {
+        const u32 nb = ceil_div<u32>(n, kScanThreads);
+        CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op, 1>), dim3(nb), dim3(kScanThreads), 0, s, in, n, part, op);
+        CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op, 1>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, part, nb,
+                    d_total, op);
+        return;
+    }
     const u32 nb = ceil_div<u32>(n, kScanTile);
     CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, n, part, op);
     if (nb <= kScanPartialsInline) {
-        CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, part, nb,
-                    d_total, op);
+        CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op, kScanRounds>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n,
+                    part, nb, d_total, op);
         return;
     }
     CPVS_LAUNCH("scan_partials", k_scan_partials<Op>, dim3(1), dim3(kScanThreads), 0, s, part, nb, d_total, op);
