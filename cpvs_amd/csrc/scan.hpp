@@ -253,6 +253,7 @@ constexpr u32 kScanPartialsInline = 4 * kScanThreads;
 // Up to kScanThreads * kScanPartialsInline elements, tiles are one round (256 elements): a level
 // of a few thousand to a few hundred thousand nodes then spreads over up to 1024 workgroups rather
 // than a handful, and each tile loads its inputs (and the outputs' prefetch) before its scan.
+// Up to 2x and 4x that many, tiles of 2 and 4 rounds keep the workgroup count at most 1024.
 constexpr u32 kScanSmallTileMax = kScanThreads * kScanPartialsInline;
 
 template <class In, class Out, class Op, int R>
@@ -270,8 +271,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down_inline(In in, Out ou
     const T grand = block_reduce(all, s_wave, op);
     if (blockIdx.x == 0 && threadIdx.x == 0 && total) *total = grand;
     const u32 base = blockIdx.x * (R * kScanThreads);
-    if constexpr (R == 1) {
-        scan_tile_latency<1>(in, out, n, base, carry, op);
+    if constexpr (R <= 4) {
+        scan_tile_latency<R>(in, out, n, base, carry, op);
     } else {
         for (int r = 0; r < R; ++r) {
             const u32 r0 = base + r * kScanThreads;
@@ -306,6 +307,15 @@ inline size_t scan_scratch_words(u32 n) {
     return 2 * (nb + 1);
 }
 
+// Reduce and down-sweep with R-round tiles (at most kScanPartialsInline of them).
+template <class In, class Out, class Op, int R>
+inline void scan_two_launches(In in, Out out, u32 n, Op op, typename Op::T* d_total, typename Op::T* part, hipStream_t s) {
+    const u32 nb = ceil_div<u32>(n, R * kScanThreads);
+    CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op, R>), dim3(nb), dim3(kScanThreads), 0, s, in, n, part, op);
+    CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op, R>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, part, nb,
+                d_total, op);
+}
+
 // out(i, exclusive_prefix_i, value_i) for all i < n; *d_total (device, may be null) gets the
 // reduction of all values.  scratch: scan_scratch_words(n) u32 of device memory (8-byte aligned).
 // One launch for n <= 2048, two up to 2M elements, three beyond.
@@ -318,10 +328,15 @@ inline void device_scan(In in, Out out, u32 n, Op op, typename Op::T* d_total, u
         return;
     }
     if (n <= kScanSmallTileMax) {
-        const u32 nb = ceil_div<u32>(n, kScanThreads);
-        CPVS_LAUNCH("scan_reduce", (k_scan_reduce<In, Op, 1>), dim3(nb), dim3(kScanThreads), 0, s, in, n, part, op);
-        CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op, 1>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, part, nb,
-                    d_total, op);
+        scan_two_launches<In, Out, Op, 1>(in, out, n, op, d_total, part, s);
+        return;
+    }
+    if (n <= 2 * kScanSmallTileMax) {
+        scan_two_launches<In, Out, Op, 2>(in, out, n, op, d_total, part, s);
+        return;
+    }
+    if (n <= 4 * kScanSmallTileMax) {
+        scan_two_launches<In, Out, Op, 4>(in, out, n, op, d_total, part, s);
         return;
     }
     const u32 nb = ceil_div<u32>(n, kScanTile);
