@@ -202,22 +202,41 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(In in, u32 n, type
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-// Exclusive scan of the tile partials in place (one block); *total = overall reduction.
+// Exclusive scan of the tile partials in place (one block); *total = overall reduction.  Each of
+// 1024 threads sums a contiguous run of partials, one block scan of those sums follows, and each
+// thread writes its run's prefixes: one pass instead of one block scan per 256 partials.
+constexpr int kPartialsThreads = 1024;
+
 template <class Op>
-__global__ __launch_bounds__(kScanThreads) void k_scan_partials(typename Op::T* partials, u32 nparts, typename Op::T* total,
-                                                                Op op) {
+__global__ __launch_bounds__(kPartialsThreads) void k_scan_partials(typename Op::T* partials, u32 nparts, typename Op::T* total,
+                                                                    Op op) {
     using T = typename Op::T;
-    __shared__ T s_wave[kScanThreads / kWave];
-    T carry = Op::identity;
-    for (u32 base = 0; base < nparts; base += kScanThreads) {
-        const u32 i = base + threadIdx.x;
-        const T v = (i < nparts) ? partials[i] : Op::identity;
-        T tot;
-        const T ex = block_exclusive_scan(v, s_wave, &tot, op);
-        if (i < nparts) partials[i] = op(carry, ex);
-        carry = op(carry, tot);
+    __shared__ T s_w[kPartialsThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const u32 E = ceil_div<u32>(nparts, kPartialsThreads);
+    const u32 b = threadIdx.x * E, e = min(b + E, nparts);
+    T sum = Op::identity;
+#pragma unroll 4
+    for (u32 i = b; i < e; ++i) sum = op(sum, partials[i]);
+    const T inc = wave_inclusive_scan(sum, op);
+    if (lane == kWave - 1) s_w[wid] = inc;
+    __syncthreads();
+    T wpre = Op::identity, tot = Op::identity;
+#pragma unroll
+    for (int w = 0; w < kPartialsThreads / kWave; ++w) {
+        const T x = s_w[w];
+        if (w < wid) wpre = op(wpre, x);
+        tot = op(tot, x);
     }
-    if (threadIdx.x == 0 && total) *total = carry;
+    T ex = __shfl_up(inc, 1, kWave);
+    if (lane == 0) ex = Op::identity;
+    T run = op(wpre, ex);
+    for (u32 i = b; i < e; ++i) {
+        const T v = partials[i];
+        partials[i] = run;
+        run = op(run, v);
+    }
+    if (threadIdx.x == 0 && total) *total = tot;
 }
 
 template <class In, class Out, class Op>
@@ -346,7 +365,7 @@ inline void device_scan(In in, Out out, u32 n, Op op, typename Op::T* d_total, u
                     part, nb, d_total, op);
         return;
     }
-    CPVS_LAUNCH("scan_partials", k_scan_partials<Op>, dim3(1), dim3(kScanThreads), 0, s, part, nb, d_total, op);
+    CPVS_LAUNCH("scan_partials", k_scan_partials<Op>, dim3(1), dim3(kPartialsThreads), 0, s, part, nb, d_total, op);
     CPVS_LAUNCH("scan_down", (k_scan_down<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, part, op);
 }
 
