@@ -78,7 +78,7 @@ def load_traffic(N):
     return out
 
 
-def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26):
+def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26, batch_lanes=8):
     """The tiled configuration: full^2 depth as (full/tile)^2 xy tiles, each built into full/tile
     z-tile DAGs (DeferredRenderer.cpp:150-187), one container with the top-level grid, and nq
     uniform lookups.  All 16 GiB of depth are generated on the device before timing."""
@@ -92,7 +92,7 @@ def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26):
     q = cp.gen_queries_uniform(SEED0 + 3, nq, 4)
     vis = cp.DeviceArray((nq,), np.uint8, ctx)
     c = cp.CompressedShadowContainer(L)
-    per_dag = []
+    per_dag, sample, sample_s = [], {}, 0.0
     t0 = time.perf_counter()
     for ty in range(L):
         for tx in range(L):
@@ -102,8 +102,13 @@ def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26):
             pyr_ms = ctx.elapsed_ms(e0, e1)
             for tz in range(L):
                 a = ctx.event(stream)
-                c.set(cp.CompressedShadow.create(mm, tz, L), tx, ty, tz)
+                sh = cp.CompressedShadow.create(mm, tz, L)
                 b = ctx.event(stream)
+                if len(per_dag) % 37 == 0:  # checked against the batched build below (time not counted)
+                    ts = time.perf_counter()
+                    sample[(tx, ty, tz)] = sh.getDAG()
+                    sample_s += time.perf_counter() - ts
+                c.set(sh, tx, ty, tz)
                 per_dag.append(ctx.elapsed_ms(a, b) + pyr_ms / L)
                 ctx.destroy_event(a)
                 ctx.destroy_event(b)
@@ -111,7 +116,17 @@ def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26):
             ctx.destroy_event(e1)
             mm.close()
     ctx.synchronize()
-    build_s = time.perf_counter() - t0
+    build_s = time.perf_counter() - t0 - sample_s
+    # The same 512 DAGs as one batch: each lane builds a tile's pyramid once and its z-tiles from
+    # it.  The words of every 37th plain DAG are compared with the batched one's.
+    keys = [(tx, ty, tz) for ty in range(L) for tx in range(L) for tz in range(L)]
+    t0 = time.perf_counter()
+    batch, _ = cp.build_batch([tiles[(tx, ty)] for tx, ty, _ in keys], [(tz, L) for _, _, tz in keys], lanes=batch_lanes)
+    ctx.synchronize()
+    batch_s = time.perf_counter() - t0
+    same = all(np.array_equal(batch[keys.index(k)].getDAG(), w) for k, w in sample.items())
+    for sh in batch:
+        sh.close()
     c.moveToGPU()
     dl, gl, words = c.info()
     for d in tiles.values():
@@ -133,6 +148,8 @@ def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26):
                         f"{L}^3 = {L ** 3} z-tile DAGs, {nq} uniform lookups",
             "dags": L ** 3, "dag_words": words, "grid_levels": gl,
             "build_s": round(build_s, 4), "Mtexels_per_s": round(full * full / build_s / 1e6, 1),
+            "batched_build_s": round(batch_s, 4), "batched_Mtexels_per_s": round(full * full / batch_s / 1e6, 1),
+            "batch_lanes": batch_lanes, "batched_words_identical": bool(same),
             "dag_build_ms_p50": round(pct(per_dag, 50), 3), "dag_build_ms_p99": round(pct(per_dag, 99), 3),
             "lookup_G_per_s": round(nq / (min(lk) * 1e-3) / 1e9, 3)}
 

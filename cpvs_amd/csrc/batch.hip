@@ -12,6 +12,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -63,17 +64,14 @@ int check_item(const cpvs_batch_item& it, uint32_t i) {
     return CPVS_OK;
 }
 
-// MinMaxHierarchy + CompressedShadow::create of one item on lane w (levels in the lane's arena).
-int build_item(cpvs_ctx* ctx, Worker& w, const cpvs_batch_item& it, int merge_mode, cpvs_dag** out) {
-    cpvs_pyramid p;
-    p.ctx = ctx;
-    p.N = it.N;
-    p.n = levels_of(it.N);
-    p.depth = const_cast<float*>(it.depth);
-    p.borrowed = true;
-    int rc = pyramid_build_levels(&p, false, w.stream, &w.levels);
-    if (rc) return rc;
-    return build_dag(ctx, w.arena, w.pinned, merge_mode, &p, it.zTileIndex, it.zTileNum, w.stream, out);
+// MinMaxHierarchy of one depth map on lane w (levels in the lane's arena).
+int build_pyramid(cpvs_ctx* ctx, Worker& w, const cpvs_batch_item& it, cpvs_pyramid* p) {
+    p->ctx = ctx;
+    p->N = it.N;
+    p->n = levels_of(it.N);
+    p->depth = const_cast<float*>(it.depth);
+    p->borrowed = true;
+    return pyramid_build_levels(p, false, w.stream, &w.levels);
 }
 
 int ensure_lanes(cpvs_ctx* ctx, uint32_t lanes) {
@@ -96,11 +94,24 @@ int run_batch(cpvs_ctx* ctx, const cpvs_batch_item* items, uint32_t count, uint3
     if (!count) return CPVS_OK;
     int rc = ensure_lanes(ctx, lanes);
     if (rc) return rc;
+    // Items of one depth map (same pointer and N: the z-tiles of a tile) form a group, built by one
+    // lane from one pyramid, as the reference's loop builds one MinMaxHierarchy per tile
+    // (src/DeferredRenderer.cpp:150-187).  Groups go out largest map first.
     std::vector<uint32_t> order(count);
     for (uint32_t i = 0; i < count; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        return (u64)items[a].N * items[a].N > (u64)items[b].N * items[b].N;
+        const u64 sa = (u64)items[a].N * items[a].N, sb = (u64)items[b].N * items[b].N;
+        if (sa != sb) return sa > sb;
+        if (items[a].depth != items[b].depth) return std::less<const float*>()(items[a].depth, items[b].depth);
+        return false;
     });
+    std::vector<uint32_t> group_start;  // group g is order[group_start[g], group_start[g + 1])
+    for (uint32_t j = 0; j < count; ++j) {
+        const cpvs_batch_item& a = items[order[j]];
+        if (j == 0 || a.depth != items[order[j - 1]].depth || a.N != items[order[j - 1]].N) group_start.push_back(j);
+    }
+    const uint32_t ngroups = (uint32_t)group_start.size();
+    group_start.push_back(count);
     std::atomic<uint32_t> next{0};
     std::atomic<bool> failed{false};
     std::mutex emu;
@@ -123,29 +134,38 @@ int run_batch(cpvs_ctx* ctx, const cpvs_batch_item* items, uint32_t count, uint3
             return;
         }
         while (!failed.load(std::memory_order_relaxed)) {
-            const uint32_t j = next.fetch_add(1, std::memory_order_relaxed);
-            if (j >= count) break;
-            const uint32_t i = order[j];
-            cpvs_dag* d = nullptr;
-            int lrc;
-            try {
-                lrc = build_item(ctx, *w, items[i], merge_mode, &d);
-            } catch (const std::bad_alloc&) {
-                set_error("host allocation failed");
-                lrc = CPVS_ENOMEM;
-            } catch (...) {
-                set_error("internal exception");
-                lrc = CPVS_EINTERNAL;
+            const uint32_t g = next.fetch_add(1, std::memory_order_relaxed);
+            if (g >= ngroups) break;
+            cpvs_pyramid p;
+            bool ok = true;
+            for (uint32_t j = group_start[g]; ok && j < group_start[g + 1]; ++j) {
+                const uint32_t i = order[j];
+                cpvs_dag* d = nullptr;
+                int lrc;
+                try {
+                    lrc = (j == group_start[g]) ? build_pyramid(ctx, *w, items[i], &p) : CPVS_OK;
+                    if (lrc == CPVS_OK)
+                        lrc = build_dag(ctx, w->arena, w->pinned, merge_mode, &p, items[i].zTileIndex, items[i].zTileNum,
+                                        w->stream, &d);
+                } catch (const std::bad_alloc&) {
+                    set_error("host allocation failed");
+                    lrc = CPVS_ENOMEM;
+                } catch (...) {
+                    set_error("internal exception");
+                    lrc = CPVS_EINTERNAL;
+                }
+                if (lrc != CPVS_OK) {
+                    char msg[600];
+                    snprintf(msg, sizeof(msg), "batch item %u: %s", i, cpvs_last_error());
+                    record(lrc, msg);
+                    ok = false;
+                    break;
+                }
+                out[i] = d;
+                if (done_ms)
+                    done_ms[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             }
-            if (lrc != CPVS_OK) {
-                char msg[600];
-                snprintf(msg, sizeof(msg), "batch item %u: %s", i, cpvs_last_error());
-                record(lrc, msg);
-                break;
-            }
-            out[i] = d;
-            if (done_ms)
-                done_ms[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (!ok) break;
         }
         (void)hipStreamSynchronize(w->stream);
     };

@@ -86,6 +86,23 @@ def test_batch_with_sort_merge(mix):
         c2.close()
 
 
+def test_batch_z_tiles_share_pyramids(ctx):
+    """Every z-tile of three maps in one batch, listed in shuffled order (the tiled configuration's
+    shape): a lane builds one pyramid per map and all of its z-tiles from it.  Words equal plain
+    builds for 1 and 3 lanes."""
+    _say("batch z-tiles of shared maps")
+    specs = [(512, 8, 8), (256, 4, 4), (1024, 16, 2)]  # N, boxes, zTileNum
+    Ds = [O.gen_depth(N, SEED0 + 77 * k, b, True) for k, (N, b, _) in enumerate(specs)]
+    depths = [cp.DeviceArray.from_numpy(D) for D in Ds]
+    items = [(k, t) for k, (_, _, T) in enumerate(specs) for t in range(T)]
+    np.random.default_rng(11).shuffle(items)
+    plain = {(k, t): _plain(Ds[k], t, specs[k][2])[1] for k, t in items}
+    for lanes in (1, 3):
+        shadows, _ = cp.build_batch([depths[k] for k, _ in items], [(t, specs[k][2]) for k, t in items], lanes=lanes)
+        for (k, t), sh in zip(items, shadows):
+            assert np.array_equal(sh.getDAG(), plain[(k, t)]), (k, t, lanes)
+
+
 def test_batch_many_small_maps(ctx):
     _say("batch 96 small maps")
     rng = np.random.default_rng(7)
