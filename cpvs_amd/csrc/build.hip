@@ -8,19 +8,20 @@
 //      texels and writes the 16-bit child mask; an exclusive scan of the partial-child counts
 //      gives each node's first child, and the scan's output stage writes the children's
 //      coordinates in (parent, child index) order -- the reference's breadth-first order.
-//   2. Leaves (level 2), column by column: one wave per 8x8 texel column (a level-3 texel),
-//      lane = texel.  The leaf nodes of a column are its partial level-3 cells, contiguous in z;
-//      a node equal to its z-neighbour (the common case along walls) is copied, not recomputed.
-//      The level-3 scan records where each (column, z) leaf sits in breadth-first order.
+//   2. Leaves (level 2): each wave takes 8 of the 8x8 texel columns (level-3 texels), lane =
+//      texel, one column at a time.  The leaf nodes of a column are its partial level-3 cells,
+//      contiguous in z; a node equal to its z-neighbour (the common case along walls) is copied,
+//      not recomputed.  The level-3 scan records where each (column, z) leaf sits in
+//      breadth-first order.
 //   3. Merge, bottom-up from the leaves to the root's children.  Key = (mask, partial leafmasks)
-//      or (mask, merged ranks of the partial children).  Default (a): a 64-bit key hash inserted
-//      into an open-addressing table (CAS on the hash, atomicMin of the node index), so every
-//      node finds the first occurrence of its key; the full key is compared against it, and a
-//      64-bit collision re-runs the build with (b).  (b): a stable LSD radix sort of (32-bit
-//      hash, node index), segment starts by a max-scan and a full-key compare against the
-//      segment head (a mismatch falls back to a scan of the segment).  Either way a scan of
-//      first-occurrence flags gives each node's rank = order of first occurrence, as mergeLevel's
-//      linear search yields (src/CompressedShadowUtil.h:154-182).
+//      or (mask, merged ranks of the partial children).  Default (a): the key's 64-bit hash goes
+//      into an open-addressing table whose 64-bit slots hold a hash tag and the smallest node
+//      index with it, so every node finds the first occurrence of its key; the full key is
+//      compared against it, and a collision re-runs the build with (b).  (b): a stable LSD radix
+//      sort of (32-bit hash, node index), segment starts by a max-scan and a full-key compare
+//      against the segment head (a mismatch falls back to a scan of the segment).  Either way a
+//      scan of first-occurrence flags gives each node's rank = order of first occurrence, as
+//      mergeLevel's linear search yields (src/CompressedShadowUtil.h:154-182).
 //   4. Compress: per level a scan of unique node sizes (1+k words, 1+2k at the leaf level),
 //      level bases root first, then one pass writes [mask, absolute child offsets] or
 //      [mask, lo, hi, ...] (src/CompressedShadow.cpp:326-392).
@@ -756,7 +757,7 @@ enum : int {
     kMetaB = 2 * kMaxLevels,             // level base offsets
     kMetaC = 3 * kMaxLevels,             // children emitted per level (scan totals)
     kMetaW = 4 * kMaxLevels,             // total words
-    kMetaColl = 4 * kMaxLevels + 1,      // 64-bit hash collisions (table merge)
+    kMetaColl = 4 * kMaxLevels + 1,      // key collisions in the hash table (table merge)
     kMetaRoot = 4 * kMaxLevels + 2,      // root mask
     kMetaSortColl = 4 * kMaxLevels + 3,  // 32-bit key collisions resolved by a scan (sort merge)
     kMetaWords = 4 * kMaxLevels + 4,

@@ -61,9 +61,9 @@ int cpvs_ctx_stream(cpvs_ctx *ctx, void **stream);
  * times the whole precompute with std::chrono, src/main.cpp:228-234.) */
 int cpvs_ctx_profile(cpvs_ctx *ctx, int mode);
 /* Selects how builds merge common subtrees, the replacement of mergeLevel's linear search
- * (src/CompressedShadowUtil.h:154-182): CPVS_MERGE_TABLE (default) or CPVS_MERGE_SORT.  A
- * 64-bit hash collision in the table path is detected by the full-key compare and the build is
- * redone with the sort path. */
+ * (src/CompressedShadowUtil.h:154-182): CPVS_MERGE_TABLE (default) or CPVS_MERGE_SORT.  Two
+ * different keys that the table path cannot tell apart (equal hash tags in one probe chain) are
+ * detected by the full-key compare, and the build is redone with the sort path. */
 int cpvs_ctx_set_merge(cpvs_ctx *ctx, int mode);
 int cpvs_ctx_profile_read(cpvs_ctx *ctx, char *buf, uint64_t cap);
 
