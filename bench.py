@@ -484,6 +484,26 @@ def main():
     build_rate = texels * len(build_ms) / (sum(build_ms) * 1e-3) / 1e6  # Mtexels/s
     lookup_rate = nq * len(lookup_ms) / (sum(lookup_ms) * 1e-3) / 1e9  # G lookups/s
 
+    # The reference's boundary takes a host image (MinMaxHierarchy(const ImageF&)): the same build
+    # from the depth map in pageable host memory, PCIe copy included, on the host clock.
+    depth_host = depth.numpy()
+    host_ms = []
+    for _ in range(4):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        mm = cp.MinMaxHierarchy(depth_host)
+        cs = cp.CompressedShadow.create(mm)
+        ctx.synchronize()
+        host_ms.append((time.perf_counter() - t0) * 1e3)
+        cs.close()
+        mm.close()
+    host_ms = host_ms[1:]  # the first run pays the host pages' first touch
+    del depth_host
+    host_input = {"build_ms_min": round(min(host_ms), 3), "build_ms_mean": round(statistics.mean(host_ms), 3),
+                  "Mtexels_per_s": round(texels / (min(host_ms) * 1e-3) / 1e6, 1),
+                  "note": "depth in pageable host memory: H2D copy + pyramid + create, host clock"}
+    log(f"[bench] host-input build: {host_input}")
+
     # Per-kernel timing (HIP events around every launch on the context stream), separate pass.
     ctx.profile(-1)
     ctx.profile(1)
@@ -590,6 +610,7 @@ def main():
         "config": {"workload": f"{N}x{N} depth map (plane + {args.boxes} boxes + 4-octave noise), "
                                f"{nq} G-buffer lookups ({args.width}x{args.height} x 2)",
                    "N": N, "zTileNum": 1, "queries": nq, "dag_words": words},
+        "host_input": host_input,
         "build_ms_mean": round(statistics.mean(build_ms), 3),
         "build_ms_p50": round(pct(build_ms, 50), 3),
         "build_ms_p99": round(pct(build_ms, 99), 3),
