@@ -244,11 +244,24 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
     uint4 ci = lc.info[min(c0 + (lane & (kLeafCols - 1)), clast)];  // lane k < kLeafCols: column c0 + k
     if (lane >= kLeafCols || c0 + lane >= ncol) ci = make_uint4(0u, 0u, 0u, 0u);
     float d[kLeafCols];
+    if (T > 1) {
+        // A column with no leaf node in this z-tile (most columns of a tile) is not read: its f
+        // values are never used.  With one z-tile every column has nodes, and the check would only
+        // cost issue slots.
 #pragma unroll
-    for (u32 k = 0; k < kLeafCols; ++k) {
-        const u32 c = min(c0 + k, clast);
-        const u32 x3 = c & (lc.w3 - 1), y3 = c >> lc.w3_shift;
-        d[k] = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
+        for (u32 k = 0; k < kLeafCols; ++k) {
+            const u32 c = min(c0 + k, clast);
+            const u32 x3 = c & (lc.w3 - 1), y3 = c >> lc.w3_shift;
+            d[k] = 0.0f;
+            if (__builtin_amdgcn_readlane(ci.x, k) != 0u) d[k] = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
+        }
+    } else {
+#pragma unroll
+        for (u32 k = 0; k < kLeafCols; ++k) {
+            const u32 c = min(c0 + k, clast);
+            const u32 x3 = c & (lc.w3 - 1), y3 = c >> lc.w3_shift;
+            d[k] = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
+        }
     }
     u32 b[kLeafCols];
 #pragma unroll
