@@ -1266,9 +1266,8 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         return CPVS_EINVAL;
     }
 
-    u32* d_words = nullptr;
-    if (hipMalloc(&d_words, sizeof(u32) * (total ? total : 1)) != hipSuccess) {
-        (void)hipGetLastError();
+    u32* d_words = static_cast<u32*>(ctx->buffers.take(sizeof(u32) * (total ? total : 1)));
+    if (!d_words) {
         set_error("build: hipMalloc of %llu DAG words failed", (unsigned long long)total);
         return CPVS_ENOMEM;
     }
@@ -1289,7 +1288,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
-        hipFree(d_words);
+        ctx->buffers.give(d_words);
         return hip_fail(e, "build write", __FILE__, __LINE__);
     }
 

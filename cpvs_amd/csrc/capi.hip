@@ -95,6 +95,7 @@ int cpvs_ctx_create(int device, cpvs_ctx** out) {
     CPVS_HIP_CHECK(hipSetDevice(device));
     cpvs_ctx* c = new cpvs_ctx();
     c->device = device;
+    c->buffers.device = device;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         c->num_cus = cus;
@@ -115,10 +116,65 @@ int cpvs_ctx_create(int device, cpvs_ctx** out) {
     GUARD_END
 }
 
+namespace cpvs {
+
+void* BufferCache::take(size_t bytes) {
+    bytes = bytes ? bytes : 1;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = free_.lower_bound(bytes);
+        if (it != free_.end() && it->first <= 2 * bytes) {
+            void* p = it->second;
+            live_[p] = it->first;
+            cached -= it->first;
+            free_.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        release();  // the cached buffers may be what is missing
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    live_[p] = bytes;
+    return p;
+}
+
+void BufferCache::give(void* p) {
+    if (!p) return;
+    (void)hipSetDevice(device);
+    (void)hipDeviceSynchronize();  // no kernel on any stream still reads p
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = live_.find(p);
+    const size_t bytes = (it != live_.end()) ? it->second : 0;
+    if (it != live_.end()) live_.erase(it);
+    if (bytes && cached + bytes <= kMaxBytes) {
+        free_.emplace(bytes, p);
+        cached += bytes;
+    } else {
+        (void)hipFree(p);
+    }
+}
+
+void BufferCache::release() {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+    cached = 0;
+}
+
+}  // namespace cpvs
+
 int cpvs_ctx_destroy(cpvs_ctx* ctx) {
     if (!ctx) return CPVS_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    ctx->buffers.release();
     ctx->arena.release();
     for (cpvs::Worker* w : ctx->lanes) {
         w->destroy();
@@ -229,7 +285,7 @@ int cpvs_pyramid_build(cpvs_ctx* ctx, const float* depth, int depth_on_device, u
     }
     if (rc != CPVS_OK) {
         if (!p->borrowed && p->depth) hipFree(p->depth);
-        if (p->levels && p->levels_owned) hipFree(p->levels);
+        if (p->levels && p->levels_owned) ctx->buffers.give(p->levels);
         delete p;
         return rc;
     }
@@ -290,7 +346,7 @@ int cpvs_pyramid_destroy(cpvs_pyramid* p) {
     if (!p) return CPVS_OK;
     hipSetDevice(p->ctx->device);
     if (!p->borrowed && p->depth) hipFree(p->depth);
-    if (p->levels && p->levels_owned) hipFree(p->levels);
+    if (p->levels && p->levels_owned) p->ctx->buffers.give(p->levels);
     delete p;
     return CPVS_OK;
 }
@@ -441,7 +497,7 @@ int cpvs_dag_traverse(cpvs_dag* d, const float* ndc, uint64_t count, int try_lea
 int cpvs_dag_destroy(cpvs_dag* d) {
     if (!d) return CPVS_OK;
     hipSetDevice(d->ctx->device);
-    if (d->d_words) hipFree(d->d_words);
+    if (d->d_words) d->ctx->buffers.give(d->d_words);
     delete d;
     return CPVS_OK;
 }

@@ -220,15 +220,14 @@ int cpvs_dag_deserialize(cpvs_ctx* ctx, const void* buf, uint64_t size, cpvs_dag
     std::vector<uint32_t> w(h.words);
     std::memcpy(w.data(), static_cast<const char*>(buf) + sizeof(FileHeader), 4 * h.words);
     CPVS_HIP_CHECK(hipSetDevice(ctx->device));
-    uint32_t* dw = nullptr;
-    if (hipMalloc(&dw, 4 * h.words) != hipSuccess) {
-        (void)hipGetLastError();
+    uint32_t* dw = static_cast<uint32_t*>(ctx->buffers.take(4 * h.words));
+    if (!dw) {
         set_error("dag_deserialize: hipMalloc of %llu words failed", (unsigned long long)h.words);
         return CPVS_ENOMEM;
     }
     const hipError_t e = hipMemcpy(dw, w.data(), 4 * h.words, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(dw);
+        ctx->buffers.give(dw);
         return hip_fail(e, "dag_deserialize copy", __FILE__, __LINE__);
     }
     cpvs_dag* d = new cpvs_dag();

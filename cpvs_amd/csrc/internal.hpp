@@ -2,8 +2,10 @@
 // each translation unit provides.
 #pragma once
 
+#include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/cpvs.h"
@@ -52,8 +54,27 @@ struct Worker {
 
 }  // namespace cpvs
 
+namespace cpvs {
+// Freed device buffers of DAG words and pyramid levels kept for reuse: a hipMalloc of a few MB costs
+// about 14 us of host time, during which a build's GPU work waits.  A buffer enters the cache only
+// after hipDeviceSynchronize (what hipFree guarantees too), so no kernel still reads it when it is
+// handed out again.  Best fit within twice the request; at most kMaxBytes are kept.
+struct BufferCache {
+    static constexpr size_t kMaxBytes = size_t(2) << 30;
+    std::mutex mu;
+    std::multimap<size_t, void*> free_;   // size -> cached buffer
+    std::unordered_map<void*, size_t> live_;  // handed-out buffer -> its size
+    size_t cached = 0;
+    int device = 0;
+    void* take(size_t bytes);   // nullptr when hipMalloc fails (even after emptying the cache)
+    void give(void* p);         // p from take(); null is ignored
+    void release();             // hipFree of every cached buffer
+};
+}  // namespace cpvs
+
 struct cpvs_ctx {
     int device = 0;
+    cpvs::BufferCache buffers;
     int num_cus = 256;  // compute units of the device (persistent grids size themselves to it)
     hipStream_t stream = nullptr;
     cpvs::Arena arena;

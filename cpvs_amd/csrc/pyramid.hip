@@ -147,8 +147,7 @@ int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* 
         if (rc) return rc;
         p->levels = into->take<float2>(total ? total : 1);
         p->levels_owned = false;
-    } else if (hipMalloc(&p->levels, sizeof(float2) * (total ? total : 1)) != hipSuccess) {
-        p->levels = nullptr;
+    } else if (!(p->levels = static_cast<float2*>(p->ctx->buffers.take(sizeof(float2) * (total ? total : 1))))) {
         set_error("pyramid: hipMalloc of %zu bytes failed", sizeof(float2) * total);
         return CPVS_ENOMEM;
     }
