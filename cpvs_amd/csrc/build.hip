@@ -660,7 +660,12 @@ __global__ void k_root_init(u32* rank, u32* uniq, u32* n_uniq) {
     *n_uniq = 1;
 }
 
-__global__ void k_set_u64(u64* dst, u64 v) { *dst = v; }
+// First launch after the scratch is laid out when no level is small: the root's coordinate, and
+// the meta words zeroed (k_small_topdown does both otherwise).
+__global__ void k_start(u64* root_coord, u64 root, u32* meta, u32 meta_words) {
+    *root_coord = root;
+    for (u32 i = 0; i < meta_words; ++i) meta[i] = 0u;
+}
 
 // ---------------------------------------------------------------------------------------
 // 4. Compress.
@@ -686,8 +691,10 @@ struct UniqueOffOut {
     }
 };
 
-__global__ void k_level_bases(const u32* __restrict__ S, u32* __restrict__ B, u32* __restrict__ total, int R, int bottom) {
+__global__ void k_level_bases(const u32* __restrict__ S, u32* __restrict__ B, u32* __restrict__ total, int R, int bottom,
+                              const u32* __restrict__ root_mask, u32* __restrict__ root_mask_out) {
     // B[R] = 0, B[L-1] = B[L] + S[L]; total = B[bottom] + S[bottom] (levels absent have S = 0).
+    *root_mask_out = *root_mask;
     unsigned long long acc = 0;
     for (int L = R; L >= bottom; --L) {
         B[L] = (u32)acc;
@@ -829,6 +836,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_topdown(SmallArgs a, co
                                                                  u64 root, LeafColumns cols) {
     __shared__ u32 s_wave[kSmallThreads / kWave];
     if (threadIdx.x == 0) a.coord[a.R][0] = root;  // create(): root offset (0, 0, 2*zTileIndex)
+    for (u32 i = threadIdx.x; i < kMetaWords; i += kSmallThreads) a.meta[i] = 0u;  // the first launch to use them
     __syncthreads();
     for (int L = a.R; L >= a.Ls; --L) {
         const u32 c = a.count[L];
@@ -975,6 +983,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_compress(SmallArgs a) {
         if (threadIdx.x == 0) a.meta[kMetaS + L] = total;
     }
     if (threadIdx.x == 0) {
+        a.meta[kMetaRoot] = a.mask[a.R][0];
         unsigned long long acc = 0;
         for (int L = a.R; L >= a.deepest; --L) {
             a.meta[kMetaB + L] = (u32)acc;
@@ -1074,7 +1083,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         rc = pyramid_node_counts(p, t, T, reinterpret_cast<unsigned long long*>(A.base), colinfo, s);
         if (rc) return rc;
     }
-    CPVS_HIP_CHECK(hipMemsetAsync(meta, 0, sizeof(u32) * kMetaWords, s));
+    // meta is zeroed by the first kernel below (k_small_topdown or k_start).
 
     Level lv[kMaxLevels];
     for (int L = bottom; L <= R; ++L) {
@@ -1146,7 +1155,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     if (Ls <= R)
         CPVS_LAUNCH("small_topdown", k_small_topdown, dim3(1), dim3(kSmallThreads), 0, s, sa, p->depth, N, T, root, lc);
     else
-        CPVS_LAUNCH("set_u64", k_set_u64, dim3(1), dim3(1), 0, s, lv[R].coord, root);
+        CPVS_LAUNCH("start", k_start, dim3(1), dim3(1), 0, s, lv[R].coord, root, meta, (u32)kMetaWords);
     for (int L = top_big; L >= deepest; --L) {
         const u32 c = lv[L].count;
         if (leaf && L == 2) {
@@ -1240,8 +1249,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         CPVS_LAUNCH("small_compress", k_small_compress, dim3(1), dim3(kSmallThreads), 0, s, sa);
     else
         CPVS_LAUNCH("level_bases", k_level_bases, dim3(1), dim3(1), 0, s, meta + kMetaS, meta + kMetaB, meta + kMetaW, R,
-                    deepest);
-    CPVS_HIP_CHECK(hipMemcpyAsync(meta + kMetaRoot, lv[R].mask, sizeof(u32), hipMemcpyDeviceToDevice, s));
+                    deepest, lv[R].mask, meta + kMetaRoot);
     CPVS_HIP_CHECK(hipGetLastError());
     CPVS_HIP_CHECK(hipMemcpyAsync(pinned, meta, sizeof(u32) * kMetaWords, hipMemcpyDeviceToHost, s));
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
