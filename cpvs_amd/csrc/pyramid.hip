@@ -226,9 +226,16 @@ __global__ __launch_bounds__(256) void k_node_counts(CountArgs a) {
     for (u32 r = 0; r < kCountPerThread; ++r) {
         const u32 j = base + r * 256 + threadIdx.x;
         if (j < w * w) {
+            // Cell z is partial iff !(z + 1 <= min*h) && !(z >= max*h), i.e. z in
+            // [floor(min*h), ceil(max*h)).  The bounds are clipped to the slab in float before the
+            // conversion: a NaN bound fails its compare and leaves that side open (as it does in
+            // the classification), infinities and huge values clip.
             const float2 v = a.lv[k][j];
             const float lo = floorf(v.x * h), hi = ceilf(v.y * h);
-            const int s = max(zlo, (int)lo), e = min(zhi, (int)hi);
+            const float fz0 = (float)zlo, fz1 = (float)zhi;
+            const float slo = (lo > fz0) ? ((lo < fz1) ? lo : fz1) : fz0;
+            const float shi = (hi < fz1) ? ((hi > fz0) ? hi : fz0) : fz1;
+            const int s = (int)slo, e = (int)shi;
             const u32 c = (e > s) ? (u32)(e - s) : 0u;
             if (colc) colc[j] = make_uint4(c, 0u, (u32)max(s, 0), 0u);
             acc += c;

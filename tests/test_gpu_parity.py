@@ -188,6 +188,77 @@ def test_uniform_and_early_ending_maps(ctx, value):
         assert cs.getTotalVisibility() == cp.CompressedShadow.SHADOW and len(w) == 1
 
 
+def _special_map(rng, N):
+    D = rng.random((N, N), dtype=np.float32)
+    for frac, val in ((0.05, np.nan), (0.03, np.inf), (0.03, -np.inf), (0.05, -0.5), (0.05, 3.0),
+                      (0.02, 1e30), (0.02, -1e30), (0.03, 1.0), (0.03, 0.0)):
+        D[rng.random((N, N)) < frac] = val
+    return D
+
+
+@pytest.mark.parametrize("N,T", [(64, 1), (64, 2), (256, 1), (256, 3)])
+def test_special_depth_values_bit_exact(ctx, N, T):
+    # NaN, infinities, negative, above-one and huge depths.  The pyramid's min/max keep a NaN only
+    # in the first operand, a NaN bound fails every classification compare (the cell is partial),
+    # and a NaN texel is never lit.
+    _say("special values", N, T)
+    rng = np.random.default_rng(7 + N + T)
+    nan_first = rng.random((N, N), dtype=np.float32)
+    nan_first[::2, ::2] = np.nan  # NaN where it reaches every pyramid level
+    maps = [_special_map(rng, N), nan_first, np.full((N, N), np.nan, np.float32),
+            np.full((N, N), np.inf, np.float32), np.full((N, N), -1e30, np.float32)]
+    for mi, D in enumerate(maps):
+        pyr = O.Pyramid(D)
+        mm = cp.MinMaxHierarchy(D, all_levels=True)
+        for L in range(1, mm.getNumLevels()):
+            assert np.array_equal(mm.getLevel(L).view(np.uint32), pyr.level(L).view(np.uint32)), (mi, L)
+        for t in range(T):
+            cs = cp.CompressedShadow.create(mm, t, T)
+            wr, nr = O.create(pyr, t, T, merge_mode=1)
+            assert cs.getNumLevels() == nr
+            assert np.array_equal(cs.getDAG(), wr), (mi, t)
+            check_dag(wr, nr)
+
+
+def test_tall_wall_columns_bit_exact(ctx):
+    # Columns whose 8x8 texel block holds both 0 and 1 are partial over the whole z range: 128
+    # leaf nodes per column at N = 1024, more than one 64-node chunk of the leaf kernel.
+    _say("tall walls")
+    N = 1024
+    rng = np.random.default_rng(11)
+    D = rng.random((N, N), dtype=np.float32)
+    walls = rng.random((N // 8, N // 8)) < 0.2
+    by, bx = np.nonzero(walls)
+    D[8 * by, 8 * bx] = 0.0
+    D[8 * by + 7, 8 * bx + 7] = 1.0
+    pyr = O.Pyramid(D)
+    for T in (1, 2):
+        for t in range(T):
+            _, w, n = _gpu_dag(D, t, T)
+            wr, nr = O.create(pyr, t, T, merge_mode=1)
+            assert np.array_equal(w, wr), (T, t)
+
+
+def test_lookups_with_special_coordinates(ctx):
+    # traverse.cs getPathFromNDC: NaN and negative positions go to cell 0, positions at or past
+    # the far edge (infinities included) to the last cell.
+    _say("special lookups")
+    N = 256
+    D = O.gen_depth(N, SEED0 + 2, 8, True)
+    cs = cp.CompressedShadow.create(cp.MinMaxHierarchy(D))
+    n, words = cs.getNumLevels(), cs.getDAG()
+    c = cp.CompressedShadowContainer(cs)
+    c.moveToGPU()
+    rng = np.random.default_rng(3)
+    q = (rng.random((65536, 4), dtype=np.float32) * 2.4 - 1.2).astype(np.float32)
+    special = np.array([np.nan, np.inf, -np.inf, -1.0, 1.0, 1e30, -1e30, 0.0], np.float32)
+    pick = rng.random(q.shape) < 0.2
+    q[pick] = special[rng.integers(0, special.size, q.shape)][pick]
+    vis = c.evaluate(cp.DeviceArray.from_numpy(q)).numpy()
+    exp = O.evaluate(words, O.top_level_grid([words]), n, 0, q, nthreads=8)
+    assert np.array_equal(vis, exp)
+
+
 def test_hash_collision_path_exercised_by_many_duplicates(ctx):
     # A periodic map yields huge groups of identical subtrees at every level.
     N = 512
