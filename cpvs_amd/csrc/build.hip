@@ -691,16 +691,30 @@ struct UniqueOffOut {
     }
 };
 
-__global__ void k_level_bases(const u32* __restrict__ S, u32* __restrict__ B, u32* __restrict__ total, int R, int bottom,
-                              const u32* __restrict__ root_mask, u32* __restrict__ root_mask_out) {
+// Device metadata block (u32 words).
+enum : int {
+    kMetaU = 0,                          // uniques per level
+    kMetaS = kMaxLevels,                 // words per level
+    kMetaB = 2 * kMaxLevels,             // level base offsets
+    kMetaC = 3 * kMaxLevels,             // children emitted per level (scan totals)
+    kMetaW = 4 * kMaxLevels,             // total words
+    kMetaColl = 4 * kMaxLevels + 1,      // key collisions in the hash table (table merge)
+    kMetaRoot = 4 * kMaxLevels + 2,      // root mask
+    kMetaSortColl = 4 * kMaxLevels + 3,  // 32-bit key collisions resolved by a scan (sort merge)
+    kMetaWords = 4 * kMaxLevels + 4,
+};
+
+__global__ void k_level_bases(u32* __restrict__ meta, int R, int bottom, const u32* __restrict__ root_mask,
+                              u32* __restrict__ host_meta) {
     // B[R] = 0, B[L-1] = B[L] + S[L]; total = B[bottom] + S[bottom] (levels absent have S = 0).
-    *root_mask_out = *root_mask;
+    meta[kMetaRoot] = *root_mask;
     unsigned long long acc = 0;
     for (int L = R; L >= bottom; --L) {
-        B[L] = (u32)acc;
-        acc += S[L];
+        meta[kMetaB + L] = (u32)acc;
+        acc += meta[kMetaS + L];
     }
-    *total = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
+    meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
+    for (u32 i = 0; i < kMetaWords; ++i) host_meta[i] = meta[i];  // the build's readback
 }
 
 // Unique node r of an inner level: [mask, absolute offsets of its partial children].
@@ -770,19 +784,6 @@ struct Level {
     u32* off = nullptr;
 };
 
-// Device metadata block (u32 words).
-enum : int {
-    kMetaU = 0,                          // uniques per level
-    kMetaS = kMaxLevels,                 // words per level
-    kMetaB = 2 * kMaxLevels,             // level base offsets
-    kMetaC = 3 * kMaxLevels,             // children emitted per level (scan totals)
-    kMetaW = 4 * kMaxLevels,             // total words
-    kMetaColl = 4 * kMaxLevels + 1,      // key collisions in the hash table (table merge)
-    kMetaRoot = 4 * kMaxLevels + 2,      // root mask
-    kMetaSortColl = 4 * kMaxLevels + 3,  // 32-bit key collisions resolved by a scan (sort merge)
-    kMetaWords = 4 * kMaxLevels + 4,
-};
-
 // ---------------------------------------------------------------------------------------
 // Small levels in one workgroup.  The upper octree levels hold few nodes (at 8192^2 the levels
 // above 6 hold a few thousand together), yet each costs about ten launches of the per-level
@@ -809,6 +810,7 @@ struct SmallArgs {
     u32* off[kMaxLevels];
     const float2* lv[kMaxLevels];
     u32* meta;
+    u32* host_meta;  // pinned host copy of meta, written when the compress step is done
 };
 
 // Exclusive sum over the workgroup, one value per thread; *total = the sum.
@@ -990,6 +992,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_compress(SmallArgs a) {
             acc += a.meta[kMetaS + L];
         }
         a.meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
+        for (u32 i = 0; i < kMetaWords; ++i) a.host_meta[i] = a.meta[i];  // the build's readback
     }
 }
 
@@ -1027,19 +1030,18 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     // --- exact node counts (one readback) ---------------------------------------------
     const u32 w3 = leaf ? (N >> 3) : 0;
     const u32 ncol = w3 * w3;
-    const size_t meta_bytes = arena_bytes(kMaxLevels, sizeof(unsigned long long)) + arena_bytes(kMetaWords, sizeof(u32));
+    const size_t meta_bytes = arena_bytes(kMetaWords, sizeof(u32));
     const size_t col_bytes = leaf ? arena_bytes(ncol, sizeof(uint4)) : 0;
     int rc = A.reserve(meta_bytes + col_bytes, s);
     if (rc) return rc;
-    unsigned long long* d_counts = A.take<unsigned long long>(kMaxLevels);
     uint4* colinfo = leaf ? A.take<uint4>(ncol) : nullptr;
-    rc = pyramid_node_counts(p, t, T, d_counts, colinfo, s);
+    CountLayout lay;
+    rc = pyramid_node_counts(p, t, T, pinned_partials(pinned), colinfo, s, &lay);
     if (rc) return rc;
-    CPVS_HIP_CHECK(hipMemcpyAsync(pinned, d_counts, sizeof(u64) * kMaxLevels, hipMemcpyDeviceToHost, s));
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
     u64 cnt[kMaxLevels] = {};
+    sum_node_counts(lay, pinned_partials(pinned), cnt);
     cnt[R] = 1;
-    for (int L = bottom; L < R; ++L) cnt[L] = pinned[L];
     for (int L = bottom; L <= R; ++L) {
         if (cnt[L] > 0xFFFFFFF0ull) {
             set_error("build: level %d has %llu nodes (limit 2^32)", L, (unsigned long long)cnt[L]);
@@ -1076,11 +1078,10 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     bool moved = false;
     rc = A.reserve(need, s, &moved);  // keeps the contents when large enough
     if (rc) return rc;
-    (void)A.take<unsigned long long>(kMaxLevels);
     colinfo = leaf ? A.take<uint4>(ncol) : nullptr;
     u32* meta = A.take<u32>(kMetaWords);
     if (leaf && moved) {  // the arena was reallocated: count the columns again
-        rc = pyramid_node_counts(p, t, T, reinterpret_cast<unsigned long long*>(A.base), colinfo, s);
+        rc = pyramid_node_counts(p, t, T, nullptr, colinfo, s);
         if (rc) return rc;
     }
     // meta is zeroed by the first kernel below (k_small_topdown or k_start).
@@ -1141,6 +1142,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     sa.Ls = Ls;
     sa.deepest = deepest;
     sa.meta = meta;
+    sa.host_meta = reinterpret_cast<u32*>(pinned);
     for (int L = bottom; L <= R; ++L) {
         sa.count[L] = lv[L].count;
         sa.coord[L] = lv[L].coord;
@@ -1248,11 +1250,10 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     if (Ls <= R)
         CPVS_LAUNCH("small_compress", k_small_compress, dim3(1), dim3(kSmallThreads), 0, s, sa);
     else
-        CPVS_LAUNCH("level_bases", k_level_bases, dim3(1), dim3(1), 0, s, meta + kMetaS, meta + kMetaB, meta + kMetaW, R,
-                    deepest, lv[R].mask, meta + kMetaRoot);
+        CPVS_LAUNCH("level_bases", k_level_bases, dim3(1), dim3(1), 0, s, meta, R, deepest, lv[R].mask,
+                    reinterpret_cast<u32*>(pinned));
     CPVS_HIP_CHECK(hipGetLastError());
-    CPVS_HIP_CHECK(hipMemcpyAsync(pinned, meta, sizeof(u32) * kMetaWords, hipMemcpyDeviceToHost, s));
-    CPVS_HIP_CHECK(hipStreamSynchronize(s));
+    CPVS_HIP_CHECK(hipStreamSynchronize(s));  // the last kernel copied meta into pinned
     u32 hm[kMetaWords];
     std::memcpy(hm, pinned, sizeof(hm));
 

@@ -105,7 +105,7 @@ int cpvs_ctx_create(int device, cpvs_ctx** out) {
         return hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
     }
     c->pinned_words = kPinnedWords;
-    e = hipHostMalloc(reinterpret_cast<void**>(&c->pinned), sizeof(u64) * c->pinned_words, hipHostMallocDefault);
+    e = hipHostMalloc(reinterpret_cast<void**>(&c->pinned), kPinnedBytes, hipHostMallocDefault);
     if (e != hipSuccess) {
         hipStreamDestroy(c->stream);
         delete c;
@@ -279,7 +279,12 @@ int cpvs_pyramid_build(cpvs_ctx* ctx, const float* depth, int depth_on_device, u
         }
     }
     int rc = pyramid_build_levels(p, (flags & CPVS_PYR_ALL_LEVELS) != 0, s);
-    if (rc == CPVS_OK) {
+    // A borrowed device depth map on the context's own stream needs no wait here: builds on that
+    // stream run after the levels, and everything else waits for them (pyramid_wait).  A copied
+    // input is waited for, so the caller may reuse its buffer, and so is a caller's stream, which
+    // the caller may destroy.
+    p->pending = p->borrowed && s == ctx->stream;
+    if (rc == CPVS_OK && !p->pending) {
         const hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) rc = hip_fail(e, "pyramid sync", __FILE__, __LINE__);
     }
@@ -306,6 +311,7 @@ int cpvs_pyramid_get(const cpvs_pyramid* p, uint32_t level, uint32_t x, uint32_t
     const uint32_t w = p->N >> level;
     CHECK_ARG(x < w && y < w, "pyramid_get: (%u,%u) outside level %u of width %u", x, y, level, w);
     CPVS_HIP_CHECK(hipSetDevice(p->ctx->device));
+    CPVS_HIP_CHECK(pyramid_wait(p, nullptr));
     if (level == 0) {
         float v;
         CPVS_HIP_CHECK(hipMemcpy(&v, p->depth + (size_t)y * p->N + x, sizeof(float), hipMemcpyDeviceToHost));
@@ -328,6 +334,7 @@ int cpvs_pyramid_copy_level(const cpvs_pyramid* p, uint32_t level, float* host, 
     CHECK_ARG((int)level < p->n, "pyramid_copy_level: level %u >= %d", level, p->n);
     const size_t w = p->N >> level;
     CPVS_HIP_CHECK(hipSetDevice(p->ctx->device));
+    CPVS_HIP_CHECK(pyramid_wait(p, nullptr));
     if (level == 0) {
         CHECK_ARG(cap >= w * w, "pyramid_copy_level: capacity %llu < %zu", (unsigned long long)cap, w * w);
         CPVS_HIP_CHECK(hipMemcpy(host, p->depth, sizeof(float) * w * w, hipMemcpyDeviceToHost));
@@ -372,18 +379,16 @@ int cpvs_estimate_build(cpvs_ctx* ctx, const cpvs_pyramid* p, uint32_t t, uint32
     std::lock_guard<std::mutex> lk(ctx->mu);
     CPVS_HIP_CHECK(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
-    rc = ctx->arena.reserve(arena_bytes(kMaxLevels, 8), s);
+    CPVS_HIP_CHECK(pyramid_wait(p, s));
+    CountLayout lay;
+    rc = pyramid_node_counts(p, t, T, pinned_partials(ctx->pinned), nullptr, s, &lay);
     if (rc) return rc;
-    unsigned long long* d_counts = ctx->arena.take<unsigned long long>(kMaxLevels);
-    rc = pyramid_node_counts(p, t, T, d_counts, nullptr, s);
-    if (rc) return rc;
-    CPVS_HIP_CHECK(hipMemcpyAsync(ctx->pinned, d_counts, sizeof(u64) * kMaxLevels, hipMemcpyDeviceToHost, s));
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
     const int n = p->n, R = n - 2;
     const int bottom = (n >= 5) ? 2 : 0;
     u64 cnt[kMaxLevels] = {};
+    sum_node_counts(lay, pinned_partials(ctx->pinned), cnt);
     cnt[R] = 1;
-    for (int L = bottom; L < R; ++L) cnt[L] = ctx->pinned[L];
     u64 cmax = 1, per_level = 0, words = 0;
     for (int L = bottom; L <= R; ++L) {
         const u64 c = cnt[L] ? cnt[L] : 1;
@@ -421,7 +426,9 @@ int cpvs_build(cpvs_ctx* ctx, const cpvs_pyramid* p, uint32_t t, uint32_t T, voi
     std::lock_guard<std::mutex> lk(ctx->mu);
     ProfInstall prof(ctx->prof_on ? &ctx->prof : nullptr);
     CPVS_HIP_CHECK(hipSetDevice(ctx->device));
-    return build_dag(ctx, ctx->arena, ctx->pinned, ctx->merge_mode, p, t, T, pick_stream(ctx, stream), out);
+    hipStream_t s = pick_stream(ctx, stream);
+    CPVS_HIP_CHECK(pyramid_wait(p, s));
+    return build_dag(ctx, ctx->arena, ctx->pinned, ctx->merge_mode, p, t, T, s, out);
     GUARD_END
 }
 

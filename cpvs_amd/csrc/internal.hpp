@@ -39,7 +39,12 @@ struct Arena {
 
 inline size_t arena_bytes(size_t count, size_t elem) { return (count * elem + 255) & ~(size_t)255; }
 
-constexpr size_t kPinnedWords = 256;  // pinned readback staging per stream lane (u64 words)
+// Pinned host staging per stream lane: kPinnedWords u64 words of readback staging, then
+// kPinnedPartials u32 node-count partials (one per count workgroup, written by the kernel).
+constexpr size_t kPinnedWords = 256;
+constexpr size_t kPinnedPartials = 65536;
+constexpr size_t kPinnedBytes = sizeof(u64) * kPinnedWords + sizeof(u32) * kPinnedPartials;
+inline u32* pinned_partials(u64* pinned) { return reinterpret_cast<u32*>(pinned + kPinnedWords); }
 
 // One build lane of a batch: a stream with its own scratch arena, pyramid-level arena and
 // pinned readback staging.  Lanes run concurrently from their own host threads.
@@ -95,6 +100,7 @@ struct cpvs_pyramid {
     float* depth = nullptr;    // N*N floats (owned unless borrowed)
     bool borrowed = false;
     bool levels_owned = true;  // false when the levels live in a batch lane's arena
+    bool pending = false;      // the levels may still be in flight on the context's stream
     float2* levels = nullptr;  // one allocation holding every stored level
     float2* lv[cpvs::kMaxLevels] = {};  // per-level pointer (null when not stored)
 };
@@ -130,10 +136,25 @@ namespace cpvs {
 // pyramid.hip.  Levels go to a new allocation owned by p, or, when `into` is given, to that
 // arena (p then does not own them; the arena is reset).
 int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* into = nullptr);
-// Exact node counts per level (counts[L] = nodes at octree level L) for z-tile t of T.
-// col_info (may be null) receives, per level-3 texel (leafmask mode), its partial-cell count (.x)
-// and first partial z cell within the z-tile (.z).
-int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, uint4* col_info, hipStream_t s);
+// Exact node counts per level for z-tile t of T.  Each count workgroup writes its partial count
+// to partials[block] (pinned host memory, kPinnedPartials entries; may be null), and
+// sum_node_counts adds them up per level once the stream has reached that point: cnt[L] = nodes
+// at octree level L.  col_info (may be null) receives, per level-3 texel (leafmask mode), its
+// partial-cell count (.x) and first partial z cell within the z-tile (.z).
+// Before work on stream s reads p's levels: a pyramid still in flight on the context's stream is
+// waited for when s is another stream.
+inline hipError_t pyramid_wait(const cpvs_pyramid* p, hipStream_t s) {
+    return (p->pending && s != p->ctx->stream) ? hipStreamSynchronize(p->ctx->stream) : hipSuccess;
+}
+
+struct CountLayout {
+    int nlev = 0;
+    int level[kMaxLevels] = {};            // pyramid level of each block range
+    u32 block_start[kMaxLevels + 1] = {};  // first block of each range
+};
+int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, u32* partials, uint4* col_info, hipStream_t s,
+                        CountLayout* layout = nullptr);
+void sum_node_counts(const CountLayout& lay, const u32* partials, u64* cnt);
 
 // Admission hook of a build: called once the exact node counts are known and before any scratch
 // or DAG memory is taken, with the scratch bytes the build needs (its arena grows only if it

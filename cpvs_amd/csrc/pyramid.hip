@@ -203,7 +203,7 @@ struct CountArgs {
     int level[kMaxLevels];
     const float2* lv[kMaxLevels];
     u32 N, T, t;
-    unsigned long long* counts;  // counts[L-1] += partial cells of level L
+    u32* partials;               // partials[block] = the block's partial cells (host memory; may be null)
     uint4* col_info;             // per level-3 texel (leaf column): x = partial cells, z = first partial z
                                  // (y is left for the column's start); may be null
 };
@@ -241,18 +241,26 @@ __global__ __launch_bounds__(256) void k_node_counts(CountArgs a) {
             acc += c;
         }
     }
-    // Block reduction, then one 64-bit atomic per block.
+    // Block reduction; every block writes its own partial (a block stays within one level and
+    // sums at most kCountTile columns of at most N/8 cells, so 32 bits hold it).  The host adds
+    // the partials up: no counter to clear before the launch and no copy after it.
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
     if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const u32 tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
-        if (tot) atomicAdd(&a.counts[L - 1], (unsigned long long)tot);
+    if (threadIdx.x == 0 && a.partials) a.partials[blockIdx.x] = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+}
+
+void sum_node_counts(const CountLayout& lay, const u32* partials, u64* cnt) {
+    for (int k = 0; k < lay.nlev; ++k) {
+        u64 c = 0;
+        for (u32 b = lay.block_start[k]; b < lay.block_start[k + 1]; ++b) c += partials[b];
+        cnt[lay.level[k] - 1] = c;  // partial cells of level L are the nodes of octree level L-1
     }
 }
 
-int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long* d_counts, uint4* col_info, hipStream_t s) {
+int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, u32* partials, uint4* col_info, hipStream_t s,
+                        CountLayout* layout) {
     CountArgs a = {};
     const int n = p->n;
     const int R = n - 2;
@@ -260,7 +268,7 @@ int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long*
     a.N = p->N;
     a.T = T;
     a.t = t;
-    a.counts = d_counts;
+    a.partials = partials;
     a.col_info = (n >= 5) ? col_info : nullptr;
     u32 blocks = 0;
     for (int L = Lmin; L <= R; ++L) {
@@ -277,7 +285,15 @@ int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, unsigned long long*
         ++a.nlev;
     }
     a.block_start[a.nlev] = blocks;
-    CPVS_HIP_CHECK(hipMemsetAsync(d_counts, 0, sizeof(unsigned long long) * kMaxLevels, s));
+    if (partials && blocks > kPinnedPartials) {
+        set_error("node counts: %u count blocks exceed the %zu pinned partials", blocks, kPinnedPartials);
+        return CPVS_EINTERNAL;
+    }
+    if (layout) {
+        layout->nlev = a.nlev;
+        for (int k = 0; k < a.nlev; ++k) layout->level[k] = a.level[k];
+        for (int k = 0; k <= a.nlev; ++k) layout->block_start[k] = a.block_start[k];
+    }
     if (blocks) CPVS_LAUNCH("node_counts", k_node_counts, dim3(blocks), dim3(256), 0, s, a);
     CPVS_HIP_CHECK(hipGetLastError());
     return CPVS_OK;

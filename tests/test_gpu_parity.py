@@ -1,5 +1,6 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle: bit-exact pyramids, DAG
 words, visibilities and generator output.  Needs a GPU."""
+import ctypes
 import hashlib
 import json
 import os
@@ -55,6 +56,33 @@ def test_pyramid_reference_known_answers(ctx, ref_fixtures):  # MinMaxTest.cpp:4
         _, L, x, y, which, exp, tol = e
         v = mm.getMin(L, x, y) if which == "min" else mm.getMax(L, x, y)
         assert (v == np.float32(exp)) if tol == 0 else abs(v - exp) < tol, e
+
+
+def test_borrowed_pyramid_waits_for_its_levels(ctx):
+    # A pyramid over a borrowed device depth map is built on the context's stream without a host
+    # wait.  Level reads and builds on another stream must still see the finished levels.
+    _say("borrowed pyramid, other stream")
+    N = 1024
+    D = O.gen_depth(N, SEED0 + 7, 32, True)
+    ref = O.Pyramid(D)
+    wr, nr = O.create(ref, 0, 1, merge_mode=1)
+    d = cp.DeviceArray.from_numpy(D)
+    mm = cp.MinMaxHierarchy(d, borrow=True, all_levels=True)
+    for L in range(1, mm.getNumLevels()):
+        assert np.array_equal(mm.getLevel(L).view(np.uint32), ref.level(L).view(np.uint32)), L
+    mm.close()
+    hip = ctypes.CDLL("libamdhip64.so")
+    st = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(st)) == 0
+    try:
+        for _ in range(3):
+            mm = cp.MinMaxHierarchy(d, borrow=True)
+            cs = cp.CompressedShadow.create(mm, stream=st)
+            assert cs.getNumLevels() == nr and np.array_equal(cs.getDAG(), wr)
+            cs.close()
+            mm.close()
+    finally:
+        hip.hipStreamDestroy(st)
 
 
 def test_pyramid_levels_1_2_need_flag(ctx):
