@@ -106,6 +106,27 @@ struct ColStartOut {
     uint4* info;
     __device__ void operator()(u32 i, u32 ex, u32) const { info[i].y = ex; }
 };
+// With several z-tiles most columns have no leaf node in a given tile.  One scan of
+// (column has nodes ? 1 : 0) << 32 | nodes gives the column starts and the list of the columns
+// with nodes, in column order (the halves cannot carry: node counts stay below 2^32).
+struct ColPairIn {
+    const uint4* info;
+    __device__ u64 operator()(u32 i) const {
+        const u32 c = info[i].x;
+        return (c ? (1ull << 32) : 0ull) | c;
+    }
+};
+struct ColListOut {
+    uint4* info;
+    u32* list;   // list[k] = the k-th column with nodes
+    u32* nlist;  // device: the number of such columns
+    u32 n;
+    __device__ void operator()(u32 i, u64 ex, u64 v) const {
+        info[i].y = (u32)ex;
+        if ((u32)v) list[(u32)(ex >> 32)] = i;
+        if (i == n - 1) *nlist = (u32)((ex + v) >> 32);
+    }
+};
 
 // Scan functors: partial-child count in, (first child index, children coordinates) out.
 struct ChildCountIn {
@@ -224,25 +245,36 @@ constexpr u32 kXcds = 8;
 constexpr u32 kLeafCols = 8;
 constexpr u32 kLeafWaves = 4;
 
+// kList (several z-tiles): the wave's columns are col_list[c0 .. c0+7], the z-tile's columns with
+// leaf nodes in column order, *n_list of them; ncol (at most the leaf node count) bounds the grid.
+// Otherwise the wave's columns are c0 .. c0+7 of all ncol.
+template <bool kList>
 __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, u32 ncol, u32 nleaf,
                                                                 const float* __restrict__ depth, u32 N, u32 T,
                                                                 u32* __restrict__ mask_out, u64* __restrict__ leafm,
                                                                 u64* __restrict__ hkey, u32* __restrict__ runhead,
                                                                 u32* __restrict__ leafslot, u32 unit_chunk,
-                                                                u64* __restrict__ tab, u32 tab_slots) {
+                                                                u64* __restrict__ tab, u32 tab_slots,
+                                                                const u32* __restrict__ col_list,
+                                                                const u32* __restrict__ n_list) {
     // Per lane: f_t and the breadth-first index of node `lane` of each of the wave's columns (a lane
     // only ever reads back its own entries, so the arrays need no barrier).
     __shared__ int s_f[kLeafWaves][kLeafCols][64];
     __shared__ u32 s_b[kLeafWaves][kLeafCols][64];
     const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // Workgroups are dealt to the XCDs in turn; XCD x takes the contiguous units [x*unit_chunk, ...).
-    const u32 unit = (blockIdx.x % kXcds) * unit_chunk + blockIdx.x / kXcds;
+    // A column list fills only its first units, so there consecutive units go to different XCDs.
+    const u32 unit = kList ? blockIdx.x : (blockIdx.x % kXcds) * unit_chunk + blockIdx.x / kXcds;
     const u32 c0 = (unit * kLeafWaves + wave) * kLeafCols;
     // Every load below is unconditional (addresses clamped into range, values masked afterwards), so
     // the compiler counts them exactly and waits only for what each step uses.
-    const u32 clast = ncol - 1;
-    uint4 ci = lc.info[min(c0 + (lane & (kLeafCols - 1)), clast)];  // lane k < kLeafCols: column c0 + k
-    if (lane >= kLeafCols || c0 + lane >= ncol) ci = make_uint4(0u, 0u, 0u, 0u);
+    const u32 nvalid = kList ? min(*n_list, ncol) : ncol;  // columns of this launch
+    const u32 clast = kList ? (nvalid ? nvalid - 1 : 0u) : ncol - 1;
+    // lane k < kLeafCols: column c0 + k of the launch (list entries past nvalid are never read)
+    const u32 my_slot = min(c0 + (lane & (kLeafCols - 1)), clast);
+    const u32 my_col = kList ? (nvalid ? col_list[my_slot] : 0u) : my_slot;
+    uint4 ci = lc.info[my_col];
+    if (lane >= kLeafCols || c0 + lane >= nvalid) ci = make_uint4(0u, 0u, 0u, 0u);
     float d[kLeafCols];
     if (T > 1) {
         // A column with no leaf node in this z-tile (most columns of a tile) is not read: its f
@@ -250,7 +282,7 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
         // cost issue slots.
 #pragma unroll
         for (u32 k = 0; k < kLeafCols; ++k) {
-            const u32 c = min(c0 + k, clast);
+            const u32 c = kList ? __builtin_amdgcn_readlane(my_col, k) : min(c0 + k, clast);
             const u32 x3 = c & (lc.w3 - 1), y3 = c >> lc.w3_shift;
             d[k] = 0.0f;
             if (__builtin_amdgcn_readlane(ci.x, k) != 0u) d[k] = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
@@ -258,7 +290,7 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
     } else {
 #pragma unroll
         for (u32 k = 0; k < kLeafCols; ++k) {
-            const u32 c = min(c0 + k, clast);
+            const u32 c = kList ? __builtin_amdgcn_readlane(my_col, k) : min(c0 + k, clast);
             const u32 x3 = c & (lc.w3 - 1), y3 = c >> lc.w3_shift;
             d[k] = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
         }
@@ -280,7 +312,7 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
     }
 
     for (u32 k = 0; k < kLeafCols; ++k) {
-        if (c0 + k >= ncol) break;
+        if (c0 + k >= nvalid) break;
         const u32 cnt = __builtin_amdgcn_readlane(ci.x, k);
         if (cnt == 0) continue;
         const u32 base = __builtin_amdgcn_readlane(ci.y, k);
@@ -1062,6 +1094,8 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     const size_t c2 = leaf ? (cnt[2] ? cnt[2] : 1) : 1;
     if (leaf) need += arena_bytes(c2 * 8, 8) + arena_bytes(c2, 8) + 3 * arena_bytes(c2, 4);
     need += 2 * arena_bytes(cmax, 4);
+    const bool col_list = leaf && T > 1;  // compact the z-tile's columns with leaf nodes
+    if (col_list) need += arena_bytes(ncol, 4) + arena_bytes(1, 4);
     if (use_table) need += arena_bytes(S, 8);
     else need += 4 * arena_bytes(cmax, 4) + arena_bytes(sort_scratch_words(cmax), 4);
     need += arena_bytes(scan_scratch_words(scan_n), 4);
@@ -1102,6 +1136,8 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     u32* runhead = leaf ? A.take<u32>(c2) : nullptr;
     u32* leafslot = leaf ? A.take<u32>(c2) : nullptr;
     u32* colbfs = leaf ? A.take<u32>(c2) : nullptr;
+    u32* clist = col_list ? A.take<u32>(ncol) : nullptr;
+    u32* nclist = col_list ? A.take<u32>(1) : nullptr;
     u32* rep = A.take<u32>(cmax);
     u32* tslot = A.take<u32>(cmax);  // table slot of each node, or segment starts when sorting
     u32* seg = tslot;
@@ -1126,7 +1162,10 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         lc.bfs = colbfs;
         lc.w3 = w3;
         lc.w3_shift = (u32)__builtin_ctz(w3);
-        device_scan(ColCountIn{colinfo}, ColStartOut{colinfo}, ncol, OpSum{}, nullptr, scan_scratch, s);
+        if (col_list)
+            device_scan(ColPairIn{colinfo}, ColListOut{colinfo, clist, nclist, ncol}, ncol, OpSum64{}, nullptr, scan_scratch, s);
+        else
+            device_scan(ColCountIn{colinfo}, ColStartOut{colinfo}, ncol, OpSum{}, nullptr, scan_scratch, s);
     }
 
     // --- 1/2. BFS classification -------------------------------------------------------------
@@ -1161,12 +1200,20 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     for (int L = top_big; L >= deepest; --L) {
         const u32 c = lv[L].count;
         if (leaf && L == 2) {
-            const u32 units = ceil_div<u32>(ncol, kLeafWaves * kLeafCols);
+            // With a column list, at most min(columns, leaf nodes) columns have nodes.
+            const u32 bound = col_list ? std::min(ncol, c) : ncol;
+            const u32 units = ceil_div<u32>(bound, kLeafWaves * kLeafCols);
             const u32 unit_chunk = ceil_div<u32>(units, kXcds);
+            const u32 slots = use_table ? table_slots(c) : 0u;
             // The leaf level is the first merged: its table is cleared by this kernel.
-            CPVS_LAUNCH("leaf_waves", k_leaf_waves, dim3(unit_chunk * kXcds), dim3(64 * kLeafWaves), 0, s, lc, ncol, c,
-                        p->depth, N, T, lv[2].mask, leafm, hkey, runhead, leafslot, unit_chunk, tab,
-                        use_table ? table_slots(c) : 0u);
+            if (col_list)
+                CPVS_LAUNCH("leaf_waves", k_leaf_waves<true>, dim3(unit_chunk * kXcds), dim3(64 * kLeafWaves), 0, s, lc,
+                            bound, c, p->depth, N, T, lv[2].mask, leafm, hkey, runhead, leafslot, unit_chunk, tab, slots,
+                            clist, nclist);
+            else
+                CPVS_LAUNCH("leaf_waves", k_leaf_waves<false>, dim3(unit_chunk * kXcds), dim3(64 * kLeafWaves), 0, s, lc,
+                            bound, c, p->depth, N, T, lv[2].mask, leafm, hkey, runhead, leafslot, unit_chunk, tab, slots,
+                            nullptr, nullptr);
             break;
         }
         CPVS_LAUNCH("classify", k_classify, grid1d(c), dim3(256), 0, s, lv[L].coord, c, p->lv[L], p->depth, N, L, T,
