@@ -1074,6 +1074,39 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     u64 cnt[kMaxLevels] = {};
     sum_node_counts(lay, pinned_partials(pinned), cnt);
     cnt[R] = 1;
+    if (cnt[R - 1] == 0) {
+        // No partial child of the root (a z-tile above or below every surface, a flat map): the
+        // DAG is the root's childmask alone, written without any further kernel.
+        const u32 word = pinned_partials(pinned)[lay.block_start[lay.nlev]];
+        if (admit) {
+            rc = admit->fn(admit->user, A, 0, sizeof(u32));
+            if (rc) return rc;
+        }
+        u32* d_words = static_cast<u32*>(ctx->buffers.take(sizeof(u32)));
+        if (!d_words) {
+            set_error("build: hipMalloc of one DAG word failed");
+            return CPVS_ENOMEM;
+        }
+        const hipError_t e = hipMemcpy(d_words, &word, sizeof(u32), hipMemcpyHostToDevice);  // an idle buffer
+        if (e != hipSuccess) {
+            ctx->buffers.give(d_words);
+            return hip_fail(e, "build root word", __FILE__, __LINE__);
+        }
+        cpvs_dag* d = new cpvs_dag();
+        d->ctx = ctx;
+        d->n = n;
+        d->words = 1;
+        d->d_words = d_words;
+        d->root_mask = word;
+        d->N = N;
+        d->zTileIndex = t;
+        d->zTileNum = T;
+        d->merge_mode = merge_mode;
+        d->nodes[R] = 1;
+        d->uniques[R] = 1;
+        *out = d;
+        return CPVS_OK;
+    }
     for (int L = bottom; L <= R; ++L) {
         if (cnt[L] > 0xFFFFFFF0ull) {
             set_error("build: level %d has %llu nodes (limit 2^32)", L, (unsigned long long)cnt[L]);

@@ -139,7 +139,7 @@ int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* 
 // Exact node counts per level for z-tile t of T.  Each count workgroup writes its partial count
 // to partials[block] (pinned host memory, kPinnedPartials entries; may be null), and
 // sum_node_counts adds them up per level once the stream has reached that point: cnt[L] = nodes
-// at octree level L.  col_info (may be null) receives, per level-3 texel (leafmask mode), its
+// at octree level L.  partials[block_start[nlev]] receives the root's childmask.  col_info (may be null) receives, per level-3 texel (leafmask mode), its
 // partial-cell count (.x) and first partial z cell within the z-tile (.z).
 // Before work on stream s reads p's levels: a pyramid still in flight on the context's stream is
 // waited for when s is another stream.

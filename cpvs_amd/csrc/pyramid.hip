@@ -248,7 +248,22 @@ __global__ __launch_bounds__(256) void k_node_counts(CountArgs a) {
     for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
     if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0 && a.partials) a.partials[blockIdx.x] = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    if (threadIdx.x == 0 && a.partials) {
+        a.partials[blockIdx.x] = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        // The root level (2x2 texels, one block): the root's childmask follows the partials.  With
+        // no partial child the DAG is that one word.  Cells z = 2t, 2t+1 of level R, as
+        // classify_node reads them for the root (0, 0, 2t).
+        if (k == a.nlev - 1) {
+            const float2* l = a.lv[k];
+            u32 m = 0;
+#pragma unroll
+            for (u32 ch = 0; ch < 8; ++ch) {
+                const float2 v = l[ch & 3];
+                m |= classify_cell(2 * a.t + (ch >> 2), v.x * h, v.y * h) << (2 * ch);
+            }
+            a.partials[a.block_start[a.nlev]] = m;
+        }
+    }
 }
 
 void sum_node_counts(const CountLayout& lay, const u32* partials, u64* cnt) {
@@ -285,7 +300,7 @@ int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, u32* partials, uint
         ++a.nlev;
     }
     a.block_start[a.nlev] = blocks;
-    if (partials && blocks > kPinnedPartials) {
+    if (partials && blocks + 1 > kPinnedPartials) {
         set_error("node counts: %u count blocks exceed the %zu pinned partials", blocks, kPinnedPartials);
         return CPVS_EINTERNAL;
     }
