@@ -23,7 +23,7 @@ namespace cpvs {
 
 int Worker::create() {
     CPVS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&pinned), kPinnedBytes, hipHostMallocDefault);
+    const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&pinned), kPinnedBytes, kPinnedFlags);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(stream);
         stream = nullptr;

@@ -747,6 +747,7 @@ __global__ void k_level_bases(u32* __restrict__ meta, int R, int bottom, const u
     }
     meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
     for (u32 i = 0; i < kMetaWords; ++i) host_meta[i] = meta[i];  // the build's readback
+    __threadfence_system();
 }
 
 // Unique node r of an inner level: [mask, absolute offsets of its partial children].
@@ -779,11 +780,10 @@ __global__ __launch_bounds__(256) void k_write_inner(const u32* __restrict__ uni
     write_inner_node(r, uniq, mask, cstart, off, base, child_rank, child_off, child_base, out);
 }
 
-__global__ __launch_bounds__(256) void k_write_leaf(const u32* __restrict__ uniq, u32 n_uniq, const u32* __restrict__ mask,
-                                                    const u64* __restrict__ leafm, const u32* __restrict__ leafslot,
-                                                    const u32* __restrict__ off, u32 base, u32* __restrict__ out) {
-    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_uniq) return;
+// Unique leaf node r: [mask, (lo, hi) of each partial slice's leafmask].
+__device__ inline void write_leaf_node(u32 r, const u32* __restrict__ uniq, const u32* __restrict__ mask,
+                                       const u64* __restrict__ leafm, const u32* __restrict__ leafslot,
+                                       const u32* __restrict__ off, u32 base, u32* __restrict__ out) {
     const u32 i = uniq[r];
     const u32 o = base + off[r];
     const u32 m = mask[i];
@@ -799,6 +799,13 @@ __global__ __launch_bounds__(256) void k_write_leaf(const u32* __restrict__ uniq
             out[o + 1 + 2 * j] = (u32)v[j];
             out[o + 2 + 2 * j] = (u32)(v[j] >> 32);
         }
+}
+
+__global__ __launch_bounds__(256) void k_write_leaf(const u32* __restrict__ uniq, u32 n_uniq, const u32* __restrict__ mask,
+                                                    const u64* __restrict__ leafm, const u32* __restrict__ leafslot,
+                                                    const u32* __restrict__ off, u32 base, u32* __restrict__ out) {
+    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n_uniq) write_leaf_node(r, uniq, mask, leafm, leafslot, off, base, out);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -843,6 +850,7 @@ struct SmallArgs {
     const float2* lv[kMaxLevels];
     u32* meta;
     u32* host_meta;  // pinned host copy of meta, written when the compress step is done
+    u32 sizes_in_merge;  // every larger level has its word offsets: k_small_merge finishes compress
 };
 
 // Exclusive sum over the workgroup, one value per thread; *total = the sum.
@@ -944,11 +952,14 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
         }
         __syncthreads();
         const InnerEq eq{a.mask[L], a.cstart[L], child_rank};
-        u32 rep[kSmallPer], nfirst = 0;
+        // One scan of (first occurrences << 16 | their words) gives ranks and word offsets (a level
+        // of at most kSmallMax nodes of at most 9 words keeps the low half below 2^16).
+        u32 rep[kSmallPer], sz[kSmallPer], packed = 0;
 #pragma unroll
         for (u32 e = 0; e < kSmallPer; ++e) {
             const u32 i = i0 + e;
             rep[e] = i;
+            sz[e] = 0;
             if (e < E && i < c) {
                 u32 r = (u32)tab[slot[e]];
                 if (r != i && !eq(i, r)) {
@@ -960,21 +971,30 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
                         }
                 }
                 rep[e] = r;
-                nfirst += (r == i) ? 1u : 0u;
+                if (r == i) {
+                    sz[e] = 1u + num_children(a.mask[L][i]);
+                    packed += (1u << 16) + sz[e];
+                }
             }
         }
         u32 total;
-        u32 ex = small_scan(nfirst, s_wave, &total);
+        const u32 ex = small_scan(packed, s_wave, &total);
+        u32 rk = ex >> 16, of = ex & 0xFFFFu;
 #pragma unroll
         for (u32 e = 0; e < kSmallPer; ++e) {
             const u32 i = i0 + e;
             if (e < E && i < c && rep[e] == i) {
-                a.rank[L][i] = ex;
-                a.uniq[L][ex] = i;
-                ++ex;
+                a.rank[L][i] = rk;
+                a.uniq[L][rk] = i;
+                if (a.sizes_in_merge) a.off[L][rk] = of;
+                ++rk;
+                of += sz[e];
             }
         }
-        if (threadIdx.x == 0) a.meta[kMetaU + L] = total;
+        if (threadIdx.x == 0) {
+            a.meta[kMetaU + L] = total >> 16;
+            if (a.sizes_in_merge) a.meta[kMetaS + L] = total & 0xFFFFu;
+        }
         __syncthreads();  // first occurrences ranked
 #pragma unroll
         for (u32 e = 0; e < kSmallPer; ++e) {
@@ -987,6 +1007,21 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
         a.rank[a.R][0] = 0;
         a.uniq[a.R][0] = 0;
         a.meta[kMetaU + a.R] = 1;
+        if (a.sizes_in_merge) {
+            // What k_small_compress does otherwise: the root's size, the level bases, the readback.
+            const u32 root = a.mask[a.R][0];
+            a.off[a.R][0] = 0;
+            a.meta[kMetaS + a.R] = 1u + num_children(root);
+            a.meta[kMetaRoot] = root;
+            unsigned long long acc = 0;
+            for (int L = a.R; L >= a.deepest; --L) {
+                a.meta[kMetaB + L] = (u32)acc;
+                acc += a.meta[kMetaS + L];
+            }
+            a.meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
+            for (u32 i = 0; i < kMetaWords; ++i) a.host_meta[i] = a.meta[i];
+            __threadfence_system();
+        }
     }
 }
 
@@ -1025,6 +1060,32 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_compress(SmallArgs a) {
         }
         a.meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
         for (u32 i = 0; i < kMetaWords; ++i) a.host_meta[i] = a.meta[i];  // the build's readback
+        __threadfence_system();
+    }
+}
+
+// The leaf level's words (blocks [0, leaf_blocks)) and the small levels' words (kSmallWriteBlocks
+// more blocks striding over each small level's uniques) in one launch: the levels' words are
+// independent, and the small levels alone would be a one-workgroup launch of their own.
+constexpr u32 kSmallWriteBlocks = 4;
+__global__ __launch_bounds__(256) void k_write_leaf_small(const u32* __restrict__ uniq, u32 n_uniq, const u32* __restrict__ mask,
+                                                          const u64* __restrict__ leafm, const u32* __restrict__ leafslot,
+                                                          const u32* __restrict__ off, u32 base, u32 leaf_blocks, SmallArgs a,
+                                                          u32* __restrict__ out) {
+    if (blockIdx.x < leaf_blocks) {
+        const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+        if (r < n_uniq) write_leaf_node(r, uniq, mask, leafm, leafslot, off, base, out);
+        return;
+    }
+    const u32 t0 = (blockIdx.x - leaf_blocks) * blockDim.x + threadIdx.x, nt = kSmallWriteBlocks * blockDim.x;
+    for (int L = a.R; L >= a.Ls; --L) {
+        const u32 U = a.meta[kMetaU + L], lbase = a.meta[kMetaB + L];
+        const bool kids = L > a.deepest;
+        const u32* child_rank = kids ? a.rank[L - 1] : nullptr;
+        const u32* child_off = kids ? a.off[L - 1] : nullptr;
+        const u32 child_base = kids ? a.meta[kMetaB + L - 1] : 0u;
+        for (u32 r = t0; r < U; r += nt)
+            write_inner_node(r, a.uniq[L], a.mask[L], a.cstart[L], a.off[L], lbase, child_rank, child_off, child_base, out);
     }
 }
 
@@ -1265,6 +1326,11 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         const u64 per_node = (leaf && L == 2) ? 17ull : 9ull;
         return L >= deepest && L < std::min(R, Ls) && (u64)lv[L].count * per_node < 0xFFFFFFFFull;
     };
+    // With every larger level's offsets known from its ranks scan, the small-level merge also
+    // computes the small levels' offsets and all level bases (no separate compress launch).
+    sa.sizes_in_merge = 1u;
+    for (int L = deepest; L < std::min(R, Ls); ++L)
+        if (!fused_sizes(L)) sa.sizes_in_merge = 0u;
 
     // --- 3. merge, bottom-up (the root is never merged) --------------------------------------
     for (int L = deepest; L < std::min(R, Ls); ++L) {
@@ -1327,11 +1393,12 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         device_scan(UniqueSizeIn{lv[L].uniq, lv[L].mask, meta + kMetaU + L, is_leaf},
                     UniqueOffOut{lv[L].off, meta + kMetaU + L}, c, OpSum{}, meta + kMetaS + L, scan_scratch, s);
     }
-    if (Ls <= R)
-        CPVS_LAUNCH("small_compress", k_small_compress, dim3(1), dim3(kSmallThreads), 0, s, sa);
-    else
+    if (Ls <= R) {
+        if (!sa.sizes_in_merge) CPVS_LAUNCH("small_compress", k_small_compress, dim3(1), dim3(kSmallThreads), 0, s, sa);
+    } else {
         CPVS_LAUNCH("level_bases", k_level_bases, dim3(1), dim3(1), 0, s, meta, R, deepest, lv[R].mask,
                     reinterpret_cast<u32*>(pinned));
+    }
     CPVS_HIP_CHECK(hipGetLastError());
     CPVS_HIP_CHECK(hipStreamSynchronize(s));  // the last kernel copied meta into pinned
     u32 hm[kMetaWords];
@@ -1360,13 +1427,20 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         set_error("build: hipMalloc of %llu DAG words failed", (unsigned long long)total);
         return CPVS_ENOMEM;
     }
-    if (Ls <= R) CPVS_LAUNCH("small_write", k_small_write, dim3(1), dim3(kSmallThreads), 0, s, sa, d_words);
+    // With a leaf level the small levels are written by the leaf write launch.
+    const bool small_in_leaf_write = Ls <= R && leaf && deepest == 2;
+    if (Ls <= R && !small_in_leaf_write)
+        CPVS_LAUNCH("small_write", k_small_write, dim3(1), dim3(kSmallThreads), 0, s, sa, d_words);
     for (int L = top_big; L >= deepest; --L) {
         const u32 U = hm[kMetaU + L];
         const u32 base = hm[kMetaB + L];
         if (leaf && L == 2) {
-            CPVS_LAUNCH("write_leaf", k_write_leaf, grid1d(U), dim3(256), 0, s, lv[2].uniq, U, lv[2].mask, leafm, leafslot, lv[2].off,
-                        base, d_words);
+            if (small_in_leaf_write)
+                CPVS_LAUNCH("write_leaf", k_write_leaf_small, dim3(grid1d(U).x + kSmallWriteBlocks), dim3(256), 0, s, lv[2].uniq, U,
+                            lv[2].mask, leafm, leafslot, lv[2].off, base, grid1d(U).x, sa, d_words);
+            else
+                CPVS_LAUNCH("write_leaf", k_write_leaf, grid1d(U), dim3(256), 0, s, lv[2].uniq, U, lv[2].mask, leafm, leafslot,
+                            lv[2].off, base, d_words);
         } else {
             const bool has_children = L > deepest;
             CPVS_LAUNCH("write_inner", k_write_inner, grid1d(U), dim3(256), 0, s, lv[L].uniq, U, lv[L].mask, lv[L].cstart,

@@ -105,7 +105,7 @@ int cpvs_ctx_create(int device, cpvs_ctx** out) {
         return hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
     }
     c->pinned_words = kPinnedWords;
-    e = hipHostMalloc(reinterpret_cast<void**>(&c->pinned), kPinnedBytes, hipHostMallocDefault);
+    e = hipHostMalloc(reinterpret_cast<void**>(&c->pinned), kPinnedBytes, kPinnedFlags);
     if (e != hipSuccess) {
         hipStreamDestroy(c->stream);
         delete c;

@@ -45,6 +45,10 @@ constexpr size_t kPinnedWords = 256;
 constexpr size_t kPinnedPartials = 65536;
 constexpr size_t kPinnedBytes = sizeof(u64) * kPinnedWords + sizeof(u32) * kPinnedPartials;
 inline u32* pinned_partials(u64* pinned) { return reinterpret_cast<u32*>(pinned + kPinnedWords); }
+// Kernels store node counts and meta words straight into the staging, and the host reads them
+// after a stream synchronize.  Coherent host memory keeps those stores out of the GPU's caches
+// (the default, non-coherent kind may hold them in L2 past the kernel's end).
+constexpr unsigned kPinnedFlags = hipHostMallocCoherent | hipHostMallocMapped;
 
 // One build lane of a batch: a stream with its own scratch arena, pyramid-level arena and
 // pinned readback staging.  Lanes run concurrently from their own host threads.
