@@ -771,15 +771,6 @@ __device__ inline void write_inner_node(u32 r, const u32* __restrict__ uniq, con
         if (j < k) out[o + 1 + j] = child_base + co[j];
 }
 
-__global__ __launch_bounds__(256) void k_write_inner(const u32* __restrict__ uniq, u32 n_uniq, const u32* __restrict__ mask,
-                                                     const u32* __restrict__ cstart, const u32* __restrict__ off, u32 base,
-                                                     const u32* __restrict__ child_rank, const u32* __restrict__ child_off,
-                                                     u32 child_base, u32* __restrict__ out) {
-    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_uniq) return;
-    write_inner_node(r, uniq, mask, cstart, off, base, child_rank, child_off, child_base, out);
-}
-
 // Unique leaf node r: [mask, (lo, hi) of each partial slice's leafmask].
 __device__ inline void write_leaf_node(u32 r, const u32* __restrict__ uniq, const u32* __restrict__ mask,
                                        const u64* __restrict__ leafm, const u32* __restrict__ leafslot,
@@ -801,11 +792,36 @@ __device__ inline void write_leaf_node(u32 r, const u32* __restrict__ uniq, cons
         }
 }
 
-__global__ __launch_bounds__(256) void k_write_leaf(const u32* __restrict__ uniq, u32 n_uniq, const u32* __restrict__ mask,
-                                                    const u64* __restrict__ leafm, const u32* __restrict__ leafslot,
-                                                    const u32* __restrict__ off, u32 base, u32* __restrict__ out) {
-    const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n_uniq) write_leaf_node(r, uniq, mask, leafm, leafslot, off, base, out);
+// Every level's words in one launch: once the offsets and bases are known the levels' words are
+// independent, and a launch per level costs a launch each.  Level k takes the blocks
+// [first[k], first[k+1]), one unique node per thread.
+struct WriteLevel {
+    const u32* uniq;
+    const u32* mask;
+    const u32* cstart;
+    const u32* off;
+    const u32* child_rank;  // null at the deepest level
+    const u32* child_off;
+    u32 U, base, child_base, leaf;
+};
+struct WriteArgs {
+    int nlev;
+    u32 first[kMaxLevels + 1];
+    WriteLevel lv[kMaxLevels];
+    const u64* leafm;
+    const u32* leafslot;
+};
+
+__global__ __launch_bounds__(256) void k_write_levels(WriteArgs a, u32* __restrict__ out) {
+    int k = 0;
+    while (k + 1 < a.nlev && blockIdx.x >= a.first[k + 1]) ++k;
+    const WriteLevel& l = a.lv[k];
+    const u32 r = (blockIdx.x - a.first[k]) * blockDim.x + threadIdx.x;
+    if (r >= l.U) return;
+    if (l.leaf)
+        write_leaf_node(r, l.uniq, l.mask, a.leafm, a.leafslot, l.off, l.base, out);
+    else
+        write_inner_node(r, l.uniq, l.mask, l.cstart, l.off, l.base, l.child_rank, l.child_off, l.child_base, out);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1061,43 +1077,6 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_compress(SmallArgs a) {
         a.meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
         for (u32 i = 0; i < kMetaWords; ++i) a.host_meta[i] = a.meta[i];  // the build's readback
         __threadfence_system();
-    }
-}
-
-// The leaf level's words (blocks [0, leaf_blocks)) and the small levels' words (kSmallWriteBlocks
-// more blocks striding over each small level's uniques) in one launch: the levels' words are
-// independent, and the small levels alone would be a one-workgroup launch of their own.
-constexpr u32 kSmallWriteBlocks = 4;
-__global__ __launch_bounds__(256) void k_write_leaf_small(const u32* __restrict__ uniq, u32 n_uniq, const u32* __restrict__ mask,
-                                                          const u64* __restrict__ leafm, const u32* __restrict__ leafslot,
-                                                          const u32* __restrict__ off, u32 base, u32 leaf_blocks, SmallArgs a,
-                                                          u32* __restrict__ out) {
-    if (blockIdx.x < leaf_blocks) {
-        const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
-        if (r < n_uniq) write_leaf_node(r, uniq, mask, leafm, leafslot, off, base, out);
-        return;
-    }
-    const u32 t0 = (blockIdx.x - leaf_blocks) * blockDim.x + threadIdx.x, nt = kSmallWriteBlocks * blockDim.x;
-    for (int L = a.R; L >= a.Ls; --L) {
-        const u32 U = a.meta[kMetaU + L], lbase = a.meta[kMetaB + L];
-        const bool kids = L > a.deepest;
-        const u32* child_rank = kids ? a.rank[L - 1] : nullptr;
-        const u32* child_off = kids ? a.off[L - 1] : nullptr;
-        const u32 child_base = kids ? a.meta[kMetaB + L - 1] : 0u;
-        for (u32 r = t0; r < U; r += nt)
-            write_inner_node(r, a.uniq[L], a.mask[L], a.cstart[L], a.off[L], lbase, child_rank, child_off, child_base, out);
-    }
-}
-
-__global__ __launch_bounds__(kSmallThreads) void k_small_write(SmallArgs a, u32* __restrict__ out) {
-    for (int L = a.R; L >= a.Ls; --L) {
-        const u32 U = a.meta[kMetaU + L], base = a.meta[kMetaB + L];
-        const bool kids = L > a.deepest;
-        const u32* child_rank = kids ? a.rank[L - 1] : nullptr;
-        const u32* child_off = kids ? a.off[L - 1] : nullptr;
-        const u32 child_base = kids ? a.meta[kMetaB + L - 1] : 0u;
-        for (u32 r = threadIdx.x; r < U; r += kSmallThreads)
-            write_inner_node(r, a.uniq[L], a.mask[L], a.cstart[L], a.off[L], base, child_rank, child_off, child_base, out);
     }
 }
 
@@ -1427,27 +1406,28 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         set_error("build: hipMalloc of %llu DAG words failed", (unsigned long long)total);
         return CPVS_ENOMEM;
     }
-    // With a leaf level the small levels are written by the leaf write launch.
-    const bool small_in_leaf_write = Ls <= R && leaf && deepest == 2;
-    if (Ls <= R && !small_in_leaf_write)
-        CPVS_LAUNCH("small_write", k_small_write, dim3(1), dim3(kSmallThreads), 0, s, sa, d_words);
-    for (int L = top_big; L >= deepest; --L) {
-        const u32 U = hm[kMetaU + L];
-        const u32 base = hm[kMetaB + L];
-        if (leaf && L == 2) {
-            if (small_in_leaf_write)
-                CPVS_LAUNCH("write_leaf", k_write_leaf_small, dim3(grid1d(U).x + kSmallWriteBlocks), dim3(256), 0, s, lv[2].uniq, U,
-                            lv[2].mask, leafm, leafslot, lv[2].off, base, grid1d(U).x, sa, d_words);
-            else
-                CPVS_LAUNCH("write_leaf", k_write_leaf, grid1d(U), dim3(256), 0, s, lv[2].uniq, U, lv[2].mask, leafm, leafslot,
-                            lv[2].off, base, d_words);
-        } else {
-            const bool has_children = L > deepest;
-            CPVS_LAUNCH("write_inner", k_write_inner, grid1d(U), dim3(256), 0, s, lv[L].uniq, U, lv[L].mask, lv[L].cstart,
-                        lv[L].off, base, has_children ? lv[L - 1].rank : nullptr, has_children ? lv[L - 1].off : nullptr,
-                        has_children ? hm[kMetaB + L - 1] : 0u, d_words);
-        }
+    WriteArgs wa{};
+    u32 wblocks = 0;
+    for (int L = R; L >= deepest; --L) {
+        WriteLevel& w = wa.lv[wa.nlev];
+        const bool kids = L > deepest;
+        w.uniq = lv[L].uniq;
+        w.mask = lv[L].mask;
+        w.cstart = lv[L].cstart;
+        w.off = lv[L].off;
+        w.child_rank = kids ? lv[L - 1].rank : nullptr;
+        w.child_off = kids ? lv[L - 1].off : nullptr;
+        w.U = hm[kMetaU + L];
+        w.base = hm[kMetaB + L];
+        w.child_base = kids ? hm[kMetaB + L - 1] : 0u;
+        w.leaf = (leaf && L == 2) ? 1u : 0u;
+        wa.first[wa.nlev++] = wblocks;
+        wblocks += ceil_div<u32>(w.U, 256);
     }
+    wa.first[wa.nlev] = wblocks;
+    wa.leafm = leafm;
+    wa.leafslot = leafslot;
+    if (wblocks) CPVS_LAUNCH("write_levels", k_write_levels, dim3(wblocks), dim3(256), 0, s, wa, d_words);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
