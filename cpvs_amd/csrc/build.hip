@@ -133,6 +133,22 @@ struct ChildCountIn {
     const u32* mask;
     __device__ u32 operator()(u32 i) const { return num_children(mask[i]); }
 };
+// A level's classification inside its emit scan: the input classifies node i, and the output
+// (which classifies it again from the same cached loads) stores the mask.  One launch per level
+// fewer than a separate classify pass.
+struct Classify {
+    const u64* coord;
+    const float2* lv;
+    const float* depth;
+    u32 N;
+    int L;
+    u32 T;
+    __device__ u32 operator()(u32 i) const { return classify_node(coord[i], lv, depth, N, L, T); }
+};
+struct ClassifyCountIn {
+    Classify cls;
+    __device__ u32 operator()(u32 i) const { return num_children(cls(i)); }
+};
 struct EmitChildrenOut {
     const u32* mask;
     const u64* coord;
@@ -140,14 +156,17 @@ struct EmitChildrenOut {
     u64* child_coord;   // used when the child level is classified node by node
     u32 child_cap;      // nodes of the child level (exact count); writes beyond it are dropped
     LeafColumns cols;   // used when cols.bfs != null (the children are leaves)
+    u32* mask_out = nullptr;  // when set, the node is classified here (cls) and its mask stored
+    Classify cls = {};
     // The node's mask and cell, loaded by the scan before any output is written.
     struct Pre {
         u32 m;
         u64 c;
     };
-    __device__ Pre load(u32 i) const { return Pre{mask[i], coord[i]}; }
+    __device__ Pre load(u32 i) const { return Pre{mask_out ? cls(i) : mask[i], coord[i]}; }
     __device__ void operator()(u32 i, u32 first, u32 k, Pre pre) const {
         cstart[i] = first;
+        if (mask_out) mask_out[i] = pre.m;
         if (!k) return;
         const u32 m = pre.m;
         const u64 c = pre.c;
@@ -1289,12 +1308,23 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
                             nullptr, nullptr);
             break;
         }
-        CPVS_LAUNCH("classify", k_classify, grid1d(c), dim3(256), 0, s, lv[L].coord, c, p->lv[L], p->depth, N, L, T,
-                    lv[L].mask);
-        if (L > bottom) {
-            EmitChildrenOut emit{lv[L].mask, lv[L].coord, lv[L].cstart, lv[L - 1].coord, lv[L - 1].count, LeafColumns{}};
+        // A level is classified inside its emit scan when that scan loads its outputs' prefetch
+        // before the block scan.  Above 1 M nodes the output stage loads after the scan, where
+        // classifying (two dependent loads) costs more than the classify launch it saves.
+        if (L > bottom && scan_prefetches(c)) {
+            const Classify cls{lv[L].coord, p->lv[L], p->depth, N, L, T};
+            EmitChildrenOut emit{lv[L].mask, lv[L].coord, lv[L].cstart, lv[L - 1].coord, lv[L - 1].count, LeafColumns{},
+                                 lv[L].mask, cls};
             if (leaf && L == 3) emit.cols = lc;
-            device_scan(ChildCountIn{lv[L].mask}, emit, c, OpSum{}, meta + kMetaC + L, scan_scratch, s);
+            device_scan(ClassifyCountIn{cls}, emit, c, OpSum{}, meta + kMetaC + L, scan_scratch, s);
+        } else {
+            CPVS_LAUNCH("classify", k_classify, grid1d(c), dim3(256), 0, s, lv[L].coord, c, p->lv[L], p->depth, N, L, T,
+                        lv[L].mask);
+            if (L > bottom) {
+                EmitChildrenOut emit{lv[L].mask, lv[L].coord, lv[L].cstart, lv[L - 1].coord, lv[L - 1].count, LeafColumns{}};
+                if (leaf && L == 3) emit.cols = lc;
+                device_scan(ChildCountIn{lv[L].mask}, emit, c, OpSum{}, meta + kMetaC + L, scan_scratch, s);
+            }
         }
     }
     CPVS_HIP_CHECK(hipGetLastError());

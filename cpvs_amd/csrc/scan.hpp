@@ -335,6 +335,10 @@ inline void scan_two_launches(In in, Out out, u32 n, Op op, typename Op::T* d_to
                 d_total, op);
 }
 
+// Whether device_scan over n elements loads every input and output prefetch of a tile before its
+// block scan (one and two-launch scans up to 4 * kScanSmallTileMax elements).
+inline bool scan_prefetches(u32 n) { return n <= 4 * kScanSmallTileMax; }
+
 // out(i, exclusive_prefix_i, value_i) for all i < n; *d_total (device, may be null) gets the
 // reduction of all values.  scratch: scan_scratch_words(n) u32 of device memory (8-byte aligned).
 // One launch for n <= 2048, two up to 2M elements, three beyond.
