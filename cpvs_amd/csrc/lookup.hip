@@ -19,6 +19,15 @@ struct GridDag {
     u32 words;  // DAG word count: every read is bounds-checked against it
 };
 
+// Workgroups are dealt to the XCDs in turn (block b to XCD b mod kXcds), each XCD with its own L2.
+// XCD x takes the contiguous workgroups [x*chunk, (x+1)*chunk): neighbouring queries (an image's
+// neighbouring pixels) then walk the same DAG nodes through one L2, and each L2 holds the part of
+// the DAG under its share of the queries rather than all of it.
+constexpr u32 kXcds = 8;
+__device__ inline u64 xcd_query(u32 chunk) {
+    return (u64)((blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds) * blockDim.x + threadIdx.x;
+}
+
 // traverse.cs:43-48: path = int(((ndc + 1) * 0.5) * float(R - 1)), clamped to [0, R-1]
 // (out-of-range positions index outside the grid in the reference).
 __device__ inline u32 quantise(float v, u32 R) {
@@ -95,8 +104,8 @@ __device__ inline u32 lookup_point(const GridDag& g, const Mat4& M, float x, flo
 // (px + dx, py + dy, pz), dx and dy in [-(f-1)/2, f/2] (integer halves), each clamped to the grid,
 // as float(lit) / float(f*f).  f = 1 is the plain lookup.
 __global__ __launch_bounds__(256) void k_evaluate_pcf(GridDag g, Mat4 M, const float* __restrict__ pos, u32 stride,
-                                                      u64 count, u32 f, float* __restrict__ vis) {
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+                                                      u64 count, u32 f, float* __restrict__ vis, u32 chunk) {
+    const u64 i = xcd_query(chunk);
     if (i >= count) return;
     const float* q = pos + i * stride;
     u32 px, py, pz;
@@ -119,37 +128,42 @@ int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int 
     if (count == 0) return CPVS_OK;
     GridDag g{dag, grid, dl, gl, words};
     const u64 blocks = (count + 255) / 256;
-    if (blocks > 0x7FFFFFFFull) {
+    if (blocks > 0x7FFFFFF0ull) {
         set_error("evaluate_pcf: %llu points exceed one launch", (unsigned long long)count);
         return CPVS_EINVAL;
     }
-    CPVS_LAUNCH("evaluate_pcf", k_evaluate_pcf, dim3((u32)blocks), dim3(256), 0, s, g, m, pos, stride, count, f, vis);
+    const u32 chunk = (u32)((blocks + kXcds - 1) / kXcds);
+    CPVS_LAUNCH("evaluate_pcf", k_evaluate_pcf, dim3(chunk * kXcds), dim3(256), 0, s, g, m, pos, stride, count, f, vis, chunk);
     CPVS_HIP_CHECK(hipGetLastError());
     return CPVS_OK;
 }
 
 __global__ __launch_bounds__(256) void k_evaluate4(GridDag g, Mat4 M, const float4* __restrict__ pos, u64 count,
-                                                   uint8_t* __restrict__ vis) {
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+                                                   uint8_t* __restrict__ vis, u32 chunk) {
+    const u64 i = xcd_query(chunk);
     if (i >= count) return;
     const float4 q = pos[i];
     vis[i] = (uint8_t)lookup_point(g, M, q.x, q.y, q.z);
 }
 
 __global__ __launch_bounds__(256) void k_evaluate3(GridDag g, Mat4 M, const float* __restrict__ pos, u64 count,
-                                                   uint8_t* __restrict__ vis) {
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+                                                   uint8_t* __restrict__ vis, u32 chunk) {
+    const u64 i = xcd_query(chunk);
     if (i >= count) return;
     const float* q = pos + 3 * i;
     vis[i] = (uint8_t)lookup_point(g, M, q[0], q[1], q[2]);
 }
 
-// Image form: a 256-thread block covers a 32x8 pixel strip as four 8x8 tiles, one per wave.
+// Image form: a 256-thread block covers a 32x8 pixel strip as four 8x8 tiles, one per wave.  The
+// blocks (gx per 8 rows, nblocks in all) are dealt to the XCDs in contiguous runs of chunk.
 __global__ __launch_bounds__(256) void k_evaluate_image(GridDag g, Mat4 M, const float4* __restrict__ pos, u32 width,
-                                                        u32 height, uint8_t* __restrict__ vis) {
+                                                        u32 height, uint8_t* __restrict__ vis, u32 gx, u32 nblocks,
+                                                        u32 chunk) {
+    const u32 b = (blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
+    if (b >= nblocks) return;
     const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const u32 x = blockIdx.x * 32 + wave * 8 + (lane & 7);
-    const u32 y = blockIdx.y * 8 + (lane >> 3);
+    const u32 x = (b % gx) * 32 + wave * 8 + (lane & 7);
+    const u32 y = (b / gx) * 8 + (lane >> 3);
     if (x >= width || y >= height) return;
     const size_t i = (size_t)y * width + x;
     const float4 q = pos[i];
@@ -161,15 +175,16 @@ int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, 
     if (count == 0) return CPVS_OK;
     GridDag g{dag, grid, dl, gl, words};
     const u64 blocks = (count + 255) / 256;
-    if (blocks > 0x7FFFFFFFull) {
+    if (blocks > 0x7FFFFFF0ull) {
         set_error("evaluate: %llu points exceed one launch", (unsigned long long)count);
         return CPVS_EINVAL;
     }
+    const u32 chunk = (u32)((blocks + kXcds - 1) / kXcds);
     if (stride == 4)
-        CPVS_LAUNCH("evaluate4", k_evaluate4, dim3((u32)blocks), dim3(256), 0, s, g, m, reinterpret_cast<const float4*>(pos),
-                           count, vis);
+        CPVS_LAUNCH("evaluate4", k_evaluate4, dim3(chunk * kXcds), dim3(256), 0, s, g, m, reinterpret_cast<const float4*>(pos),
+                           count, vis, chunk);
     else
-        CPVS_LAUNCH("evaluate3", k_evaluate3, dim3((u32)blocks), dim3(256), 0, s, g, m, pos, count, vis);
+        CPVS_LAUNCH("evaluate3", k_evaluate3, dim3(chunk * kXcds), dim3(256), 0, s, g, m, pos, count, vis, chunk);
     CPVS_HIP_CHECK(hipGetLastError());
     return CPVS_OK;
 }
@@ -178,9 +193,14 @@ int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, in
                           u32 height, const Mat4& m, uint8_t* vis, hipStream_t s) {
     if (width == 0 || height == 0) return CPVS_OK;
     GridDag g{dag, grid, dl, gl, words};
-    const dim3 gridDim(ceil_div<u32>(width, 32), ceil_div<u32>(height, 8));
-    CPVS_LAUNCH("evaluate_image", k_evaluate_image, gridDim, dim3(256), 0, s, g, m, reinterpret_cast<const float4*>(pos), width,
-                       height, vis);
+    const u64 gx = ceil_div<u32>(width, 32), nblocks = gx * ceil_div<u32>(height, 8);
+    if (nblocks > 0x7FFFFFF0ull) {
+        set_error("evaluate_image: %ux%u pixels exceed one launch", width, height);
+        return CPVS_EINVAL;
+    }
+    const u32 chunk = (u32)((nblocks + kXcds - 1) / kXcds);
+    CPVS_LAUNCH("evaluate_image", k_evaluate_image, dim3(chunk * kXcds), dim3(256), 0, s, g, m, reinterpret_cast<const float4*>(pos),
+                width, height, vis, (u32)gx, (u32)nblocks, chunk);
     CPVS_HIP_CHECK(hipGetLastError());
     return CPVS_OK;
 }

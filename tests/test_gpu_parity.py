@@ -396,15 +396,17 @@ def test_evaluate_image_matches_flat(ctx):
     N = 512
     D = O.gen_depth(N, SEED0 + 2, 16, True)
     cs = cp.CompressedShadow.create(cp.MinMaxHierarchy(D))
+    words, n = cs.getDAG(), cs.getNumLevels()
     c = cp.CompressedShadowContainer(cs)
     c.moveToGPU()
-    W, H = 333, 77
+    W, H = 333, 77  # ragged: neither a multiple of the 32-pixel blocks nor of the 8 rows
     img = O.gen_queries_uniform(3, W * H, 4).reshape(H, W, 4)
     d_img = cp.DeviceArray.from_numpy(img)
     a = c.evaluate_image(d_img).numpy().reshape(-1)
     flat = cp.DeviceArray.from_numpy(img.reshape(-1, 4))
     b = c.evaluate(flat).numpy()
     assert np.array_equal(a, b)
+    assert np.array_equal(b, O.evaluate(words, O.top_level_grid([words]), n, 0, img.reshape(-1, 4), nthreads=8))
 
 
 # --- generator ------------------------------------------------------------------------------------
