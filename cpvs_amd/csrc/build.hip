@@ -256,13 +256,14 @@ struct LeafHash {
 // * Measured on MI355X: 8 columns per wave 0.31 ms, 4 columns 0.32 ms, 16 columns 0.42 ms; one
 //   column per one-wave workgroup (loads issued per column, run heads found by a ballot per node)
 //   0.55 ms; one lane per node over LDS-staged columns 0.51 ms (64 texels of work per node).
+//   Workgroups of 1, 2, 4 and 8 waves: 0.308, 0.315, 0.320 and 0.335 ms.
 // Workgroups are dealt to the XCDs in turn (block b to XCD b mod kXcds), each XCD with its own L2.
 // XCD x takes the contiguous units [x*unit_chunk, (x+1)*unit_chunk): neighbouring columns, whose
 // 8-texel depth rows share 128-byte lines, are then read through one L2.
 // ---------------------------------------------------------------------------------------
 constexpr u32 kXcds = 8;
 constexpr u32 kLeafCols = 8;
-constexpr u32 kLeafWaves = 4;
+constexpr u32 kLeafWaves = 1;
 
 // kList (several z-tiles): the wave's columns are col_list[c0 .. c0+7], the z-tile's columns with
 // leaf nodes in column order, *n_list of them; ncol (at most the leaf node count) bounds the grid.
