@@ -28,6 +28,10 @@ __device__ inline u64 xcd_query(u32 chunk) {
     return (u64)((blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds) * blockDim.x + threadIdx.x;
 }
 
+// Threads per workgroup of the flat and PCF lookups (measured on 16.6 M G-buffer queries: 64, 128,
+// 256, 512 and 1024 threads took 0.301, 0.297, 0.289, 0.277 and 0.276 ms).
+constexpr u32 kLookupThreads = 512;
+
 // traverse.cs:43-48: path = int(((ndc + 1) * 0.5) * float(R - 1)), clamped to [0, R-1]
 // (out-of-range positions index outside the grid in the reference).
 __device__ inline u32 quantise(float v, u32 R) {
@@ -103,7 +107,7 @@ __device__ inline u32 lookup_point(const GridDag& g, const Mat4& M, float x, flo
 // unused: shader/traverse.cs:16-17): the fraction of lit leaf cells among the f x f paths
 // (px + dx, py + dy, pz), dx and dy in [-(f-1)/2, f/2] (integer halves), each clamped to the grid,
 // as float(lit) / float(f*f).  f = 1 is the plain lookup.
-__global__ __launch_bounds__(256) void k_evaluate_pcf(GridDag g, Mat4 M, const float* __restrict__ pos, u32 stride,
+__global__ __launch_bounds__(kLookupThreads) void k_evaluate_pcf(GridDag g, Mat4 M, const float* __restrict__ pos, u32 stride,
                                                       u64 count, u32 f, float* __restrict__ vis, u32 chunk) {
     const u64 i = xcd_query(chunk);
     if (i >= count) return;
@@ -127,18 +131,18 @@ int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int 
                         const Mat4& m, u32 f, float* vis, hipStream_t s) {
     if (count == 0) return CPVS_OK;
     GridDag g{dag, grid, dl, gl, words};
-    const u64 blocks = (count + 255) / 256;
+    const u64 blocks = (count + kLookupThreads - 1) / kLookupThreads;
     if (blocks > 0x7FFFFFF0ull) {
         set_error("evaluate_pcf: %llu points exceed one launch", (unsigned long long)count);
         return CPVS_EINVAL;
     }
     const u32 chunk = (u32)((blocks + kXcds - 1) / kXcds);
-    CPVS_LAUNCH("evaluate_pcf", k_evaluate_pcf, dim3(chunk * kXcds), dim3(256), 0, s, g, m, pos, stride, count, f, vis, chunk);
+    CPVS_LAUNCH("evaluate_pcf", k_evaluate_pcf, dim3(chunk * kXcds), dim3(kLookupThreads), 0, s, g, m, pos, stride, count, f, vis, chunk);
     CPVS_HIP_CHECK(hipGetLastError());
     return CPVS_OK;
 }
 
-__global__ __launch_bounds__(256) void k_evaluate4(GridDag g, Mat4 M, const float4* __restrict__ pos, u64 count,
+__global__ __launch_bounds__(kLookupThreads) void k_evaluate4(GridDag g, Mat4 M, const float4* __restrict__ pos, u64 count,
                                                    uint8_t* __restrict__ vis, u32 chunk) {
     const u64 i = xcd_query(chunk);
     if (i >= count) return;
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(256) void k_evaluate4(GridDag g, Mat4 M, const floa
     vis[i] = (uint8_t)lookup_point(g, M, q.x, q.y, q.z);
 }
 
-__global__ __launch_bounds__(256) void k_evaluate3(GridDag g, Mat4 M, const float* __restrict__ pos, u64 count,
+__global__ __launch_bounds__(kLookupThreads) void k_evaluate3(GridDag g, Mat4 M, const float* __restrict__ pos, u64 count,
                                                    uint8_t* __restrict__ vis, u32 chunk) {
     const u64 i = xcd_query(chunk);
     if (i >= count) return;
@@ -174,17 +178,17 @@ int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, 
                     const Mat4& m, uint8_t* vis, hipStream_t s) {
     if (count == 0) return CPVS_OK;
     GridDag g{dag, grid, dl, gl, words};
-    const u64 blocks = (count + 255) / 256;
+    const u64 blocks = (count + kLookupThreads - 1) / kLookupThreads;
     if (blocks > 0x7FFFFFF0ull) {
         set_error("evaluate: %llu points exceed one launch", (unsigned long long)count);
         return CPVS_EINVAL;
     }
     const u32 chunk = (u32)((blocks + kXcds - 1) / kXcds);
     if (stride == 4)
-        CPVS_LAUNCH("evaluate4", k_evaluate4, dim3(chunk * kXcds), dim3(256), 0, s, g, m, reinterpret_cast<const float4*>(pos),
+        CPVS_LAUNCH("evaluate4", k_evaluate4, dim3(chunk * kXcds), dim3(kLookupThreads), 0, s, g, m, reinterpret_cast<const float4*>(pos),
                            count, vis, chunk);
     else
-        CPVS_LAUNCH("evaluate3", k_evaluate3, dim3(chunk * kXcds), dim3(256), 0, s, g, m, pos, count, vis, chunk);
+        CPVS_LAUNCH("evaluate3", k_evaluate3, dim3(chunk * kXcds), dim3(kLookupThreads), 0, s, g, m, pos, count, vis, chunk);
     CPVS_HIP_CHECK(hipGetLastError());
     return CPVS_OK;
 }
