@@ -564,8 +564,12 @@ __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __rest
     if (act) tslot[i] = slot;
 }
 
+// Threads per workgroup of the one-thread-per-node merge and write kernels (at 8192^2, 128 and 256
+// tie; 512 and 1024 make resolve and the writes slower by 5-10 µs).
+constexpr u32 kNodeThreads = 256;
+
 template <class Eq>
-__global__ __launch_bounds__(256) void k_resolve(const u32* __restrict__ tslot, const u64* __restrict__ tab,
+__global__ __launch_bounds__(kNodeThreads) void k_resolve(const u32* __restrict__ tslot, const u64* __restrict__ tab,
                                                  const u32* __restrict__ runhead, u32 n, Eq eq, u32* __restrict__ rep,
                                                  u32* __restrict__ collisions) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -692,7 +696,7 @@ struct FirstSizeOut {
 
 // Copies take their first occurrence's rank.  The merge table of the next level (tab_slots > 0)
 // is cleared here too: this level's last table read (k_resolve) ran before.
-__global__ __launch_bounds__(256) void k_rank_copies(const u32* __restrict__ rep, const u32* __restrict__ runhead, u32 count,
+__global__ __launch_bounds__(kNodeThreads) void k_rank_copies(const u32* __restrict__ rep, const u32* __restrict__ runhead, u32 count,
                                                      u32* __restrict__ rank, u64* __restrict__ tab, u32 tab_slots) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
@@ -832,7 +836,7 @@ struct WriteArgs {
     const u32* leafslot;
 };
 
-__global__ __launch_bounds__(256) void k_write_levels(WriteArgs a, u32* __restrict__ out) {
+__global__ __launch_bounds__(kNodeThreads) void k_write_levels(WriteArgs a, u32* __restrict__ out) {
     int k = 0;
     while (k + 1 < a.nlev && blockIdx.x >= a.first[k + 1]) ++k;
     const WriteLevel& l = a.lv[k];
@@ -1355,13 +1359,13 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             if (is_leaf) {
                 CPVS_LAUNCH("insert_leaf", k_insert_leaf, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, hkey, runhead, c, tab,
                             Sl, tslot);
-                CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c), dim3(256), 0, s, tslot, tab, runhead, c,
+                CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, runhead, c,
                             LeafEq{lv[2].mask, leafm, leafslot, nullptr}, rep, meta + kMetaColl);
             } else {
                 CPVS_LAUNCH("insert_inner", k_insert_inner, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, lv[L].mask,
                             lv[L].cstart, child_rank, c,
                             tab, Sl, tslot);
-                CPVS_LAUNCH("resolve", k_resolve<InnerEq>, grid1d(c), dim3(256), 0, s, tslot, tab, nullptr, c,
+                CPVS_LAUNCH("resolve", k_resolve<InnerEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, nullptr, c,
                             InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaColl);
             }
         } else {
@@ -1387,7 +1391,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         else
             device_scan(FirstFlagIn{rep}, FirstRankOut{lv[L].rank, lv[L].uniq}, c, OpSum{}, meta + kMetaU + L, scan_scratch, s);
         const u32 S_next = (use_table && L + 1 < std::min(R, Ls)) ? table_slots(lv[L + 1].count) : 0u;
-        CPVS_LAUNCH("rank_copies", k_rank_copies, grid1d(c), dim3(256), 0, s, rep, rh, c, lv[L].rank, tab, S_next);
+        CPVS_LAUNCH("rank_copies", k_rank_copies, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, rep, rh, c, lv[L].rank, tab, S_next);
     }
     if (Ls <= R)
         CPVS_LAUNCH("small_merge", k_small_merge, dim3(1), dim3(kSmallThreads), 0, s, sa);
@@ -1453,12 +1457,12 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         w.child_base = kids ? hm[kMetaB + L - 1] : 0u;
         w.leaf = (leaf && L == 2) ? 1u : 0u;
         wa.first[wa.nlev++] = wblocks;
-        wblocks += ceil_div<u32>(w.U, 256);
+        wblocks += ceil_div<u32>(w.U, kNodeThreads);
     }
     wa.first[wa.nlev] = wblocks;
     wa.leafm = leafm;
     wa.leafslot = leafslot;
-    if (wblocks) CPVS_LAUNCH("write_levels", k_write_levels, dim3(wblocks), dim3(256), 0, s, wa, d_words);
+    if (wblocks) CPVS_LAUNCH("write_levels", k_write_levels, dim3(wblocks), dim3(kNodeThreads), 0, s, wa, d_words);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
