@@ -549,7 +549,8 @@ def main():
     roofline = roof(rname)
     pyramid_roofline = roof("pyr_tile_root")
     lookup_roofline = roof("evaluate4")
-    lookup_roofline["bound"] = "latency (dependent loads; DAG cache-resident)"
+    # Dependent gathers from a cache-resident DAG: set by load throughput and L2 locality, not HBM.
+    lookup_roofline["bound"] = "L2 (dependent gathers; DAG cache-resident)"
 
     tiled = None
     if not args.no_tiled:
