@@ -568,23 +568,36 @@ __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __rest
 // tie; 512 and 1024 make resolve and the writes slower by 5-10 µs).
 constexpr u32 kNodeThreads = 256;
 
+// With partials set, each workgroup also reduces the ranks scan's input over its nodes (FirstSizeIn:
+// 1 << 32 | the node's words, for a first occurrence) into partials[block]: the scan that follows
+// needs no reduce pass.  The workgroups are the scan's one-round tiles.
+static_assert(kNodeThreads == kScanThreads, "resolve workgroups are the ranks scan's tiles");
 template <class Eq>
 __global__ __launch_bounds__(kNodeThreads) void k_resolve(const u32* __restrict__ tslot, const u64* __restrict__ tab,
                                                  const u32* __restrict__ runhead, u32 n, Eq eq, u32* __restrict__ rep,
-                                                 u32* __restrict__ collisions) {
+                                                 u32* __restrict__ collisions, const u32* __restrict__ mask, u32 leaf,
+                                                 u64* __restrict__ partials) {
+    __shared__ u64 s_wave[kNodeThreads / kWave];
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (runhead && runhead[i] != i) {
-        rep[i] = kNoRep;  // resolved through its run's first node
-        return;
+    u64 v = 0;
+    if (i < n) {
+        if (runhead && runhead[i] != i) {
+            rep[i] = kNoRep;  // resolved through its run's first node
+        } else {
+            u32 r = (u32)tab[tslot[i]];  // the smallest index with this node's tag
+            if (r >= n) r = i;
+            if (r != i && !eq(i, r)) {
+                atomicAdd(collisions, 1u);  // two keys share a 64-bit hash: the build is redone by sorting
+                r = i;
+            }
+            rep[i] = r;
+            if (partials && r == i) v = (1ull << 32) | (u64)(1u + (num_children(mask[i]) << leaf));
+        }
     }
-    u32 r = (u32)tab[tslot[i]];  // the smallest index with this node's tag
-    if (r >= n) r = i;
-    if (r != i && !eq(i, r)) {
-        atomicAdd(collisions, 1u);  // two keys share a 64-bit hash: the build is redone by sorting
-        r = i;
+    if (partials) {  // uniform across the launch
+        v = block_reduce(v, s_wave, OpSum64{});
+        if (threadIdx.x == 0) partials[blockIdx.x] = v;
     }
-    rep[i] = r;
 }
 
 // --- (b) radix sort ----------------------------------------------------------------------------
@@ -1352,6 +1365,12 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         const bool is_leaf = leaf && L == 2;
         const u32* child_rank = (L > deepest) ? lv[L - 1].rank : nullptr;
         const u32* rh = is_leaf ? runhead : nullptr;
+        // Levels ranked by the 64-bit scan in one launch (at most kScanPartialsInline one-round
+        // tiles) take its tile partials from k_resolve (table merge): no reduce pass.  Larger levels
+        // keep the 8-round tiles, whose partials scan and down pass cost less than one-round ones.
+        u64* rpart = (use_table && fused_sizes(L) && ceil_div<u32>(c, kScanThreads) <= kScanPartialsInline)
+                         ? reinterpret_cast<u64*>(scan_scratch)
+                         : nullptr;
         if (use_table) {
             const u32 Sl = table_slots(c);
             // Cleared by the leaf kernel (leaf level) or the previous level's k_rank_copies.
@@ -1360,13 +1379,13 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
                 CPVS_LAUNCH("insert_leaf", k_insert_leaf, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, hkey, runhead, c, tab,
                             Sl, tslot);
                 CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, runhead, c,
-                            LeafEq{lv[2].mask, leafm, leafslot, nullptr}, rep, meta + kMetaColl);
+                            LeafEq{lv[2].mask, leafm, leafslot, nullptr}, rep, meta + kMetaColl, lv[2].mask, 1u, rpart);
             } else {
                 CPVS_LAUNCH("insert_inner", k_insert_inner, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, lv[L].mask,
                             lv[L].cstart, child_rank, c,
                             tab, Sl, tslot);
                 CPVS_LAUNCH("resolve", k_resolve<InnerEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, nullptr, c,
-                            InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaColl);
+                            InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaColl, lv[L].mask, 0u, rpart);
             }
         } else {
             if (is_leaf)
@@ -1384,7 +1403,11 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
                             InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaSortColl);
             rh = nullptr;  // every node got its representative directly
         }
-        if (fused_sizes(L))
+        if (fused_sizes(L) && rpart)  // tile partials came from k_resolve
+            device_scan_from_partials(FirstSizeIn{rep, lv[L].mask, is_leaf ? 1u : 0u},
+                                      FirstSizeOut{lv[L].rank, lv[L].uniq, lv[L].off, meta + kMetaU + L, meta + kMetaS + L, c},
+                                      c, OpSum64{}, nullptr, rpart, s);
+        else if (fused_sizes(L))
             device_scan(FirstSizeIn{rep, lv[L].mask, is_leaf ? 1u : 0u},
                         FirstSizeOut{lv[L].rank, lv[L].uniq, lv[L].off, meta + kMetaU + L, meta + kMetaS + L, c}, c, OpSum64{},
                         nullptr, scan_scratch, s);

@@ -316,15 +316,10 @@ struct ArrOutExclusive {
     __device__ void operator()(u32 i, u32 ex, u32) const { a[i] = ex; }
 };
 
-// Scratch words (u32) needed by device_scan for n elements (room for 64-bit partials).
-// Covers every scan of at most n elements (callers size scratch for their largest scan): one-round
-// tiles make a scan of 262,144 elements use 1024 partials, more than one of 300,000.
-inline size_t scan_scratch_words(u32 n) {
-    size_t nb = ceil_div<u32>(n ? n : 1, kScanTile);
-    const u32 small = n < kScanSmallTileMax ? n : kScanSmallTileMax;
-    if (small > kScanTile) nb = std::max(nb, (size_t)ceil_div<u32>(small, kScanThreads));
-    return 2 * (nb + 1);
-}
+// Scratch words (u32) needed by device_scan and device_scan_from_partials for n elements: one
+// 64-bit partial per one-round tile (the finest tiling any path uses) and one more.  Monotone in
+// n, so it covers every scan of at most n elements (callers size scratch for their largest scan).
+inline size_t scan_scratch_words(u32 n) { return 2 * ((size_t)ceil_div<u32>(n ? n : 1, kScanThreads) + 1); }
 
 // Reduce and down-sweep with R-round tiles (at most kScanPartialsInline of them).
 template <class In, class Out, class Op, int R>
@@ -371,6 +366,30 @@ inline void device_scan(In in, Out out, u32 n, Op op, typename Op::T* d_total, u
     }
     CPVS_LAUNCH("scan_partials", k_scan_partials<Op>, dim3(1), dim3(kPartialsThreads), 0, s, part, nb, d_total, op);
     CPVS_LAUNCH("scan_down", (k_scan_down<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, part, op);
+}
+
+// Down-sweep over one-round tiles whose partials[t] already hold tile t's exclusive prefix.
+template <class In, class Out, class Op>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down_tiles(In in, Out out, u32 n, const typename Op::T* partials,
+                                                                  Op op) {
+    scan_tile_latency<1>(in, out, n, blockIdx.x * kScanThreads, partials[blockIdx.x], op);
+}
+
+// device_scan for an input whose producer already stored part[t] = the reduction of in(i) over the
+// one-round tile t (i / kScanThreads == t), t < ceil(n / kScanThreads): no reduce pass.  One launch
+// up to kScanPartialsInline tiles, two beyond.  part: scan_scratch_words(n) words.
+template <class In, class Out, class Op>
+inline void device_scan_from_partials(In in, Out out, u32 n, Op op, typename Op::T* d_total, typename Op::T* part,
+                                      hipStream_t s) {
+    if (!n) return;
+    const u32 nt = ceil_div<u32>(n, kScanThreads);
+    if (nt <= kScanPartialsInline) {
+        CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op, 1>), dim3(nt), dim3(kScanThreads), 0, s, in, out, n, part, nt,
+                    d_total, op);
+        return;
+    }
+    CPVS_LAUNCH("scan_partials", k_scan_partials<Op>, dim3(1), dim3(kPartialsThreads), 0, s, part, nt, d_total, op);
+    CPVS_LAUNCH("scan_down", (k_scan_down_tiles<In, Out, Op>), dim3(nt), dim3(kScanThreads), 0, s, in, out, n, part, op);
 }
 
 // ------------------------------------------------------------------------------------------
