@@ -547,8 +547,17 @@ __device__ inline u32 table_insert_group(u64* tab, u32 S, u64 h, u32 i, bool act
     return active ? gs[ls] : 0u;
 }
 
+// zero[0 .. nzero): the ranks scan's tile partials that k_resolve adds into next (cleared here, in a
+// launch that runs anyway).
+__device__ inline void clear_partials(u64* zero, u32 nzero) {
+    const u32 stride = gridDim.x * blockDim.x;
+    for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < nzero; j += stride) zero[j] = 0ull;
+}
+
 __global__ __launch_bounds__(kInsertBlock) void k_insert_leaf(const u64* __restrict__ hkey, const u32* __restrict__ runhead,
-                                                              u32 n, u64* __restrict__ tab, u32 S, u32* __restrict__ tslot) {
+                                                              u32 n, u64* __restrict__ tab, u32 S, u32* __restrict__ tslot,
+                                                              u64* __restrict__ zero, u32 nzero) {
+    clear_partials(zero, nzero);
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < n && runhead[i] == i;  // a run's first node has the run's smallest index
     const u32 slot = table_insert_group(tab, S, act ? hkey[i] : 0ull, i, act);
@@ -557,7 +566,9 @@ __global__ __launch_bounds__(kInsertBlock) void k_insert_leaf(const u64* __restr
 
 __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
                                                                const u32* __restrict__ child_rank, u32 n, u64* __restrict__ tab,
-                                                               u32 S, u32* __restrict__ tslot) {
+                                                               u32 S, u32* __restrict__ tslot, u64* __restrict__ zero,
+                                                               u32 nzero) {
+    clear_partials(zero, nzero);
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < n;
     const u32 slot = table_insert_group(tab, S, act ? inner_key_hash(mask, cstart, child_rank, i) : 0ull, i, act);
@@ -569,14 +580,15 @@ __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __rest
 constexpr u32 kNodeThreads = 256;
 
 // With partials set, each workgroup also reduces the ranks scan's input over its nodes (FirstSizeIn:
-// 1 << 32 | the node's words, for a first occurrence) into partials[block]: the scan that follows
-// needs no reduce pass.  The workgroups are the scan's one-round tiles.
-static_assert(kNodeThreads == kScanThreads, "resolve workgroups are the ranks scan's tiles");
+// 1 << 32 | the node's words, for a first occurrence) into the partial of the scan tile it lies in
+// (tiles of `rounds` workgroups: a store for one, an atomic add into a partial the insert kernel
+// cleared otherwise), so the scan that follows needs no reduce pass.
+static_assert(kNodeThreads == kScanThreads, "resolve workgroups are the ranks scan's rounds");
 template <class Eq>
 __global__ __launch_bounds__(kNodeThreads) void k_resolve(const u32* __restrict__ tslot, const u64* __restrict__ tab,
                                                  const u32* __restrict__ runhead, u32 n, Eq eq, u32* __restrict__ rep,
                                                  u32* __restrict__ collisions, const u32* __restrict__ mask, u32 leaf,
-                                                 u64* __restrict__ partials) {
+                                                 u64* __restrict__ partials, u32 rounds) {
     __shared__ u64 s_wave[kNodeThreads / kWave];
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     u64 v = 0;
@@ -596,7 +608,12 @@ __global__ __launch_bounds__(kNodeThreads) void k_resolve(const u32* __restrict_
     }
     if (partials) {  // uniform across the launch
         v = block_reduce(v, s_wave, OpSum64{});
-        if (threadIdx.x == 0) partials[blockIdx.x] = v;
+        if (threadIdx.x == 0) {
+            if (rounds == 1)
+                partials[blockIdx.x] = v;
+            else if (v)
+                atomicAdd(reinterpret_cast<unsigned long long*>(partials + blockIdx.x / rounds), (unsigned long long)v);
+        }
     }
 }
 
@@ -1365,27 +1382,27 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         const bool is_leaf = leaf && L == 2;
         const u32* child_rank = (L > deepest) ? lv[L - 1].rank : nullptr;
         const u32* rh = is_leaf ? runhead : nullptr;
-        // Levels ranked by the 64-bit scan in one launch (at most kScanPartialsInline one-round
-        // tiles) take its tile partials from k_resolve (table merge): no reduce pass.  Larger levels
-        // keep the 8-round tiles, whose partials scan and down pass cost less than one-round ones.
-        u64* rpart = (use_table && fused_sizes(L) && ceil_div<u32>(c, kScanThreads) <= kScanPartialsInline)
-                         ? reinterpret_cast<u64*>(scan_scratch)
-                         : nullptr;
+        // Levels ranked by the 64-bit scan take its tile partials from k_resolve (table merge): no
+        // reduce pass.  Tiles of more than one round are summed with atomics into partials that the
+        // insert kernel clears.
+        u64* rpart = (use_table && fused_sizes(L)) ? reinterpret_cast<u64*>(scan_scratch) : nullptr;
+        const u32 rrounds = scan_rounds(c);
+        const u32 nzero = (rpart && rrounds > 1) ? ceil_div<u32>(c, rrounds * kScanThreads) : 0u;
         if (use_table) {
             const u32 Sl = table_slots(c);
             // Cleared by the leaf kernel (leaf level) or the previous level's k_rank_copies.
             if (L == deepest && !is_leaf) CPVS_LAUNCH("table_clear", k_table_clear, grid1d(Sl), dim3(256), 0, s, tab, Sl);
             if (is_leaf) {
                 CPVS_LAUNCH("insert_leaf", k_insert_leaf, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, hkey, runhead, c, tab,
-                            Sl, tslot);
+                            Sl, tslot, rpart, nzero);
                 CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, runhead, c,
-                            LeafEq{lv[2].mask, leafm, leafslot, nullptr}, rep, meta + kMetaColl, lv[2].mask, 1u, rpart);
+                            LeafEq{lv[2].mask, leafm, leafslot, nullptr}, rep, meta + kMetaColl, lv[2].mask, 1u, rpart, rrounds);
             } else {
                 CPVS_LAUNCH("insert_inner", k_insert_inner, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, lv[L].mask,
                             lv[L].cstart, child_rank, c,
-                            tab, Sl, tslot);
+                            tab, Sl, tslot, rpart, nzero);
                 CPVS_LAUNCH("resolve", k_resolve<InnerEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, nullptr, c,
-                            InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaColl, lv[L].mask, 0u, rpart);
+                            InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaColl, lv[L].mask, 0u, rpart, rrounds);
             }
         } else {
             if (is_leaf)

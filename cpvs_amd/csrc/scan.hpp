@@ -368,28 +368,38 @@ inline void device_scan(In in, Out out, u32 n, Op op, typename Op::T* d_total, u
     CPVS_LAUNCH("scan_down", (k_scan_down<In, Out, Op>), dim3(nb), dim3(kScanThreads), 0, s, in, out, n, part, op);
 }
 
-// Down-sweep over one-round tiles whose partials[t] already hold tile t's exclusive prefix.
-template <class In, class Out, class Op>
-__global__ __launch_bounds__(kScanThreads) void k_scan_down_tiles(In in, Out out, u32 n, const typename Op::T* partials,
-                                                                  Op op) {
-    scan_tile_latency<1>(in, out, n, blockIdx.x * kScanThreads, partials[blockIdx.x], op);
+// Rounds per tile that device_scan uses for n > kScanTile elements (a single-block scan below).
+inline u32 scan_rounds(u32 n) {
+    return n <= kScanSmallTileMax ? 1u : n <= 2 * kScanSmallTileMax ? 2u : n <= 4 * kScanSmallTileMax ? 4u : (u32)kScanRounds;
 }
 
-// device_scan for an input whose producer already stored part[t] = the reduction of in(i) over the
-// one-round tile t (i / kScanThreads == t), t < ceil(n / kScanThreads): no reduce pass.  One launch
-// up to kScanPartialsInline tiles, two beyond.  part: scan_scratch_words(n) words.
+// device_scan for an input whose producer already stored part[t] = the reduction of in(i) over tile
+// t of scan_rounds(n) * kScanThreads elements, t < ceil(n / that): the same down pass as
+// device_scan without its reduce pass.  part: scan_scratch_words(n) words.
 template <class In, class Out, class Op>
 inline void device_scan_from_partials(In in, Out out, u32 n, Op op, typename Op::T* d_total, typename Op::T* part,
                                       hipStream_t s) {
     if (!n) return;
-    const u32 nt = ceil_div<u32>(n, kScanThreads);
+    const u32 R = scan_rounds(n);
+    const u32 nt = ceil_div<u32>(n, R * kScanThreads);
     if (nt <= kScanPartialsInline) {
-        CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op, 1>), dim3(nt), dim3(kScanThreads), 0, s, in, out, n, part, nt,
-                    d_total, op);
+        if (R == 1)
+            CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op, 1>), dim3(nt), dim3(kScanThreads), 0, s, in, out, n, part, nt,
+                        d_total, op);
+        else if (R == 2)
+            CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op, 2>), dim3(nt), dim3(kScanThreads), 0, s, in, out, n, part, nt,
+                        d_total, op);
+        else if (R == 4)
+            CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op, 4>), dim3(nt), dim3(kScanThreads), 0, s, in, out, n, part, nt,
+                        d_total, op);
+        else
+            CPVS_LAUNCH("scan_down", (k_scan_down_inline<In, Out, Op, kScanRounds>), dim3(nt), dim3(kScanThreads), 0, s, in, out, n,
+                        part, nt, d_total, op);
         return;
     }
+    // R == kScanRounds here: more than kScanPartialsInline tiles only above 4 * kScanSmallTileMax.
     CPVS_LAUNCH("scan_partials", k_scan_partials<Op>, dim3(1), dim3(kPartialsThreads), 0, s, part, nt, d_total, op);
-    CPVS_LAUNCH("scan_down", (k_scan_down_tiles<In, Out, Op>), dim3(nt), dim3(kScanThreads), 0, s, in, out, n, part, op);
+    CPVS_LAUNCH("scan_down", (k_scan_down<In, Out, Op>), dim3(nt), dim3(kScanThreads), 0, s, in, out, n, part, op);
 }
 
 // ------------------------------------------------------------------------------------------
