@@ -487,15 +487,20 @@ __global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ tab, u32 
     if (i < S) tab[i] = kEmptySlot;
 }
 
-// Open addressing, linear probing; the table has more slots than keys, so every probe ends.
-// Large groups of equal keys hit one slot, so the slot is read first with a plain load: a slot
-// only ever goes from empty to one tag, and its index only decreases, so a stale read merely
-// leads to an atomic that was not needed.
-__device__ inline u32 table_insert(u64* tab, u32 S, u64 h, u32 i) {
+// Open addressing, linear probing.  Large groups of equal keys hit one slot, so the slot is read
+// first with a plain load: a slot only ever goes from empty to one tag, and its index only
+// decreases, so a stale read merely leads to an atomic that was not needed.
+// A probe gives up after kMaxProbes occupied slots (a table too small for its keys: the leaf
+// level's table is sized for a quarter of its nodes), counts that in *full and returns a slot in
+// range; the host then builds again with a larger table.  Node i's representative is then still a
+// node r with rep[r] = r (k_resolve reads whatever the returned slot holds and checks the keys),
+// so every index the rest of the build derives stays in range.
+constexpr u32 kMaxProbes = 256;
+__device__ inline u32 table_insert(u64* tab, u32 S, u64 h, u32 i, u32* full) {
     const u32 tag = (u32)(h >> 32);
     const u64 mine = ((u64)tag << 32) | i;
     u32 slot = (u32)(h ^ (h >> 31)) & (S - 1);
-    for (;;) {
+    for (u32 probe = 0; probe < kMaxProbes; ++probe) {
         u64 cur = __builtin_nontemporal_load(tab + slot);
         if (cur == kEmptySlot) {
             cur = atomicCAS(reinterpret_cast<unsigned long long*>(tab + slot), (unsigned long long)kEmptySlot,
@@ -508,6 +513,8 @@ __device__ inline u32 table_insert(u64* tab, u32 S, u64 h, u32 i) {
         }
         slot = (slot + 1) & (S - 1);
     }
+    atomicAdd(full, 1u);
+    return slot;
 }
 
 // A table slot hit by many threads at once serialises their atomics, and equal keys are common
@@ -521,7 +528,7 @@ __device__ inline u32 table_insert(u64* tab, u32 S, u64 h, u32 i) {
 constexpr u32 kInsertBlock = 512;
 constexpr u32 kGroupSlots = 2 * kInsertBlock;
 
-__device__ inline u32 table_insert_group(u64* tab, u32 S, u64 h, u32 i, bool active) {
+__device__ inline u32 table_insert_group(u64* tab, u32 S, u64 h, u32 i, bool active, u32* full) {
     __shared__ u64 gh[kGroupSlots];
     __shared__ u32 gi[kGroupSlots];
     __shared__ u32 gs[kGroupSlots];
@@ -542,7 +549,7 @@ __device__ inline u32 table_insert_group(u64* tab, u32 S, u64 h, u32 i, bool act
         atomicMin(gi + ls, i);
     }
     __syncthreads();
-    if (active && gi[ls] == i) gs[ls] = table_insert(tab, S, h, i);
+    if (active && gi[ls] == i) gs[ls] = table_insert(tab, S, h, i, full);
     __syncthreads();
     return active ? gs[ls] : 0u;
 }
@@ -554,24 +561,87 @@ __device__ inline void clear_partials(u64* zero, u32 nzero) {
     for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < nzero; j += stride) zero[j] = 0ull;
 }
 
-__global__ __launch_bounds__(kInsertBlock) void k_insert_leaf(const u64* __restrict__ hkey, const u32* __restrict__ runhead,
-                                                              u32 n, u64* __restrict__ tab, u32 S, u32* __restrict__ tslot,
-                                                              u64* __restrict__ zero, u32 nzero) {
+// The leaf level: only a run's first node is inserted (1.3 M of 6.7 M nodes at 8192^2), so each
+// thread takes kLeafPer nodes and a workgroup merges a window of kLeafWindow nodes in LDS.  The
+// window's distinct hashes are listed as they are found, and the list is spread over the threads
+// for the global inserts: a workgroup waits for about one global insert per thread, not per node.
+constexpr u32 kLeafInsertThreads = 256;
+constexpr u32 kLeafPer = 4;
+constexpr u32 kLeafWindow = kLeafInsertThreads * kLeafPer;
+constexpr u32 kLeafSlots = 2 * kLeafWindow;  // LDS table at most half full, so every probe ends
+static_assert(kLeafSlots <= 65536, "LDS slot numbers are listed as 16-bit values");
+
+__global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u64* __restrict__ hkey,
+                                                                    const u32* __restrict__ runhead, u32 n,
+                                                                    u64* __restrict__ tab, u32 S, u32* __restrict__ tslot,
+                                                                    u32* __restrict__ full, u64* __restrict__ zero,
+                                                                    u32 nzero) {
+    __shared__ u64 gh[kLeafSlots];
+    __shared__ u32 gi[kLeafSlots];  // smallest node index of the slot's hash, then its global slot
+    __shared__ unsigned short glist[kLeafWindow];  // LDS slots in use
+    __shared__ u32 gcount;
     clear_partials(zero, nzero);
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = i < n && runhead[i] == i;  // a run's first node has the run's smallest index
-    const u32 slot = table_insert_group(tab, S, act ? hkey[i] : 0ull, i, act);
-    if (act) tslot[i] = slot;
+    for (u32 j = threadIdx.x; j < kLeafSlots; j += blockDim.x) {
+        gh[j] = 0ull;  // 0 marks an empty LDS slot
+        gi[j] = kNoRep;
+    }
+    if (threadIdx.x == 0) gcount = 0;
+    const u32 i0 = blockIdx.x * kLeafWindow + threadIdx.x;
+    u32 rh[kLeafPer];
+#pragma unroll
+    for (u32 k = 0; k < kLeafPer; ++k) {
+        const u32 i = i0 + k * kLeafInsertThreads;
+        rh[k] = (i < n) ? runhead[i] : kNoRep;
+    }
+    u64 h[kLeafPer];
+#pragma unroll
+    for (u32 k = 0; k < kLeafPer; ++k) {
+        const u32 i = i0 + k * kLeafInsertThreads;
+        h[k] = (rh[k] == i) ? hkey[i] : 0ull;  // a run's first node has the run's smallest index
+        if (h[k] == 0ull && rh[k] == i) h[k] = 1ull;
+    }
+    __syncthreads();
+    u32 ls[kLeafPer];
+#pragma unroll
+    for (u32 k = 0; k < kLeafPer; ++k) {
+        const u32 i = i0 + k * kLeafInsertThreads;
+        ls[k] = 0;
+        if (rh[k] != i) continue;
+        u32 sl = (u32)(h[k] ^ (h[k] >> 29)) & (kLeafSlots - 1);
+        for (;;) {
+            const u64 cur = atomicCAS(reinterpret_cast<unsigned long long*>(gh + sl), 0ull, (unsigned long long)h[k]);
+            if (cur == 0ull) {
+                glist[atomicAdd(&gcount, 1u)] = (unsigned short)sl;
+                break;
+            }
+            if (cur == h[k]) break;
+            sl = (sl + 1) & (kLeafSlots - 1);
+        }
+        atomicMin(gi + sl, i);
+        ls[k] = sl;
+    }
+    __syncthreads();
+    const u32 m = gcount;  // at most kLeafWindow
+    for (u32 j = threadIdx.x; j < m; j += blockDim.x) {
+        const u32 sl = glist[j];
+        gi[sl] = table_insert(tab, S, gh[sl], gi[sl], full);
+    }
+    __syncthreads();
+#pragma unroll
+    for (u32 k = 0; k < kLeafPer; ++k) {
+        const u32 i = i0 + k * kLeafInsertThreads;
+        if (rh[k] == i) tslot[i] = gi[ls[k]];
+    }
 }
 
 __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
                                                                const u32* __restrict__ child_rank, u32 n, u64* __restrict__ tab,
-                                                               u32 S, u32* __restrict__ tslot, u64* __restrict__ zero,
-                                                               u32 nzero) {
+                                                               u32 S, u32* __restrict__ tslot, u32* __restrict__ full,
+                                                               u64* __restrict__ zero, u32 nzero) {
     clear_partials(zero, nzero);
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < n;
-    const u32 slot = table_insert_group(tab, S, act ? inner_key_hash(mask, cstart, child_rank, i) : 0ull, i, act);
+    const u32 slot = table_insert_group(tab, S, act ? inner_key_hash(mask, cstart, child_rank, i) : 0ull, i, act, full);
     if (act) tslot[i] = slot;
 }
 
@@ -787,7 +857,8 @@ enum : int {
     kMetaColl = 4 * kMaxLevels + 1,      // key collisions in the hash table (table merge)
     kMetaRoot = 4 * kMaxLevels + 2,      // root mask
     kMetaSortColl = 4 * kMaxLevels + 3,  // 32-bit key collisions resolved by a scan (sort merge)
-    kMetaWords = 4 * kMaxLevels + 4,
+    kMetaFull = 4 * kMaxLevels + 4,      // table inserts that found no free slot (table merge)
+    kMetaWords = 4 * kMaxLevels + 5,
 };
 
 __global__ void k_level_bases(u32* __restrict__ meta, int R, int bottom, const u32* __restrict__ root_mask,
@@ -1137,15 +1208,21 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_compress(SmallArgs a) {
 inline dim3 grid1d(u32 n, u32 block = 256) { return dim3(ceil_div<u32>(n ? n : 1, block)); }
 
 constexpr int kRetryWithSort = -1000;
+constexpr int kRetryFullTable = -1001;
+static_assert(sizeof(u32) * kMetaWords <= sizeof(u64) * kPinnedWords, "meta readback fits the pinned staging");
 
 u32 table_slots(u32 n) {
     u32 S = 1024;
     while (S < n + n / 2 + 1) S <<= 1;
     return S;
 }
+u32 leaf_table_slots(u32 c, bool small) { return table_slots(small ? c / 4 : c); }
 
+// small_leaf: the leaf level's merge table is sized for a quarter of its nodes (only a run's first
+// node is inserted: a fifth of them at 8192^2), which keeps the table's clear and its random
+// accesses small; a table found too small is reported as kRetryFullTable.
 int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, int merge_mode,
-               cpvs_dag** out, const Admit* admit) {
+               cpvs_dag** out, const Admit* admit, bool small_leaf) {
     const int n = p->n;
     const int R = n - 2;
     const bool leaf = n >= 5;
@@ -1331,7 +1408,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             const u32 bound = col_list ? std::min(ncol, c) : ncol;
             const u32 units = ceil_div<u32>(bound, kLeafWaves * kLeafCols);
             const u32 unit_chunk = ceil_div<u32>(units, kXcds);
-            const u32 slots = use_table ? table_slots(c) : 0u;
+            const u32 slots = use_table ? leaf_table_slots(c, small_leaf) : 0u;
             // The leaf level is the first merged: its table is cleared by this kernel.
             if (col_list)
                 CPVS_LAUNCH("leaf_waves", k_leaf_waves<true>, dim3(unit_chunk * kXcds), dim3(64 * kLeafWaves), 0, s, lc,
@@ -1389,18 +1466,18 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         const u32 rrounds = scan_rounds(c);
         const u32 nzero = (rpart && rrounds > 1) ? ceil_div<u32>(c, rrounds * kScanThreads) : 0u;
         if (use_table) {
-            const u32 Sl = table_slots(c);
+            const u32 Sl = is_leaf ? leaf_table_slots(c, small_leaf) : table_slots(c);
             // Cleared by the leaf kernel (leaf level) or the previous level's k_rank_copies.
             if (L == deepest && !is_leaf) CPVS_LAUNCH("table_clear", k_table_clear, grid1d(Sl), dim3(256), 0, s, tab, Sl);
             if (is_leaf) {
-                CPVS_LAUNCH("insert_leaf", k_insert_leaf, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, hkey, runhead, c, tab,
-                            Sl, tslot, rpart, nzero);
+                CPVS_LAUNCH("insert_leaf", k_insert_leaf, dim3(ceil_div<u32>(c, kLeafWindow)), dim3(kLeafInsertThreads), 0, s, hkey,
+                            runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
                 CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, runhead, c,
                             LeafEq{lv[2].mask, leafm, leafslot, nullptr}, rep, meta + kMetaColl, lv[2].mask, 1u, rpart, rrounds);
             } else {
                 CPVS_LAUNCH("insert_inner", k_insert_inner, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, lv[L].mask,
                             lv[L].cstart, child_rank, c,
-                            tab, Sl, tslot, rpart, nzero);
+                            tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
                 CPVS_LAUNCH("resolve", k_resolve<InnerEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, nullptr, c,
                             InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaColl, lv[L].mask, 0u, rpart, rrounds);
             }
@@ -1458,6 +1535,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     u32 hm[kMetaWords];
     std::memcpy(hm, pinned, sizeof(hm));
 
+    if (use_table && hm[kMetaFull] != 0) return small_leaf ? kRetryFullTable : kRetryWithSort;
     if (use_table && hm[kMetaColl] != 0) return kRetryWithSort;
     for (int L = deepest + 1; L <= R; ++L) {
         if (hm[kMetaC + L] != lv[L - 1].count) {
@@ -1533,8 +1611,9 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
 
 int build_dag(cpvs_ctx* ctx, Arena& A, u64* pinned, int merge_mode, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
               cpvs_dag** out, const Admit* admit) {
-    int rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out, admit);
-    if (rc == kRetryWithSort) rc = build_once(ctx, A, pinned, p, t, T, s, CPVS_MERGE_SORT, out, admit);
+    int rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out, admit, true);
+    if (rc == kRetryFullTable) rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out, admit, false);
+    if (rc == kRetryWithSort) rc = build_once(ctx, A, pinned, p, t, T, s, CPVS_MERGE_SORT, out, admit, false);
     return rc;
 }
 

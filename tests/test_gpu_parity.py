@@ -300,6 +300,23 @@ def test_hash_collision_path_exercised_by_many_duplicates(ctx):
     assert nodes[2] > 100 * uniq[2]
 
 
+def test_leaf_table_overflow_builds_again(ctx):
+    # White noise: almost every leaf node is distinct, more keys than the leaf level's merge table
+    # (sized for a quarter of its nodes) has slots, so inserts give up and the build runs again with
+    # a table for every node.
+    N = 512
+    D = np.random.default_rng(7).random((N, N), dtype=np.float32)
+    _say("noise", N)
+    cs, w, n = _gpu_dag(D)
+    wr, _ = _oracle_dag(D)
+    assert np.array_equal(w, wr)
+    nodes, uniq = cs.level_stats()
+    slots = 1024
+    while slots < nodes[2] // 4 + nodes[2] // 8 + 1:
+        slots <<= 1
+    assert uniq[2] > slots
+
+
 def test_invalid_arguments_are_rejected(ctx):
     with pytest.raises(cp.CpvsError):
         cp.MinMaxHierarchy(np.zeros((48, 48), np.float32))  # not a power of two
