@@ -1018,36 +1018,46 @@ __device__ inline u32 small_scan(u32 v, u32* s_wave, u32* total) {
 // Thread t owns nodes [t*E, t*E + E) of a level of c nodes (contiguous, so scans keep order).
 __device__ inline u32 small_per_thread(u32 c) { return (c + kSmallThreads - 1) / kSmallThreads; }
 
+// The small levels' cells stay in LDS (two buffers: a level and its children); only the children
+// of the last small level, which the multi-workgroup path classifies, go to memory.  Each node's
+// mask and cell are kept in registers from its classification to its emission.
 __global__ __launch_bounds__(kSmallThreads) void k_small_topdown(SmallArgs a, const float* __restrict__ depth, u32 N, u32 T,
                                                                  u64 root, LeafColumns cols) {
     __shared__ u32 s_wave[kSmallThreads / kWave];
-    if (threadIdx.x == 0) a.coord[a.R][0] = root;  // create(): root offset (0, 0, 2*zTileIndex)
+    __shared__ u64 s_coord[2][kSmallMax];
+    if (threadIdx.x == 0) s_coord[a.R & 1][0] = root;  // create(): root offset (0, 0, 2*zTileIndex)
     for (u32 i = threadIdx.x; i < kMetaWords; i += kSmallThreads) a.meta[i] = 0u;  // the first launch to use them
     __syncthreads();
     for (int L = a.R; L >= a.Ls; --L) {
-        const u32 c = a.count[L];
+        const u32 c = a.count[L];  // at most kSmallMax
         const u32 E = small_per_thread(c), i0 = threadIdx.x * E;
-        u32 k[kSmallPer], sum = 0;
+        const u64* cell = s_coord[L & 1];
+        u32 k[kSmallPer], m[kSmallPer], sum = 0;
+        u64 cc[kSmallPer];
 #pragma unroll
         for (u32 e = 0; e < kSmallPer; ++e) {
             k[e] = 0;
+            m[e] = 0;
+            cc[e] = 0;
             const u32 i = i0 + e;
             if (e < E && i < c) {
-                const u32 m = classify_node(a.coord[L][i], a.lv[L], depth, N, L, T);
-                a.mask[L][i] = m;
-                k[e] = num_children(m);
+                cc[e] = cell[i];
+                m[e] = classify_node(cc[e], a.lv[L], depth, N, L, T);
+                a.mask[L][i] = m[e];
+                k[e] = num_children(m[e]);
                 sum += k[e];
             }
         }
         u32 total;
         u32 first = small_scan(sum, s_wave, &total);
-        EmitChildrenOut emit{a.mask[L], a.coord[L], a.cstart[L], a.coord[L - 1], a.count[L - 1],
-                             (L == 3) ? cols : LeafColumns{}};
+        // Children on a small level (count at most kSmallMax) go to the other LDS buffer.
+        u64* child = (L - 1 >= a.Ls) ? s_coord[(L - 1) & 1] : a.coord[L - 1];
+        EmitChildrenOut emit{a.mask[L], nullptr, a.cstart[L], child, a.count[L - 1], (L == 3) ? cols : LeafColumns{}};
 #pragma unroll
         for (u32 e = 0; e < kSmallPer; ++e) {
             const u32 i = i0 + e;
             if (e < E && i < c) {
-                emit(i, first, k[e], emit.load(i));
+                emit(i, first, k[e], EmitChildrenOut::Pre{m[e], cc[e]});
                 first += k[e];
             }
         }
@@ -1076,26 +1086,61 @@ __device__ inline u32 lds_insert(u64* tab, u32 S, u64 h, u32 i) {
 // Merge of the small levels Ls .. R-1 (the root is never merged), then the root's rank.  A node
 // takes the smallest index with its slot's tag; if that node's key differs (a collision of tags
 // or of 64-bit hashes), it looks for its first occurrence by comparing keys with every lower index.
+// The small levels' ranks are also kept in LDS (two buffers: a level and its children), where the
+// next level's keys, key compares and the rank copies read them.  A level's masks and child starts
+// are loaded while the level below is merged, and staged in LDS for the keys and compares.
 __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
     __shared__ u64 tab[kSmallSlots];
     __shared__ u32 s_wave[kSmallThreads / kWave];
+    __shared__ u32 s_rank[2][kSmallMax];
+    __shared__ u32 s_mask[kSmallMax];
+    __shared__ u32 s_cstart[kSmallMax];
+    u32 pm[kSmallPer], pc[kSmallPer];  // masks and child starts of the next level's nodes
+    auto prefetch = [&](int L) {
+        const u32 c = a.count[L], E = small_per_thread(c), i0 = threadIdx.x * E;
+#pragma unroll
+        for (u32 e = 0; e < kSmallPer; ++e) {
+            const u32 i = i0 + e;
+            pm[e] = 0;
+            pc[e] = 0;
+            if (e < E && i < c) {
+                pm[e] = a.mask[L][i];
+                pc[e] = a.cstart[L][i];
+            }
+        }
+    };
+    if (a.Ls < a.R) prefetch(a.Ls);
     for (int L = a.Ls; L < a.R; ++L) {
-        const u32 c = a.count[L];
+        const u32 c = a.count[L];  // at most kSmallMax
         u32 S = 64;
         while (S < 2 * c) S <<= 1;
         for (u32 j = threadIdx.x; j < S; j += kSmallThreads) tab[j] = kEmptySlot;
+        {
+            const u32 E = small_per_thread(c), i0 = threadIdx.x * E;
+#pragma unroll
+            for (u32 e = 0; e < kSmallPer; ++e) {
+                const u32 i = i0 + e;
+                if (e < E && i < c) {
+                    s_mask[i] = pm[e];
+                    s_cstart[i] = pc[e];
+                }
+            }
+        }
         __syncthreads();
-        const u32* child_rank = (L > a.deepest) ? a.rank[L - 1] : nullptr;
+        if (L + 1 < a.R) prefetch(L + 1);
+        // The first small level's children were ranked by the multi-workgroup path, in memory.
+        const u32* child_rank = (L > a.deepest) ? ((L > a.Ls) ? s_rank[(L - 1) & 1] : a.rank[L - 1]) : nullptr;
+        u32* rank_l = s_rank[L & 1];
         const u32 E = small_per_thread(c), i0 = threadIdx.x * E;
         u32 slot[kSmallPer];
 #pragma unroll
         for (u32 e = 0; e < kSmallPer; ++e) {
             const u32 i = i0 + e;
             slot[e] = 0;
-            if (e < E && i < c) slot[e] = lds_insert(tab, S, inner_key_hash(a.mask[L], a.cstart[L], child_rank, i), i);
+            if (e < E && i < c) slot[e] = lds_insert(tab, S, inner_key_hash(s_mask, s_cstart, child_rank, i), i);
         }
         __syncthreads();
-        const InnerEq eq{a.mask[L], a.cstart[L], child_rank};
+        const InnerEq eq{s_mask, s_cstart, child_rank};
         // One scan of (first occurrences << 16 | their words) gives ranks and word offsets (a level
         // of at most kSmallMax nodes of at most 9 words keeps the low half below 2^16).
         u32 rep[kSmallPer], sz[kSmallPer], packed = 0;
@@ -1116,7 +1161,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
                 }
                 rep[e] = r;
                 if (r == i) {
-                    sz[e] = 1u + num_children(a.mask[L][i]);
+                    sz[e] = 1u + num_children(s_mask[i]);
                     packed += (1u << 16) + sz[e];
                 }
             }
@@ -1129,6 +1174,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
             const u32 i = i0 + e;
             if (e < E && i < c && rep[e] == i) {
                 a.rank[L][i] = rk;
+                rank_l[i] = rk;
                 a.uniq[L][rk] = i;
                 if (a.sizes_in_merge) a.off[L][rk] = of;
                 ++rk;
@@ -1143,7 +1189,11 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
 #pragma unroll
         for (u32 e = 0; e < kSmallPer; ++e) {
             const u32 i = i0 + e;
-            if (e < E && i < c && rep[e] != i) a.rank[L][i] = a.rank[L][rep[e]];
+            if (e < E && i < c && rep[e] != i) {
+                const u32 r = rank_l[rep[e]];
+                a.rank[L][i] = r;
+                rank_l[i] = r;
+            }
         }
         __syncthreads();  // the next level reads these ranks
     }
