@@ -53,10 +53,10 @@ def table_slots(n):
 def leaf_waves_bytes(N, nodes_leaf):
     """Compulsory traffic of k_leaf_waves: the 8x8 depth block of every level-3 column (256 B), per
     leaf node its breadth-first index read and its run-head index and leafmask slot written
-    (12 B), and the clear of the leaf level's merge table (8 B per slot), which this kernel does.
-    The first node of each run also writes mask, hash and leafmasks; those bytes are left out, so
-    the figure is a lower bound."""
-    return 256 * (N // 8) ** 2 + 12 * nodes_leaf + 8 * table_slots(nodes_leaf)
+    (12 B), and the clear of the leaf level's merge table (8 B per slot; the table is sized for a
+    quarter of the leaf nodes), which this kernel does.  The first node of each run also writes
+    mask, hash and leafmasks; those bytes are left out, so the figure is a lower bound."""
+    return 256 * (N // 8) ** 2 + 12 * nodes_leaf + 8 * table_slots(nodes_leaf // 4)
 
 
 def pyramid_root_bytes(N):

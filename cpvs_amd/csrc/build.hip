@@ -1269,11 +1269,16 @@ u32 table_slots(u32 n) {
     while (S < n + n / 2 + 1) S <<= 1;
     return S;
 }
-u32 leaf_table_slots(u32 c, bool small) { return table_slots(small ? c / 4 : c); }
+// Below 2^20 leaf nodes the table is small anyway, and small maps have more distinct leaves per
+// node (measured: 256^2 and 512^2 maps fill 85-96% of a quarter-size table, 1024^2 40%, 8192^2 4%).
+constexpr u32 kSmallLeafTableMin = 1u << 20;
+u32 leaf_table_slots(u32 c, bool small) { return table_slots((small && c >= kSmallLeafTableMin) ? c / 4 : c); }
 
-// small_leaf: the leaf level's merge table is sized for a quarter of its nodes (only a run's first
-// node is inserted: a fifth of them at 8192^2), which keeps the table's clear and its random
-// accesses small; a table found too small is reported as kRetryFullTable.
+// small_leaf: a leaf level of at least kSmallLeafTableMin nodes gets a merge table sized for a
+// quarter of them (only a run's first node is inserted: a fifth of them at 8192^2, with 4% of the
+// nodes distinct), which keeps the table's clear and its random accesses small; a table found too
+// small (maps of mostly distinct leaves, such as white noise) is reported as kRetryFullTable and
+// the build runs again.
 int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, int merge_mode,
                cpvs_dag** out, const Admit* admit, bool small_leaf) {
     const int n = p->n;

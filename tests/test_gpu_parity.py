@@ -302,15 +302,16 @@ def test_hash_collision_path_exercised_by_many_duplicates(ctx):
 
 def test_leaf_table_overflow_builds_again(ctx):
     # White noise: almost every leaf node is distinct, more keys than the leaf level's merge table
-    # (sized for a quarter of its nodes) has slots, so inserts give up and the build runs again with
-    # a table for every node.
-    N = 512
+    # (sized for a quarter of its nodes from 2^20 leaf nodes on) has slots, so inserts give up and
+    # the build runs again with a table for every node.
+    N = 1024
     D = np.random.default_rng(7).random((N, N), dtype=np.float32)
     _say("noise", N)
     cs, w, n = _gpu_dag(D)
     wr, _ = _oracle_dag(D)
     assert np.array_equal(w, wr)
     nodes, uniq = cs.level_stats()
+    assert nodes[2] >= 1 << 20
     slots = 1024
     while slots < nodes[2] // 4 + nodes[2] // 8 + 1:
         slots <<= 1
