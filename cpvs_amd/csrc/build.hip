@@ -174,20 +174,30 @@ struct EmitChildrenOut {
         u32 j = first;
         // getChildCoordinates (src/CompressedShadowUtil.cpp:101-117): partial children in
         // ascending child index, child cell doubled.
+        if (cols.bfs) {
+            // Leaf children: children ch and ch + 4 lie in one column, so the node's (at most
+            // four) column records are loaded first, all together, and the stores follow.
+            uint4 ci[4];
+#pragma unroll
+            for (u32 q = 0; q < 4; ++q) {
+                ci[q] = make_uint4(0u, 0u, 0u, 0u);
+                if (m & ((2u << (2 * q)) | (2u << (2 * (q + 4))))) ci[q] = cols.info[(y + (q >> 1)) * cols.w3 + x + (q & 1)];
+            }
+#pragma unroll
+            for (u32 ch = 0; ch < 8; ++ch) {
+                if (!(m & (2u << (2 * ch)))) continue;
+                const uint4 r = ci[ch & 3];
+                const u32 slot = r.y + (z + (ch >> 2) - r.z);
+                if (j < child_cap && slot < child_cap) cols.bfs[slot] = j;
+                ++j;
+            }
+            return;
+        }
 #pragma unroll
         for (u32 ch = 0; ch < 8; ++ch) {
             if (!(m & (2u << (2 * ch)))) continue;
             const u32 cx = x + (ch & 1), cy = y + ((ch >> 1) & 1), cz = z + (ch >> 2);
-            if (j < child_cap) {
-                if (cols.bfs) {
-                    const u32 col = cy * cols.w3 + cx;
-                    const uint4 ci = cols.info[col];
-                    const u32 slot = ci.y + (cz - ci.z);
-                    if (slot < child_cap) cols.bfs[slot] = j;
-                } else {
-                    child_coord[j] = pack_xyz(2 * cx, 2 * cy, 2 * cz);
-                }
-            }
+            if (j < child_cap) child_coord[j] = pack_xyz(2 * cx, 2 * cy, 2 * cz);
             ++j;
         }
     }
