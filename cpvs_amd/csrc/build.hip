@@ -453,11 +453,13 @@ struct LeafEq {
             b = runhead[b];
             if (a == b) return true;
         }
-        const u32 m = mask[a];
-        if (m != mask[b]) return false;
+        // Masks and leafmask slots are loaded together (one round trip before the leafmasks).
+        const u32 m = mask[a], mb = mask[b];
+        const u32 sa = leafslot[a], sb = leafslot[b];
+        if (m != mb) return false;
         const u32 k = num_children(m);
-        const u64* pa = leafm + (size_t)leafslot[a] * 8;
-        const u64* pb = leafm + (size_t)leafslot[b] * 8;
+        const u64* pa = leafm + (size_t)sa * 8;
+        const u64* pb = leafm + (size_t)sb * 8;
         bool eq = true;
 #pragma unroll
         for (u32 j = 0; j < 8; ++j)
@@ -470,13 +472,18 @@ struct InnerEq {
     const u32* cstart;
     const u32* child_rank;
     __device__ bool operator()(u32 a, u32 b) const {
-        const u32 m = mask[a];
-        if (m != mask[b]) return false;
+        // Masks and first-child indices are loaded together (one round trip before the ranks).
+        const u32 m = mask[a], mb = mask[b];
+        const u32 ca = cstart[a], cb = cstart[b];
+        if (m != mb) return false;
         const u32 k = num_children(m);
         if (!k) return true;
         u32 ra[8], rb[8];
-        load_child_ranks(cstart, child_rank, a, k, ra);
-        load_child_ranks(cstart, child_rank, b, k, rb);
+#pragma unroll
+        for (u32 j = 0; j < 8; ++j) {
+            ra[j] = (j < k) ? child_rank[ca + j] : 0u;
+            rb[j] = (j < k) ? child_rank[cb + j] : 0u;
+        }
         bool eq = true;
 #pragma unroll
         for (u32 j = 0; j < 8; ++j) eq = eq && (ra[j] == rb[j]);
