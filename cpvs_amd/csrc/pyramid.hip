@@ -33,7 +33,11 @@ __global__ __launch_bounds__(256) void k_pyr_tile(const float* __restrict__ in, 
     if (ROOT) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const float4 v = *reinterpret_cast<const float4*>(in + (size_t)(y0 + r) * W + x0);
+            // Streamed past the caches (nontemporal): the map is read once here; measured at
+            // 8192^2 the root launch takes 60 -> 52 µs, the leaf kernel's later depth reads 302 ->
+            // 306 µs.
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(in + (size_t)(y0 + r) * W + x0));
             p[r][0] = make_float2(v.x, v.x);
             p[r][1] = make_float2(v.y, v.y);
             p[r][2] = make_float2(v.z, v.z);
