@@ -322,7 +322,8 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
         for (u32 k = 0; k < kLeafCols; ++k) {
             const u32 c = kList ? __builtin_amdgcn_readlane(my_col, k) : min(c0 + k, clast);
             const u32 x3 = c & (lc.w3 - 1), y3 = c >> lc.w3_shift;
-            d[k] = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
+            // Nontemporal: the map's last read (the merge kernels that follow keep more of the L2).
+            d[k] = __builtin_nontemporal_load(depth + (size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7));
         }
     }
     u32 b[kLeafCols];
