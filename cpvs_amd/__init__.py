@@ -484,7 +484,7 @@ class Server:
     arrival order (the plain path).  Requests borrow their device buffers until waited for."""
 
     STAT_NAMES = ("builds", "lookups", "batches", "queries", "evicted", "resident", "resident_bytes",
-                  "scratch_bytes", "reserved_bytes", "budget")
+                  "scratch_bytes", "reserved_bytes", "budget", "trims", "cached_bytes")
 
     def __init__(self, lanes: int = 4, budget_bytes: int = 200 << 30, serial: bool = False,
                  ctx: Optional[Context] = None):

@@ -847,24 +847,33 @@ __global__ void k_start(u64* root_coord, u64 root, u32* meta, u32 meta_words) {
 // ---------------------------------------------------------------------------------------
 // 4. Compress.
 // ---------------------------------------------------------------------------------------
+// Sizes of the unique nodes of a level whose word count may pass 2^32 (the levels not ranked by
+// the fused 64-bit scan hold at least 2^32 / 17 nodes).  The sums run in 64 bits; an offset or a
+// level total at or above 2^32 is stored saturated, so the host sees the overflow and rejects the
+// DAG before anything is written.
 struct UniqueSizeIn {
     const u32* uniq;
     const u32* mask;
     const u32* n_uniq;  // device count of uniques of the level
     u32 leaf;           // 1 at the leafmask level (two words per partial child)
-    __device__ u32 operator()(u32 r) const {
-        if (r >= *n_uniq) return 0u;
-        return 1u + (num_children(mask[uniq[r]]) << leaf);
+    __device__ u64 operator()(u32 r) const {
+        if (r >= *n_uniq) return 0ull;
+        return 1ull + (u64)(num_children(mask[uniq[r]]) << leaf);
     }
 };
 
+__device__ inline u32 saturate32(u64 v) { return v > 0xFFFFFFFFull ? 0xFFFFFFFFu : (u32)v; }
+
 // Offsets of the unique nodes only (the scan runs over the level's node count, which the host
-// knows, while the unique count is on the device).
+// knows, while the unique count is on the device), and the level's word count.
 struct UniqueOffOut {
     u32* off;
     const u32* n_uniq;
-    __device__ void operator()(u32 r, u32 ex, u32) const {
-        if (r < *n_uniq) off[r] = ex;
+    u32* words;  // device: the level's word count (saturated)
+    u32 n;
+    __device__ void operator()(u32 r, u64 ex, u64 v) const {
+        if (r < *n_uniq) off[r] = saturate32(ex);
+        if (r == n - 1) *words = saturate32(ex + v);
     }
 };
 
@@ -1276,6 +1285,13 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_compress(SmallArgs a) {
     }
 }
 
+// The DAG's word buffer: from the admission's allocator when it has one (a server keeps its budget
+// there), else from the context's buffer cache.
+void* alloc_words(cpvs_ctx* ctx, const Admit* admit, u64 words) {
+    const size_t bytes = sizeof(u32) * words;
+    return (admit && admit->alloc) ? admit->alloc(admit->user, bytes) : ctx->buffers.take(bytes);
+}
+
 inline dim3 grid1d(u32 n, u32 block = 256) { return dim3(ceil_div<u32>(n ? n : 1, block)); }
 
 constexpr int kRetryWithSort = -1000;
@@ -1329,7 +1345,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             rc = admit->fn(admit->user, A, 0, sizeof(u32));
             if (rc) return rc;
         }
-        u32* d_words = static_cast<u32*>(ctx->buffers.take(sizeof(u32)));
+        u32* d_words = static_cast<u32*>(alloc_words(ctx, admit, 1));
         if (!d_words) {
             set_error("build: hipMalloc of one DAG word failed");
             return CPVS_ENOMEM;
@@ -1598,7 +1614,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         const u32 c = lv[L].count;
         const u32 is_leaf = (leaf && L == 2) ? 1u : 0u;
         device_scan(UniqueSizeIn{lv[L].uniq, lv[L].mask, meta + kMetaU + L, is_leaf},
-                    UniqueOffOut{lv[L].off, meta + kMetaU + L}, c, OpSum{}, meta + kMetaS + L, scan_scratch, s);
+                    UniqueOffOut{lv[L].off, meta + kMetaU + L, meta + kMetaS + L, c}, c, OpSum64{}, nullptr, scan_scratch, s);
     }
     if (Ls <= R) {
         if (!sa.sizes_in_merge) CPVS_LAUNCH("small_compress", k_small_compress, dim3(1), dim3(kSmallThreads), 0, s, sa);
@@ -1630,7 +1646,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         return CPVS_EINVAL;
     }
 
-    u32* d_words = static_cast<u32*>(ctx->buffers.take(sizeof(u32) * (total ? total : 1)));
+    u32* d_words = static_cast<u32*>(alloc_words(ctx, admit, total ? total : 1));
     if (!d_words) {
         set_error("build: hipMalloc of %llu DAG words failed", (unsigned long long)total);
         return CPVS_ENOMEM;

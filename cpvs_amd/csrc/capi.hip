@@ -1,4 +1,5 @@
 // capi.hip -- the extern "C" boundary (include/cpvs.h): handles, argument checks, errors.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -25,21 +26,31 @@ void set_error(const char* fmt, ...) {
 thread_local Prof* t_prof = nullptr;
 
 void Prof::resolve() {
-    for (auto& r : pending) {
+    std::vector<Rec> recs;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        recs.swap(pending);
+    }
+    std::vector<std::pair<const char*, float>> done;
+    done.reserve(recs.size());
+    for (auto& r : recs) {
         float ms = 0.0f;
-        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
-            auto& e = acc[r.name];
-            e.first += ms;
-            e.second += 1;
-        }
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess)
+            done.emplace_back(r.name, ms);
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
-    pending.clear();
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& d : done) {
+        auto& e = acc[d.first];
+        e.first += d.second;
+        e.second += 1;
+    }
 }
 
 void Prof::reset() {
     resolve();
+    std::lock_guard<std::mutex> lk(mu);
     acc.clear();
 }
 
@@ -118,19 +129,49 @@ int cpvs_ctx_create(int device, cpvs_ctx** out) {
 
 namespace cpvs {
 
-void* BufferCache::take(size_t bytes) {
+namespace {
+// Best fit of at most `most` bytes, from one of the cache's lists; mu held.
+void* fit_from(std::multimap<size_t, void*>& m, size_t bytes, size_t most, size_t* got) {
+    auto it = m.lower_bound(bytes);
+    if (it == m.end() || it->first > most) return nullptr;
+    void* p = it->second;
+    *got = it->first;
+    m.erase(it);
+    return p;
+}
+}  // namespace
+
+void* BufferCache::take_cached(size_t bytes, size_t limit) {
     bytes = bytes ? bytes : 1;
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        auto it = free_.lower_bound(bytes);
-        if (it != free_.end() && it->first <= 2 * bytes) {
-            void* p = it->second;
-            live_[p] = it->first;
-            cached -= it->first;
-            free_.erase(it);
-            return p;
+    const size_t most = std::min(2 * bytes, std::max(limit, bytes));
+    std::unique_lock<std::mutex> lk(mu);
+    size_t got = 0;
+    void* p = fit_from(free_, bytes, most, &got);
+    if (!p && !pending_.empty()) {
+        auto it = pending_.lower_bound(bytes);
+        if (it != pending_.end() && it->first <= most) {
+            // Every buffer freed so far becomes idle with one device synchronisation (those freed
+            // while it runs stay pending: their readers may have been queued after it began).
+            std::multimap<size_t, void*> parked;
+            parked.swap(pending_);
+            lk.unlock();
+            (void)hipSetDevice(device);
+            (void)hipDeviceSynchronize();
+            lk.lock();
+            ++device_syncs;
+            free_.insert(parked.begin(), parked.end());
+            p = fit_from(free_, bytes, most, &got);
         }
     }
+    if (!p) return nullptr;
+    live_[p] = got;
+    cached -= got;
+    return p;
+}
+
+void* BufferCache::take(size_t bytes) {
+    bytes = bytes ? bytes : 1;
+    if (void* p = take_cached(bytes)) return p;
     void* p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -145,26 +186,69 @@ void* BufferCache::take(size_t bytes) {
     return p;
 }
 
-void BufferCache::give(void* p) {
+void BufferCache::park(void* p, bool idle) {
     if (!p) return;
-    (void)hipSetDevice(device);
-    (void)hipDeviceSynchronize();  // no kernel on any stream still reads p
+    bool drop = false;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = live_.find(p);
+        const size_t bytes = (it != live_.end()) ? it->second : 0;
+        if (it != live_.end()) live_.erase(it);
+        if (bytes && cached + bytes <= kMaxBytes) {
+            (idle ? free_ : pending_).emplace(bytes, p);
+            cached += bytes;
+        } else {
+            drop = true;
+        }
+    }
+    if (drop) {
+        (void)hipSetDevice(device);
+        (void)hipFree(p);  // waits for the device: no queued work still reads p
+    }
+}
+
+void BufferCache::give(void* p) { park(p, false); }
+void BufferCache::give_idle(void* p) { park(p, true); }
+
+size_t BufferCache::size_of(void* p) {
     std::lock_guard<std::mutex> lk(mu);
     auto it = live_.find(p);
-    const size_t bytes = (it != live_.end()) ? it->second : 0;
-    if (it != live_.end()) live_.erase(it);
-    if (bytes && cached + bytes <= kMaxBytes) {
-        free_.emplace(bytes, p);
-        cached += bytes;
-    } else {
-        (void)hipFree(p);
+    return it == live_.end() ? 0 : it->second;
+}
+
+size_t BufferCache::cached_bytes() {
+    std::lock_guard<std::mutex> lk(mu);
+    return cached;
+}
+
+size_t BufferCache::trim(size_t keep) {
+    std::vector<void*> drop;
+    size_t freed = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto* m : {&free_, &pending_}) {
+            while (cached > keep && !m->empty()) {
+                auto it = std::prev(m->end());  // largest first
+                drop.push_back(it->second);
+                cached -= it->first;
+                freed += it->first;
+                m->erase(it);
+            }
+        }
+        if (!drop.empty()) ++trims;
     }
+    (void)hipSetDevice(device);
+    for (void* p : drop) (void)hipFree(p);  // hipFree waits for the device
+    return freed;
 }
 
 void BufferCache::release() {
     std::lock_guard<std::mutex> lk(mu);
-    for (auto& kv : free_) (void)hipFree(kv.second);
-    free_.clear();
+    (void)hipSetDevice(device);
+    for (auto* m : {&free_, &pending_}) {
+        for (auto& kv : *m) (void)hipFree(kv.second);
+        m->clear();
+    }
     cached = 0;
 }
 
@@ -203,10 +287,10 @@ int cpvs_ctx_profile(cpvs_ctx* ctx, int mode) {
 
 int cpvs_ctx_profile_read(cpvs_ctx* ctx, char* buf, uint64_t cap) {
     CHECK_ARG(ctx && buf && cap, "ctx_profile_read: NULL argument");
-    ctx->prof.resolve();
+    const auto acc = ctx->prof.snapshot();
     std::string out;
     char line[256];
-    for (const auto& kv : ctx->prof.acc) {
+    for (const auto& kv : acc) {
         snprintf(line, sizeof(line), "%s\t%.6f\t%ld\n", kv.first.c_str(), kv.second.first, kv.second.second);
         out += line;
     }
@@ -500,6 +584,18 @@ int cpvs_dag_traverse(cpvs_dag* d, const float* ndc, uint64_t count, int try_lea
     return rc;
     GUARD_END
 }
+
+}  // extern "C"
+
+namespace cpvs {
+void dag_destroy_idle(cpvs_dag* d) {
+    if (!d) return;
+    if (d->d_words) d->ctx->buffers.give_idle(d->d_words);
+    delete d;
+}
+}  // namespace cpvs
+
+extern "C" {
 
 int cpvs_dag_destroy(cpvs_dag* d) {
     if (!d) return CPVS_OK;

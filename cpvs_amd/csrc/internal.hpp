@@ -65,19 +65,35 @@ struct Worker {
 
 namespace cpvs {
 // Freed device buffers of DAG words and pyramid levels kept for reuse: a hipMalloc of a few MB costs
-// about 14 us of host time, during which a build's GPU work waits.  A buffer enters the cache only
-// after hipDeviceSynchronize (what hipFree guarantees too), so no kernel still reads it when it is
-// handed out again.  Best fit within twice the request; at most kMaxBytes are kept.
+// about 14 us of host time, during which a build's GPU work waits.  Best fit within twice the
+// request; at most kMaxBytes are kept (beyond that a buffer is released with hipFree, which waits
+// for the device).
+// A freed buffer may still be read by work queued on any stream, so give() parks it as pending
+// and the free itself waits for nothing.  A pending buffer is handed out again only after a
+// device synchronisation that follows its free: take() performs one when no ready buffer fits and
+// a pending one would, which readies every pending buffer at once.  give_idle() is for buffers
+// the caller knows no queued work reads (a server's evicted DAG: no lookup pins it, its build has
+// completed); they are ready at once.
 struct BufferCache {
     static constexpr size_t kMaxBytes = size_t(2) << 30;
     std::mutex mu;
-    std::multimap<size_t, void*> free_;   // size -> cached buffer
+    std::multimap<size_t, void*> free_;     // size -> ready buffer
+    std::multimap<size_t, void*> pending_;  // size -> freed buffer not yet known idle
     std::unordered_map<void*, size_t> live_;  // handed-out buffer -> its size
-    size_t cached = 0;
+    size_t cached = 0;   // bytes in free_ and pending_
+    uint64_t device_syncs = 0, trims = 0;
     int device = 0;
-    void* take(size_t bytes);   // nullptr when hipMalloc fails (even after emptying the cache)
-    void give(void* p);         // p from take(); null is ignored
-    void release();             // hipFree of every cached buffer
+    void* take(size_t bytes);            // nullptr when hipMalloc fails (even after emptying the cache)
+    // A cached buffer of at most min(2 * bytes, limit) bytes, or nullptr; never allocates.
+    void* take_cached(size_t bytes, size_t limit = ~size_t(0));
+    void give(void* p);                  // p from take(); null is ignored
+    void give_idle(void* p);             // p from take(), with no queued work reading it
+    size_t size_of(void* p);             // bytes of a handed-out buffer (0 if unknown)
+    size_t cached_bytes();
+    size_t trim(size_t keep);            // hipFree cached buffers until at most `keep` bytes stay
+    void release();                      // hipFree of every cached buffer
+private:
+    void park(void* p, bool idle);
 };
 }  // namespace cpvs
 
@@ -167,7 +183,15 @@ void sum_node_counts(const CountLayout& lay, const u32* partials, u64* cnt);
 struct Admit {
     int (*fn)(void* user, const Arena& arena, u64 scratch_bytes, u64 dag_bytes_max);
     void* user;
+    // Allocates the DAG's words (within the admitted bound) when set; the context's buffer cache
+    // otherwise.  nullptr = out of memory.
+    void* (*alloc)(void* user, size_t bytes) = nullptr;
 };
+
+// Destroys a DAG whose words no queued work reads: its buffer is ready for reuse at once.
+void dag_destroy_idle(cpvs_dag* d);
+// Bytes of the stored pyramid levels of an N x N map (levels >= 3, or all with store_all).
+size_t pyramid_levels_bytes(u32 N, bool store_all);
 
 // build.hip: one DAG from a pyramid on stream s, scratch from A, readbacks staged in pinned
 // (kPinnedWords words of pinned host memory).

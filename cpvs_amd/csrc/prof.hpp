@@ -7,6 +7,7 @@
 #pragma once
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,10 +20,22 @@ struct Prof {
         const char* name;
         hipEvent_t a, b;
     };
+    // Builds (under the context's lock) and lookups (without it) may record at once, and a reader
+    // may resolve meanwhile: every access to pending and acc holds mu.
+    std::mutex mu;
     std::vector<Rec> pending;
     std::map<std::string, std::pair<double, long>> acc;  // name -> (total ms, launches)
+    void add(const char* name, hipEvent_t a, hipEvent_t b) {
+        std::lock_guard<std::mutex> lk(mu);
+        pending.push_back({name, a, b});
+    }
     void resolve();
     void reset();
+    std::map<std::string, std::pair<double, long>> snapshot() {
+        resolve();
+        std::lock_guard<std::mutex> lk(mu);
+        return acc;
+    }
 };
 
 extern thread_local Prof* t_prof;
@@ -39,7 +52,7 @@ struct ProfScope {
         hipEvent_t b = nullptr;
         if (hipEventCreate(&b) == hipSuccess) {
             (void)hipEventRecord(b, s);
-            t_prof->pending.push_back({name, a, b});
+            t_prof->add(name, a, b);
         } else {
             (void)hipEventDestroy(a);
         }

@@ -133,6 +133,16 @@ __global__ __launch_bounds__(256) void k_pyr_small(const float* __restrict__ in,
     }
 }
 
+size_t pyramid_levels_bytes(u32 N, bool store_all) {
+    int n = 1;
+    while ((1u << (n - 1)) < N) ++n;
+    const bool keep12 = store_all || n < 5;
+    size_t total = 0;
+    for (int L = 1; L < n; ++L)
+        if (L > 2 || keep12) total += (size_t)(N >> L) * (N >> L);
+    return arena_bytes(total ? total : 1, sizeof(float2));
+}
+
 int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* into) {
     const u32 N = p->N;
     const int n = p->n;

@@ -81,7 +81,7 @@ struct cpvs_server {
     std::unordered_map<uint64_t, Entry> store;
     uint64_t resident_bytes = 0, reserved_bytes = 0, scratch_bytes = 0;
     uint64_t next_ticket = 1, clock = 0;
-    uint64_t n_builds = 0, n_lookups = 0, n_batches = 0, n_evicted = 0, n_queries = 0;
+    uint64_t n_builds = 0, n_lookups = 0, n_batches = 0, n_evicted = 0, n_queries = 0, n_trims = 0;
     bool stop = false;
     std::vector<Worker*> lanes;
     Worker* lk = nullptr;          // lookup lane
@@ -95,7 +95,9 @@ namespace {
 
 // Called with s->mu held.  Evicts unpinned DAGs, least recently used first, until `need` more
 // bytes fit; false if they cannot.  The evicted DAGs go to `victims`, to be destroyed once the
-// lock is released (hipFree waits for the device).
+// lock is released.  No lookup pins a victim and its build has completed, so no queued work
+// reads its words: they return to the context's buffer cache at once, with no device
+// synchronisation, and the next build's words are taken from there.
 bool make_room(cpvs_server* s, uint64_t need, std::vector<cpvs_dag*>& victims) {
     while (s->resident_bytes + s->reserved_bytes + s->scratch_bytes + need > s->budget) {
         auto victim = s->store.end();
@@ -112,25 +114,25 @@ bool make_room(cpvs_server* s, uint64_t need, std::vector<cpvs_dag*>& victims) {
 }
 
 void destroy_all(std::vector<cpvs_dag*>& v) {
-    for (cpvs_dag* d : v) cpvs_dag_destroy(d);
+    for (cpvs_dag* d : v) dag_destroy_idle(d);
     v.clear();
 }
 
 struct AdmitState {
     cpvs_server* s;
     uint32_t lane;
-    uint64_t reserved = 0;  // bytes reserved by this build (DAG bound + scratch growth)
+    uint64_t reserved = 0;   // bytes reserved by this build (DAG bound + scratch growth)
+    uint64_t dag_bound = 0;  // the part of it that bounds the DAG's word buffer
 };
 
-int admit_build(void* user, const Arena& arena, u64 scratch_bytes, u64 dag_bytes_max) {
-    AdmitState* a = static_cast<AdmitState*>(user);
+// Reserves `need` bytes for lane a (replacing what it reserved before), evicting or waiting for
+// memory; CPVS_EBUDGET when they can never fit.
+int reserve_bytes(AdmitState* a, uint64_t need) {
     cpvs_server* s = a->s;
     std::vector<cpvs_dag*> victims;
     std::unique_lock<std::mutex> lk(s->mu);
     s->reserved_bytes -= a->reserved;  // a retry replaces the first request
     a->reserved = 0;
-    const uint64_t growth = scratch_bytes > arena.cap ? scratch_bytes + (scratch_bytes >> 3) - arena.cap : 0;
-    const uint64_t need = growth + dag_bytes_max;
     int rc = CPVS_OK;
     for (;;) {
         if (make_room(s, need, victims)) break;
@@ -156,6 +158,37 @@ int admit_build(void* user, const Arena& arena, u64 scratch_bytes, u64 dag_bytes
     return rc;
 }
 
+int admit_build(void* user, const Arena& arena, u64 scratch_bytes, u64 dag_bytes_max) {
+    AdmitState* a = static_cast<AdmitState*>(user);
+    const uint64_t growth = scratch_bytes > arena.cap ? scratch_bytes + (scratch_bytes >> 3) - arena.cap : 0;
+    a->dag_bound = dag_bytes_max;
+    return reserve_bytes(a, growth + dag_bytes_max);
+}
+
+// A build's DAG words: a cached buffer when one fits, else new memory.  Device memory of the
+// server is its resident DAGs, the reservations of builds in flight (this one's included, which
+// bounds these words), the lanes' arenas and the context's cached buffers; before new memory is
+// taken the cache is trimmed so that this sum stays within the budget (hipFree waits for the
+// device, so this happens only when no cached buffer fits and the budget is tight).
+void* alloc_dag_words(void* user, size_t bytes) {
+    AdmitState* a = static_cast<AdmitState*>(user);
+    cpvs_server* s = a->s;
+    BufferCache& bc = s->ctx->buffers;
+    if (void* p = bc.take_cached(bytes, a->dag_bound)) return p;  // within the reservation
+    uint64_t used;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        used = s->resident_bytes + s->reserved_bytes + s->scratch_bytes;
+    }
+    const uint64_t keep = used < s->budget ? s->budget - used : 0;
+    if (bc.cached_bytes() > keep) {
+        bc.trim(keep);
+        std::lock_guard<std::mutex> lk(s->mu);
+        ++s->n_trims;
+    }
+    return bc.take(bytes);
+}
+
 void finish(cpvs_server* s, Req* r, int status) {
     r->status = status;
     if (status != CPVS_OK) r->error = cpvs_last_error();
@@ -171,9 +204,21 @@ int run_build(cpvs_server* s, uint32_t li, Req* r) {
     p.n = levels_of(r->N);
     p.depth = const_cast<float*>(r->depth);
     p.borrowed = true;
-    int rc = pyramid_build_levels(&p, false, w.stream, &w.levels);
     AdmitState a{s, li, 0};
-    Admit adm{admit_build, &a};
+    int rc = CPVS_OK;
+    // The lane's pyramid-level arena grows before the build's admission: reserve that growth first.
+    const size_t lv_bytes = pyramid_levels_bytes(r->N, false);
+    if (lv_bytes > w.levels.cap) rc = reserve_bytes(&a, lv_bytes + (lv_bytes >> 3) - w.levels.cap);
+    if (rc == CPVS_OK) rc = pyramid_build_levels(&p, false, w.stream, &w.levels);
+    if (rc == CPVS_OK && a.reserved) {
+        std::lock_guard<std::mutex> lk(s->mu);  // the growth is lane scratch from here on
+        s->reserved_bytes -= a.reserved;
+        a.reserved = 0;
+        s->scratch_bytes -= s->lane_scratch[li];
+        s->lane_scratch[li] = w.arena.cap + w.levels.cap;
+        s->scratch_bytes += s->lane_scratch[li];
+    }
+    Admit adm{admit_build, &a, alloc_dag_words};
     cpvs_dag* d = nullptr;
     if (rc == CPVS_OK) rc = build_dag(s->ctx, w.arena, w.pinned, s->ctx->merge_mode, &p, r->t, r->T, w.stream, &d, &adm);
     std::lock_guard<std::mutex> lk(s->mu);
@@ -184,7 +229,7 @@ int run_build(cpvs_server* s, uint32_t li, Req* r) {
     if (rc == CPVS_OK) {
         Entry e;
         e.dag = d;
-        e.bytes = 4 * d->words;
+        e.bytes = s->ctx->buffers.size_of(d->d_words);  // the buffer, which may exceed the words
         e.last_use = ++s->clock;
         s->store[r->ticket] = e;
         s->resident_bytes += e.bytes;
@@ -492,7 +537,7 @@ int cpvs_server_release(cpvs_server* s, uint64_t dag) {
         s->resident_bytes -= it->second.bytes;
         s->store.erase(it);
     }
-    cpvs_dag_destroy(d);
+    dag_destroy_idle(d);  // unpinned: no lookup reads it
     s->cv_mem.notify_all();
     return CPVS_OK;
 }
@@ -512,7 +557,8 @@ int cpvs_server_stats(cpvs_server* s, uint64_t* out, uint32_t cap) {
     CHECK_ARG(s && out, "server_stats: NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
     const uint64_t v[] = {s->n_builds, s->n_lookups, s->n_batches, s->n_queries, s->n_evicted,
-                          (uint64_t)s->store.size(), s->resident_bytes, s->scratch_bytes, s->reserved_bytes, s->budget};
+                          (uint64_t)s->store.size(), s->resident_bytes, s->scratch_bytes, s->reserved_bytes, s->budget,
+                          s->n_trims, (uint64_t)s->ctx->buffers.cached_bytes()};
     for (uint32_t i = 0; i < cap && i < sizeof(v) / sizeof(v[0]); ++i) out[i] = v[i];
     return CPVS_OK;
 }

@@ -220,6 +220,13 @@ int cpvs_dag_evaluate(const cpvs_dag *dag, const float *ndc, uint32_t stride, ui
  * within hbm_budget_bytes (resident DAGs + builds in flight, each reserving its scratch growth
  * and an upper bound of its DAG from the exact node counts).  Least recently used DAGs without
  * queued lookups are evicted to admit a build; a build that cannot fit fails with CPVS_EBUDGET.
+ * Resident bytes are the DAGs' buffer sizes; the lanes' pyramid-level and scratch arenas count as
+ * lane scratch, and their growth is admitted before it happens.  An evicted or released DAG's
+ * buffer returns to the context's buffer cache at once, with no device synchronisation (no lookup
+ * pins it), and later builds take their words from there; when no cached buffer fits, the cache is
+ * trimmed (hipFree) so that resident + reserved + scratch + cached bytes stay within the budget.
+ * Work a caller queues itself on a DAG from cpvs_server_dag must therefore complete before that
+ * DAG is released or can be evicted.
  * serial != 0: one request at a time in arrival order on one stream (the plain path). */
 typedef struct cpvs_server cpvs_server;
 int cpvs_server_create(cpvs_ctx *ctx, uint32_t lanes, uint64_t hbm_budget_bytes, int serial, cpvs_server **out);
@@ -236,7 +243,7 @@ int cpvs_server_dag(cpvs_server *s, uint64_t dag, const cpvs_dag **out);
 int cpvs_server_release(cpvs_server *s, uint64_t dag);
 int cpvs_server_resident(cpvs_server *s, uint64_t *ids, uint32_t cap, uint32_t *count);
 /* builds, lookups, lookup launches, queries, evictions, resident DAGs, resident bytes, lane scratch
- * bytes, reserved bytes, budget (the first cap of them). */
+ * bytes, reserved bytes, budget, cache trims, the context's cached bytes (the first cap of them). */
 int cpvs_server_stats(cpvs_server *s, uint64_t *out, uint32_t cap);
 /* Finishes every queued request, then frees the resident DAGs. */
 int cpvs_server_destroy(cpvs_server *s);

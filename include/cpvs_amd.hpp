@@ -134,7 +134,9 @@ public:
         return std::unique_ptr<CompressedShadow>(new CompressedShadow(d));
     }
 
-    ~CompressedShadow() { cpvs_dag_destroy(d_); }
+    ~CompressedShadow() {
+        if (owns_) cpvs_dag_destroy(d_);
+    }
     CompressedShadow(const CompressedShadow&) = delete;
     CompressedShadow& operator=(const CompressedShadow&) = delete;
 
@@ -179,17 +181,23 @@ public:
         return std::unique_ptr<CompressedShadow>(new CompressedShadow(d));
     }
 
+    const cpvs_dag* handle() const { return d_; }
+
 private:
-    explicit CompressedShadow(cpvs_dag* d) : d_(d) {}
+    friend class CompressedShadowContainer;
+    // owns = false: a view of a DAG a container owns (what CompressedShadowContainer::get returns).
+    explicit CompressedShadow(cpvs_dag* d, bool owns = true) : d_(d), owns_(owns) {}
     cpvs_dag* d_ = nullptr;
+    bool owns_ = true;
     mutable std::vector<uint> host_;
 };
 
 // CompressedShadowContainer (src/CompressedShadowContainer.h:18-92).
 class CompressedShadowContainer {
 public:
-    explicit CompressedShadowContainer(uint length) {
+    explicit CompressedShadowContainer(uint length) : length_(length) {
         check(cpvs_container_create(Context::instance().get(), length, &c_), "cpvs_container_create");
+        views_.resize((size_t)length * length * length);
     }
     explicit CompressedShadowContainer(std::unique_ptr<CompressedShadow> shadow) : CompressedShadowContainer(1u) {
         set(std::move(shadow), 0, 0, 0);
@@ -199,13 +207,17 @@ public:
     CompressedShadowContainer& operator=(const CompressedShadowContainer&) = delete;
 
     void set(std::unique_ptr<CompressedShadow> shadow, uint x, uint y, uint z) {
-        check(cpvs_container_set(c_, shadow->release(), x, y, z), "cpvs_container_set");
+        cpvs_dag* d = shadow->d_;
+        check(cpvs_container_set(c_, d, x, y, z), "cpvs_container_set");
+        shadow->release();  // the container owns the DAG now
+        views_[slot(x, y, z)].reset(new CompressedShadow(d, false));
     }
-    // get (src/CompressedShadowContainer.h:41-46): the DAG handle, null after freeOnCPU.
-    const cpvs_dag* get(uint x, uint y, uint z) const {
+    // get (src/CompressedShadowContainer.h:41-46): the shadow in slot (x, y, z), owned by the
+    // container; null when the slot is empty or after freeOnCPU / moveToGPU.
+    const CompressedShadow* get(uint x, uint y, uint z) const {
         const cpvs_dag* d = nullptr;
         check(cpvs_container_get(c_, x, y, z, &d), "cpvs_container_get");
-        return d;
+        return d ? views_[slot(x, y, z)].get() : nullptr;
     }
     // evaluate (src/CompressedShadowContainer.cpp:93-124) over device positions (3 or 4 floats
     // per point) with a column-major light view-projection matrix; vis gets 0/1 per point.
@@ -222,14 +234,23 @@ public:
                                           stream),
               "cpvs_container_evaluate_pcf");
     }
-    void freeOnCPU() { check(cpvs_container_free_on_cpu(c_), "cpvs_container_free_on_cpu"); }
+    void freeOnCPU() {
+        check(cpvs_container_free_on_cpu(c_), "cpvs_container_free_on_cpu");
+        for (auto& v : views_) v.reset();
+    }
     void copyToGPU() { check(cpvs_container_copy_to_gpu(c_), "cpvs_container_copy_to_gpu"); }
-    void moveToGPU() { check(cpvs_container_move_to_gpu(c_), "cpvs_container_move_to_gpu"); }
+    void moveToGPU() {
+        check(cpvs_container_move_to_gpu(c_), "cpvs_container_move_to_gpu");
+        for (auto& v : views_) v.reset();
+    }
     void setFilterSize(uint size) { check(cpvs_container_set_filter_size(c_, size), "cpvs_container_set_filter_size"); }
     cpvs_container* handle() const { return c_; }
 
 private:
+    size_t slot(uint x, uint y, uint z) const { return ((size_t)z * length_ + y) * length_ + x; }
     cpvs_container* c_ = nullptr;
+    uint length_ = 1;
+    std::vector<std::unique_ptr<CompressedShadow>> views_;  // non-owning, one per slot
 };
 
 }  // namespace cpvs_amd

@@ -112,6 +112,14 @@ int main(int argc, char** argv) {
         std::unique_ptr<CompressedShadow> copy = CompressedShadow::create(pyr("depths32x32"));
         CompressedShadowContainer c(std::move(copy));
         c.setFilterSize(3);
+        // get(x, y, z) is a CompressedShadow of the container's (src/CompressedShadowContainer.h:41-46).
+        const CompressedShadow* in = c.get(0, 0, 0);
+        ++cases;
+        if (!in || in->getDAG() != d.getDAG() || in->getNumLevels() != d.getNumLevels() ||
+            in->getTotalVisibility() != d.getTotalVisibility()) {
+            std::printf("FAIL container get\n");
+            ++fails;
+        }
         c.moveToGPU();
         ++cases;
         if (c.get(0, 0, 0) != nullptr || words == 0) {
