@@ -148,7 +148,7 @@ def test_traverse_mode_must_match_dag(ctx, ref_fixtures):
 
 
 @pytest.mark.parametrize("N", [16, 32, 64, 128, 256, 512, 1024])
-@pytest.mark.parametrize("T", [1, 2, 4])
+@pytest.mark.parametrize("T", [1, 2, 4, 8])
 def test_generator_maps_bit_exact(ctx, N, T):
     D = O.gen_depth(N, SEED0 + 7 * N + T, max(1, 32 * N * N // (1 << 20)), N >= 128)
     mm = cp.MinMaxHierarchy(D)
