@@ -129,7 +129,7 @@ def _maps():
     return out
 
 
-@pytest.mark.parametrize("T", [1, 2, 4])
+@pytest.mark.parametrize("T", [1, 2, 4, 8])
 def test_structural_builder_agrees(T):
     for N, D in _maps():
         p = O.Pyramid(D)
@@ -140,7 +140,7 @@ def test_structural_builder_agrees(T):
             assert np.array_equal(w, ws), (N, T, t)
 
 
-@pytest.mark.parametrize("T", [1, 2, 4])
+@pytest.mark.parametrize("T", [1, 2, 4, 8])
 def test_faithful_and_hashed_merge_agree(T):
     for N in (64, 256, 1024):
         D = O.gen_depth(N, SEED0 + 11 * N, max(1, 32 * N * N // (1024 * 1024)), N >= 256)
