@@ -45,6 +45,8 @@ _SIGS = {
     "cpvs_pyramid_destroy": [vp],
     "cpvs_estimate_build": [vp, vp, u32, u32, vp, C.POINTER(u64), C.POINTER(u64)],
     "cpvs_build": [vp, vp, u32, u32, vp, PP],
+    "cpvs_build_noalloc": [vp, vp, u32, u32, vp, vp, u64, C.POINTER(u64), C.POINTER(u32)],
+    "cpvs_build_noalloc_scratch_bytes": [],
     "cpvs_dag_info": [vp, C.POINTER(u32), C.POINTER(u64), C.POINTER(i32)],
     "cpvs_dag_level_stats": [vp, vp, vp, i32],
     "cpvs_dag_copy_words": [vp, vp, u64],
@@ -127,6 +129,7 @@ def load():
         fn = getattr(lib, name)
         fn.restype = C.c_int
         fn.argtypes = args
+    lib.cpvs_build_noalloc_scratch_bytes.restype = C.c_size_t
     _lib = lib
     return lib
 

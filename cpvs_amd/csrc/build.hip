@@ -1313,8 +1313,10 @@ u32 leaf_table_slots(u32 c, bool small) { return table_slots((small && c >= kSma
 // nodes distinct), which keeps the table's clear and its random accesses small; a table found too
 // small (maps of mostly distinct leaves, such as white noise) is reported as kRetryFullTable and
 // the build runs again.
+__global__ void k_store_word(u32* dst, u32 v) { *dst = v; }
+
 int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, int merge_mode,
-               cpvs_dag** out, const Admit* admit, bool small_leaf) {
+               cpvs_dag** out, const Admit* admit, bool small_leaf, DagOut* into) {
     const int n = p->n;
     const int R = n - 2;
     const bool leaf = n >= 5;
@@ -1341,6 +1343,18 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         // No partial child of the root (a z-tile above or below every surface, a flat map): the
         // DAG is the root's childmask alone, written without any further kernel.
         const u32 word = pinned_partials(pinned)[lay.block_start[lay.nlev]];
+        if (into) {  // a caller's buffer: the word is stored by a kernel, nothing is allocated
+            if (into->cap < 1) {
+                set_error("build: the DAG has 1 word, the output holds %llu", (unsigned long long)into->cap);
+                return CPVS_EINVAL;
+            }
+            CPVS_LAUNCH("store_word", k_store_word, dim3(1), dim3(1), 0, s, into->words, word);
+            CPVS_HIP_CHECK(hipGetLastError());
+            CPVS_HIP_CHECK(hipStreamSynchronize(s));
+            into->nwords = 1;
+            into->root_mask = word;
+            return CPVS_OK;
+        }
         if (admit) {
             rc = admit->fn(admit->user, A, 0, sizeof(u32));
             if (rc) return rc;
@@ -1646,7 +1660,12 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         return CPVS_EINVAL;
     }
 
-    u32* d_words = static_cast<u32*>(alloc_words(ctx, admit, total ? total : 1));
+    if (into && total > into->cap) {
+        set_error("build: the DAG has %llu words, the output holds %llu", (unsigned long long)total,
+                  (unsigned long long)into->cap);
+        return CPVS_EINVAL;
+    }
+    u32* d_words = into ? into->words : static_cast<u32*>(alloc_words(ctx, admit, total ? total : 1));
     if (!d_words) {
         set_error("build: hipMalloc of %llu DAG words failed", (unsigned long long)total);
         return CPVS_ENOMEM;
@@ -1676,8 +1695,13 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
-        ctx->buffers.give(d_words);
+        if (!into) ctx->buffers.give(d_words);
         return hip_fail(e, "build write", __FILE__, __LINE__);
+    }
+    if (into) {
+        into->nwords = total;
+        into->root_mask = hm[kMetaRoot];
+        return CPVS_OK;
     }
 
     cpvs_dag* d = new cpvs_dag();
@@ -1702,10 +1726,10 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
 }  // namespace
 
 int build_dag(cpvs_ctx* ctx, Arena& A, u64* pinned, int merge_mode, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
-              cpvs_dag** out, const Admit* admit) {
-    int rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out, admit, true);
-    if (rc == kRetryFullTable) rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out, admit, false);
-    if (rc == kRetryWithSort) rc = build_once(ctx, A, pinned, p, t, T, s, CPVS_MERGE_SORT, out, admit, false);
+              cpvs_dag** out, const Admit* admit, DagOut* into) {
+    int rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out, admit, true, into);
+    if (rc == kRetryFullTable) rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out, admit, false, into);
+    if (rc == kRetryWithSort) rc = build_once(ctx, A, pinned, p, t, T, s, CPVS_MERGE_SORT, out, admit, false, into);
     return rc;
 }
 

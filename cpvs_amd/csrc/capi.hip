@@ -63,6 +63,10 @@ int Arena::reserve(size_t bytes, hipStream_t s, bool* moved) {
     used = 0;
     if (moved) *moved = false;
     if (bytes <= cap) return CPVS_OK;
+    if (fixed) {
+        set_error("build: %zu bytes of scratch needed, the fixed scratch holds %zu", bytes, cap);
+        return CPVS_EINVAL;
+    }
     if (moved) *moved = true;
     if (base) {
         CPVS_HIP_CHECK(hipStreamSynchronize(s));
@@ -114,6 +118,13 @@ int cpvs_ctx_create(int device, cpvs_ctx** out) {
     if (e != hipSuccess) {
         delete c;
         return hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    }
+    // The no-allocation build's fixed scratch on this device (its address, found once per device).
+    int rc0 = noalloc_setup(device);
+    if (rc0 != CPVS_OK) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return rc0;
     }
     c->pinned_words = kPinnedWords;
     e = hipHostMalloc(reinterpret_cast<void**>(&c->pinned), kPinnedBytes, kPinnedFlags);

@@ -24,6 +24,7 @@ struct Arena {
     char* base = nullptr;
     size_t cap = 0;
     size_t used = 0;
+    bool fixed = false;  // a caller's buffer of `cap` bytes: never grown (reserve fails beyond it)
     // Ensure capacity and reset `used`; *moved (optional) tells whether the buffer was reallocated
     // (its contents are then lost).
     int reserve(size_t bytes, hipStream_t s, bool* moved = nullptr);
@@ -193,10 +194,22 @@ void dag_destroy_idle(cpvs_dag* d);
 // Bytes of the stored pyramid levels of an N x N map (levels >= 3, or all with store_all).
 size_t pyramid_levels_bytes(u32 N, bool store_all);
 
+// A build's words written into a caller's device buffer instead of a new cpvs_dag.
+struct DagOut {
+    u32* words = nullptr;  // device, cap words
+    u64 cap = 0;
+    u64 nwords = 0;        // out: the DAG's word count
+    u32 root_mask = 0;     // out
+};
+
 // build.hip: one DAG from a pyramid on stream s, scratch from A, readbacks staged in pinned
-// (kPinnedWords words of pinned host memory).
+// (kPinnedWords words of pinned host memory).  With `into`, the words go to into->words (an
+// error if they exceed into->cap) and *out is not touched.
 int build_dag(cpvs_ctx* ctx, Arena& A, u64* pinned, int merge_mode, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
-              cpvs_dag** out, const Admit* admit = nullptr);
+              cpvs_dag** out, const Admit* admit = nullptr, DagOut* into = nullptr);
+
+// noalloc.hip: finds this device's fixed scratch (once per device); called by cpvs_ctx_create.
+int noalloc_setup(int device);
 
 // lookup.hip
 struct Mat4 {

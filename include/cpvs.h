@@ -97,6 +97,20 @@ int cpvs_estimate_build(cpvs_ctx *ctx, const cpvs_pyramid *p, uint32_t zTileInde
  * The DAG's words stay in device memory.  Returns after the build completed on the stream. */
 int cpvs_build(cpvs_ctx *ctx, const cpvs_pyramid *p, uint32_t zTileIndex, uint32_t zTileNum, void *stream,
                cpvs_dag **out);
+
+/* No counterpart in the reference: cpvs_build for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc or hipFree: its scratch is a fixed device array of cpvs_build_noalloc_scratch_bytes()
+ * bytes per device (64 MiB, held from context creation until the process ends; enough for maps up
+ * to 1024^2 and most of 2048^2), and the DAG's words go to d_words (device, cap_words words).
+ * *num_words gets the word count, *root_mask (may be NULL) the root's childmask.  A map whose
+ * scratch or words exceed those sizes fails with CPVS_EINVAL and a message naming both sizes.
+ * The words equal cpvs_build's.  Calls on one device share its array: calls through one context
+ * are serialised, calls through several contexts on one device race.  Returns after the build
+ * completed on the stream. */
+int cpvs_build_noalloc(cpvs_ctx *ctx, const cpvs_pyramid *p, uint32_t zTileIndex, uint32_t zTileNum, void *stream,
+                       uint32_t *d_words, uint64_t cap_words, uint64_t *num_words, uint32_t *root_mask);
+size_t cpvs_build_noalloc_scratch_bytes(void);
 /* CompressedShadow::getNumLevels / getDAG().size() / getTotalVisibility
  * (src/CompressedShadow.h:69-77, src/CompressedShadow.cpp:66-72). */
 int cpvs_dag_info(const cpvs_dag *d, uint32_t *num_levels, uint64_t *num_words, int *total_visibility);
