@@ -51,12 +51,68 @@ def table_slots(n):
 
 
 def leaf_waves_bytes(N, nodes_leaf):
-    """Compulsory traffic of k_leaf_waves: the 8x8 depth block of every level-3 column (256 B), per
-    leaf node its breadth-first index read and its run-head index and leafmask slot written
-    (12 B), and the clear of the leaf level's merge table (8 B per slot; the table is sized for a
-    quarter of the leaf nodes), which this kernel does.  The first node of each run also writes
-    mask, hash and leafmasks; those bytes are left out, so the figure is a lower bound."""
-    return 256 * (N // 8) ** 2 + 12 * nodes_leaf + 8 * table_slots(nodes_leaf // 4)
+    """Algorithmic bytes of k_leaf_waves: the 8x8 depth block of every level-3 column (256 B) and,
+    per leaf node, its breadth-first index read and its run-head index and leafmask slot written
+    (12 B).  The first node of each run also writes mask, hash and leafmasks; those bytes are left
+    out, so the figure is a lower bound.  The kernel also clears the leaf level's merge table
+    (leaf_table_clear_bytes), which is not part of the leaf computation."""
+    return 256 * (N // 8) ** 2 + 12 * nodes_leaf
+
+
+def leaf_table_clear_bytes(nodes_leaf):
+    """The leaf level's merge table (8 B per slot, sized for a quarter of the leaf nodes), cleared
+    at the end of k_leaf_waves."""
+    return 8 * table_slots(nodes_leaf // 4)
+
+
+def host_cpu():
+    """The host's CPU: logical CPUs, those this process may run on, and the model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count(), "usable_cpus": aff, "model": model}
+
+
+FAITHFUL_CHILD = r"""
+import sys, time
+sys.path.insert(0, {root!r})
+import oracle as O
+D = O.gen_depth({N}, {seed}, {boxes}, {noise})
+p = O.Pyramid(D)
+print("start", flush=True)
+t0 = time.perf_counter()
+w, n = O.create(p, 0, 1, merge_mode=0)
+print("done", time.perf_counter() - t0, len(w), flush=True)
+"""
+
+
+def time_faithful_create(N, seed, boxes, noise, cap_s):
+    """The reference's own create(): its linear-search mergeLevel (O(n*u),
+    src/CompressedShadowUtil.h:154-182), in the oracle, in a child process stopped after cap_s
+    seconds of create().  Returns (seconds or None when capped, words or None)."""
+    import subprocess
+    code = FAITHFUL_CHILD.format(root=ROOT, N=N, seed=seed, boxes=boxes, noise=noise)
+    proc = subprocess.Popen([sys.executable, "-c", code], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+    try:
+        first = proc.stdout.readline()  # map generated, pyramid built: create() starts now
+        if not first.startswith("start"):
+            proc.kill()
+            proc.wait()
+            return None, None
+        out, _ = proc.communicate(timeout=cap_s)
+        parts = out.split()
+        return float(parts[1]), int(parts[2])
+    except subprocess.TimeoutExpired:
+        proc.kill()
+        proc.wait()
+        return None, None
 
 
 def pyramid_root_bytes(N):
@@ -117,13 +173,27 @@ def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26, batch
             mm.close()
     ctx.synchronize()
     build_s = time.perf_counter() - t0 - sample_s
-    # The same 512 DAGs as one batch: each lane builds a tile's pyramid once and its z-tiles from
-    # it.  The words of every 37th plain DAG are compared with the batched one's.
+    # The same 512 DAGs as one batch (a tile's z-tiles share its pyramid), in the configuration the
+    # autotuner picks among forests and one-map-per-chain lanes.  The words of every 37th plain DAG
+    # are compared with the batched one's.
     keys = [(tx, ty, tz) for ty in range(L) for tx in range(L) for tz in range(L)]
-    t0 = time.perf_counter()
-    batch, _ = cp.build_batch([tiles[(tx, ty)] for tx, ty, _ in keys], [(tz, L) for _, _, tz in keys], lanes=batch_lanes)
-    ctx.synchronize()
-    batch_s = time.perf_counter() - t0
+    bd, bt = [tiles[(tx, ty)] for tx, ty, _ in keys], [(tz, L) for _, _, tz in keys]
+    cands = [cp.batch_config(lanes=batch_lanes, forest_mtexels=0)] + \
+        [cp.batch_config(lanes=ln, forest_mtexels=mt) for ln in (1, 2, 4) for mt in (64, 256)] + \
+        [cp.batch_config(lanes=2, forest_mtexels=64, leaf_cols=4), cp.batch_config(lanes=2, forest_mtexels=64, graph=True)]
+    best, tune_ms = cp.tune_batch_config(bd, cands, bt)
+    cfg = cands[best]
+    log(f"[bench] tiled: tuner picked {cfg.as_dict()} ({[round(m, 1) for m in tune_ms]} ms)")
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        batch, _ = cp.build_batch(bd, bt, config=cfg)
+        ctx.synchronize()
+        runs.append(time.perf_counter() - t0)
+        if len(runs) < 3:
+            for sh in batch:
+                sh.close()
+    batch_s = sorted(runs)[1]
     same = all(np.array_equal(batch[keys.index(k)].getDAG(), w) for k, w in sample.items())
     for sh in batch:
         sh.close()
@@ -149,9 +219,45 @@ def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26, batch
             "dags": L ** 3, "dag_words": words, "grid_levels": gl,
             "build_s": round(build_s, 4), "Mtexels_per_s": round(full * full / build_s / 1e6, 1),
             "batched_build_s": round(batch_s, 4), "batched_Mtexels_per_s": round(full * full / batch_s / 1e6, 1),
-            "batch_lanes": batch_lanes, "batched_words_identical": bool(same),
+            "batch_config": cfg.as_dict(),
+            "tune_ms": {json.dumps(c.as_dict(), sort_keys=True): round(m, 2) for c, m in zip(cands, tune_ms)},
+            "batched_words_identical": bool(same),
             "dag_build_ms_p50": round(pct(per_dag, 50), 3), "dag_build_ms_p99": round(pct(per_dag, 99), 3),
             "lookup_G_per_s": round(nq / (min(lk) * 1e-3) / 1e9, 3)}
+
+
+def run_c0(cp, np, ctx, N=1024, boxes=32, nq=1 << 20, reps=20):
+    """The 1024^2 configuration: plane + 32 boxes (no noise), 1,048,576 uniform lookups."""
+    stream = ctx.stream
+    d = cp.gen_depth(N, SEED0, boxes, False)
+    q = cp.gen_queries_uniform(SEED0, nq, 4)
+    vis = cp.DeviceArray((nq,), np.uint8, ctx)
+    bms, lms = [], []
+    for it in range(reps + 3):
+        e0 = ctx.event(stream)
+        mm = cp.MinMaxHierarchy(d, borrow=True)
+        cs = cp.CompressedShadow.create(mm)
+        e1 = ctx.event(stream)
+        c = cp.CompressedShadowContainer(cs)
+        c.copyToGPU()
+        e2 = ctx.event(stream)
+        c.evaluate(q, None, vis)
+        e3 = ctx.event(stream)
+        if it >= 3:
+            bms.append(ctx.elapsed_ms(e0, e1))
+            lms.append(ctx.elapsed_ms(e2, e3))
+        for e in (e0, e1, e2, e3):
+            ctx.destroy_event(e)
+        words = cs.num_words()
+        c.close()
+        mm.close()
+    d.free()
+    q.free()
+    vis.free()
+    return {"workload": f"{N}x{N} depth (plane + {boxes} boxes), {nq} uniform lookups",
+            "build_ms_p50": round(pct(bms, 50), 4), "build_ms_p99": round(pct(bms, 99), 4),
+            "Mtexels_per_s": round(N * N / (pct(bms, 50) * 1e-3) / 1e6, 1), "dag_words": words,
+            "lookup_G_per_s": round(nq / (pct(lms, 50) * 1e-3) / 1e9, 3)}
 
 
 def splitmix64(seed):
@@ -202,11 +308,20 @@ def run_batched(cp, np, ctx, maps=256, kmin=8, kmax=13, qper=1 << 20, reps=3):
         out, done = cp.build_batch(depths)
         return out, list(done), time.perf_counter() - t0
 
-    # warm-up both paths (arena sizing, lane creation), then the tuner picks the lane count
+    # warm-up of the plain path (arena sizing), then the autotuner picks the batch configuration:
+    # forests (many maps through one launch chain) of several sizes on 1-4 lanes, leaf kernel
+    # shapes, hipGraph capture, and one map per launch chain on 4 or 8 lanes.
     for sh in plain_once()[0]:
         sh.close()
-    best, tune_ms = cp.tune_batch(depths, candidates=(1, 2, 3, 4, 6, 8))
-    log(f"[bench] batched: tuner picked {best} lanes ({tune_ms})")
+    cands = [cp.batch_config(lanes=ln, forest_mtexels=0) for ln in (4, 8)] + \
+        [cp.batch_config(lanes=ln, forest_mtexels=mt) for ln in (1, 2, 4) for mt in (64, 256, 1024)] + \
+        [cp.batch_config(lanes=2, forest_mtexels=256, leaf_cols=4), cp.batch_config(lanes=2, forest_mtexels=256, leaf_waves=2),
+         cp.batch_config(lanes=2, forest_mtexels=256, graph=True)]
+    bi, tms = cp.tune_batch_config(depths, cands)
+    best = cands[bi].as_dict()
+    tune_ms = {json.dumps(c.as_dict(), sort_keys=True): m for c, m in zip(cands, tms)}
+    lane_ms = min(m for c, m in zip(cands, tms) if c.forest_mtexels == 0)
+    log(f"[bench] batched: tuner picked {best} ({[round(m, 1) for m in tms]} ms)")
     plain_runs, batch_runs = [], []
     plain, batch = None, None
     for r in range(reps):
@@ -282,8 +397,9 @@ def run_batched(cp, np, ctx, maps=256, kmin=8, kmax=13, qper=1 << 20, reps=3):
     rate = {k: nq / (min(v) * 1e-3) / 1e9 for k, v in lk.items()}
     return {"workload": f"{maps} maps of 2^k x 2^k, k uniform in [{kmin},{kmax}] (boxes 32*(N/1024)^2 + noise), "
                         f"{nq} uniform lookups ({qper} per map, interleaved round-robin)",
-            "maps": maps, "texels": int(texels), "lanes": best,
-            "tune_ms": {str(k): round(v, 2) for k, v in (tune_ms or {}).items()},
+            "maps": maps, "texels": int(texels), "config": best,
+            "tune_ms": {k: round(v, 2) for k, v in tune_ms.items()},
+            "lane_mode_best_ms": round(lane_ms, 2),
             "plain_build_ms": round(p_sec * 1e3, 2), "batched_build_ms": round(b_sec * 1e3, 2),
             "plain_Mtexels_per_s": round(texels / p_sec / 1e6, 1),
             "batched_Mtexels_per_s": round(texels / b_sec / 1e6, 1),
@@ -299,7 +415,7 @@ def run_batched(cp, np, ctx, maps=256, kmin=8, kmax=13, qper=1 << 20, reps=3):
 
 
 def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_size=4, load=0.7,
-                budget=200 << 30, lanes=4):
+                budget=200 << 30, lanes=4, pressure_budget=32 << 30):
     """Serving: a trace of nreq requests, each a build (probability p_build) of a 2^k map, k uniform
     in [kmin, kmax], or a lookup of 2^U(10,22) uniform points against one of a warm set of
     resident DAGs (warm_per_size per size, built before timing).  Depth maps (one per size) and
@@ -326,7 +442,7 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
     vis = cp.DeviceArray((out_bytes,), np.uint8, ctx)
     ctx.synchronize()
 
-    def serve(serial, arrivals):
+    def serve(serial, arrivals, budget=budget):
         srv = cp.Server(lanes=lanes, budget_bytes=budget, serial=serial, ctx=ctx)
         try:
             warm = []
@@ -337,6 +453,7 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
                 srv.wait(t)
             tickets, kinds, lag = [], [], []
             off = 0
+            missed = 0
             t0 = time.perf_counter()
             for (kind, a, n, qo), at in zip(trace, arrivals):
                 if at is not None:
@@ -350,20 +467,32 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
                 if kind == 0:
                     tickets.append(srv.build(depth[a]))
                 else:
-                    tickets.append(srv.lookup(warm[a], q, vis, count=n, offset=qo, vis_offset=off))
+                    try:
+                        tickets.append(srv.lookup(warm[a], q, vis, count=n, offset=qo, vis_offset=off))
+                    except cp.CpvsError:  # its DAG was evicted under a tight budget: not served
+                        missed += 1
+                        continue
                     off += n
                 kinds.append(kind)
-            lat = [srv.wait(t) for t in tickets]
+            lat, failed = [], 0
+            for t in tickets:
+                try:
+                    lat.append(srv.wait(t))
+                except cp.CpvsError:  # a build the budget could not admit
+                    lat.append(None)
+                    failed += 1
+            lat_kinds = [(x, k) for x, k in zip(lat, kinds) if x is not None]
             sec = time.perf_counter() - t0
             st = srv.stats()
         finally:
             srv.close()
-        b = [x for x, k in zip(lat, kinds) if k == 0]
-        l = [x for x, k in zip(lat, kinds) if k == 1]
+        b = [x for x, k in lat_kinds if k == 0]
+        l = [x for x, k in lat_kinds if k == 1]
         return {"s": sec, "req_per_s": len(trace) / sec,
                 "build_ms_p50": pct(b, 50), "build_ms_p99": pct(b, 99),
                 "lookup_ms_p50": pct(l, 50), "lookup_ms_p99": pct(l, 99),
                 "lookup_batches": st["batches"], "lookups": st["lookups"], "evicted": st["evicted"],
+                "trims": st["trims"], "lookups_on_evicted": missed, "builds": st["builds"], "failed": failed,
                 "submit_lag_ms_p99": pct(lag, 99) * 1e3 if lag else None}
 
     none = [None] * len(trace)
@@ -376,6 +505,10 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
         arr.append(t)
     lat = {m: serve(m == "serial", arr) for m in ("serial", "served")}
     log(f"[bench] serving latency at {rate:.0f} req/s: {lat}")
+    # The same Poisson trace under a budget that forces evictions: the resident DAGs of the builds
+    # outgrow it, so least recently used DAGs are evicted (no device synchronisation on that path).
+    press = {m: serve(m == "serial", arr, pressure_budget) for m in ("serial", "served")}
+    log(f"[bench] serving under a {pressure_budget >> 30} GiB budget: {press}")
     vis.free()
     q.free()
     for d in depth.values():
@@ -394,7 +527,15 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
                            for m, v in lat.items()},
             "lookup_launches": {m: v["lookup_batches"] for m, v in lat.items()},
             "evicted": {m: v["evicted"] for m, v in lat.items()},
-            "submit_lag_ms_p99": {m: r(v["submit_lag_ms_p99"]) for m, v in lat.items()}}
+            "submit_lag_ms_p99": {m: r(v["submit_lag_ms_p99"]) for m, v in lat.items()},
+            "pressure": {"budget_GiB": pressure_budget >> 30,
+                         "latency_ms": {m: {k: r(v[k]) for k in ("build_ms_p50", "build_ms_p99", "lookup_ms_p50",
+                                                                  "lookup_ms_p99")} for m, v in press.items()},
+                         "req_per_s": {m: round(v["req_per_s"], 1) for m, v in press.items()},
+                         "evicted": {m: v["evicted"] for m, v in press.items()},
+                         "cache_trims": {m: v["trims"] for m, v in press.items()},
+                         "lookups_on_evicted_dags": {m: v["lookups_on_evicted"] for m, v in press.items()},
+                         "requests_failed": {m: v["failed"] for m, v in press.items()}}}
 
 
 def main():
@@ -410,6 +551,10 @@ def main():
     ap.add_argument("--no-tiled", action="store_true", help="skip the 65536^2 tiled configuration")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-map batched configuration")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving configuration")
+    ap.add_argument("--no-check", action="store_true", help="skip the brute-force check of the headline lookups")
+    ap.add_argument("--no-c0", action="store_true", help="skip the 1024^2 configuration")
+    ap.add_argument("--faithful-cap", type=float, default=20.0,
+                    help="seconds the faithful (linear-search) create() may run at the headline map")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -452,6 +597,7 @@ def main():
 
     build_ms, lookup_ms, step_ms = [], [], []
     words = None
+    dag_words_host = None
     levels = None
     for it in range(args.warmup + args.steps):
         t0 = time.perf_counter()
@@ -462,6 +608,7 @@ def main():
         if words is None:
             words = cs.num_words()
             levels = cs.level_stats()
+            dag_words_host = cs.getDAG() if not args.no_cpu_baseline else None
         c = cp.CompressedShadowContainer(cs)
         c.copyToGPU()
         e2 = ctx.event(stream)
@@ -547,6 +694,11 @@ def main():
 
     rname = dominant[0] if dominant[0] in algo else "leaf_waves"
     roofline = roof(rname)
+    if rname == "leaf_waves":
+        # The HBM roofline is the ceiling; what holds the kernel below it is instruction issue.
+        roofline["limiter"] = ("instruction issue: VALU and SALU (the run heads' key hash), "
+                               "not HBM bandwidth")
+        roofline["table_clear_bytes"] = leaf_table_clear_bytes(int(nodes[2]))
     pyramid_roofline = roof("pyr_tile_root")
     lookup_roofline = roof("evaluate4")
     # Dependent gathers from a cache-resident DAG: set by load throughput and L2 locality, not HBM.
@@ -567,17 +719,44 @@ def main():
         serving = run_serving(cp, np, ctx)
         log(f"[bench] serving: {serving}")
 
+    # Every one of the headline lookups against the brute-force depth compare (outside the timed
+    # region): the result of the last timed evaluate().
+    check = None
+    if not args.no_check:
+        import oracle as O
+        D = depth.numpy()
+        qh = queries.numpy()
+        vh = vis.numpy()
+        R = np.float32((1 << (len(levels[0]) - 1)) - 1)
+        same = True
+        for lo in range(0, nq, 1 << 22):
+            hi = min(lo + (1 << 22), nq)
+            p = (((qh[lo:hi, :3] + np.float32(1)) * np.float32(0.5)) * R).astype(np.int64)
+            same = same and bool(np.array_equal(vh[lo:hi], O.brute_visible(D, 1, p[:, 0], p[:, 1], p[:, 2])))
+        check = {"lookups_equal_brute_force": same, "queries_checked": nq, "lit_fraction": round(float(vh.mean()), 4)}
+        del qh, vh
+        log(f"[bench] headline lookups vs brute force: {check}")
+
+    c0 = None
+    if not args.no_c0:
+        c0 = run_c0(cp, np, ctx)
+        log(f"[bench] 1024^2: {c0}")
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle as O
-        log("[bench] CPU baseline: oracle create() (hash merge, identical words) on the same map")
+        hc = host_cpu()
+        log(f"[bench] CPU baseline on {hc}: oracle create() (hash merge, identical words) on the same map")
         D = depth.numpy()
         t0 = time.perf_counter()
         pyr = O.Pyramid(D)
         w, _ = O.create(pyr, 0, 1, merge_mode=1)
         t1 = time.perf_counter()
+        words_equal = None
+        if dag_words_host is not None:
+            words_equal = bool(np.array_equal(w, dag_words_host))
         assert len(w) == words, (len(w), words)
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = max(1, min(16, hc["usable_cpus"]))  # the box's CPU share for one GPU is 16
         sample = nq
         qh = queries.numpy()
         grid = O.top_level_grid([w])
@@ -587,9 +766,25 @@ def main():
         t4 = time.perf_counter()
         O.evaluate(w, grid, len(nodes), 0, qh[:2_000_000], nthreads=1)
         t5 = time.perf_counter()
+        del qh
+        # The reference's create() with its own linear-search merge: timed at 1024^2 (plane + 32
+        # boxes), and stopped after a cap at the headline map, where it needs hours.
+        log("[bench] CPU baseline: faithful create() (linear-search mergeLevel) at 1024^2 and, capped, at the headline map")
+        f0_s, f0_words = time_faithful_create(1024, SEED0, 32, False, 300)
+        f1_s, _ = time_faithful_create(N, SEED0 + 1, args.boxes, True, args.faithful_cap)
         cpu = {"value": round(texels / (t1 - t0) / 1e6, 3), "unit": "Mtexels/s", "cores": 4, "kind": "port",
+               "cores_note": "MinMaxHierarchy on 4 threads (src/MinMaxHierarchy.cpp:56-66), create() on 1",
+               "host": hc,
                "sample": f"one full {N}^2 build: MinMaxHierarchy on 4 threads + create() single-threaded with the "
                          f"hash-table mergeLevel (word-identical to the linear search), {t1 - t0:.2f} s",
+               "words_equal_gpu": words_equal,
+               "faithful_create": {
+                   "1024^2_plane_32_boxes": {"s": None if f0_s is None else round(f0_s, 3), "words": f0_words,
+                                             "Mtexels_per_s": None if not f0_s else round(1024 * 1024 / f0_s / 1e6, 3)},
+                   f"{N}^2": {"s": f">{args.faithful_cap:.0f}" if f1_s is None else round(f1_s, 2),
+                              "cap_s": args.faithful_cap},
+                   "note": "the reference's linear-search mergeLevel (src/CompressedShadowUtil.h:154-182), "
+                           "1 thread, create() only (pyramid excluded)"},
                "lookup_G_per_s": round(sample / (t3 - t2) / 1e9, 4), "lookup_threads": threads,
                "lookup_G_per_s_1thread": round(2_000_000 / (t5 - t4) / 1e9, 4),
                "lookup_sample": f"traverse.cs transcription over all {sample} queries on {threads} threads "
@@ -612,6 +807,8 @@ def main():
                                f"{nq} G-buffer lookups ({args.width}x{args.height} x 2)",
                    "N": N, "zTileNum": 1, "queries": nq, "dag_words": words},
         "host_input": host_input,
+        "headline_check": check,
+        "c0_1024": c0,
         "build_ms_mean": round(statistics.mean(build_ms), 3),
         "build_ms_p50": round(pct(build_ms, 50), 3),
         "build_ms_p99": round(pct(build_ms, 99), 3),

@@ -573,19 +573,57 @@ def _batch_items(depths: Sequence[DeviceArray], tiles=None):
     return items
 
 
-def build_batch(depths: Sequence[DeviceArray], tiles=None, lanes: int = 0, ctx: Optional[Context] = None):
+def batch_config(lanes: int = 4, forest_mtexels: int = 256, leaf_cols: int = 8, leaf_waves: int = 1,
+                 graph: bool = False) -> "_lib.BatchConfig":
+    """A cpvs_batch_config (forest_mtexels = 0: one map per launch chain on `lanes` streams)."""
+    return _lib.BatchConfig(lanes, forest_mtexels, leaf_cols, leaf_waves, int(graph))
+
+
+def build_batch(depths: Sequence[DeviceArray], tiles=None, lanes: int = 0, ctx: Optional[Context] = None,
+                config=None):
     """MinMaxHierarchy + CompressedShadow::create for every depth map at once (cpvs_build_batch).
 
     depths: square float32 DeviceArrays; tiles: optional (zTileIndex, zTileNum) per map; lanes: 0 =
-    the context's tuned count.  Returns (shadows, done_ms) with done_ms[i] the completion time of
-    map i in ms since the call began."""
+    the context's tuned count; config: a batch_config() (the context's when None).  Returns
+    (shadows, done_ms) with done_ms[i] the completion time of map i in ms since the call began."""
     ctx = ctx or default_context()
     n = len(depths)
     items = _batch_items(depths, tiles)
     out = (C.c_void_p * max(n, 1))()
     done = np.zeros(max(n, 1), np.float64)
-    call("cpvs_build_batch", ctx.handle, items, n, lanes, out, done.ctypes.data_as(C.c_void_p))
+    if config is not None:
+        if lanes:
+            config = _lib.BatchConfig(lanes, config.forest_mtexels, config.leaf_cols, config.leaf_waves, config.graph)
+        call("cpvs_build_batch_config", ctx.handle, items, n, C.byref(config), out, done.ctypes.data_as(C.c_void_p))
+    else:
+        call("cpvs_build_batch", ctx.handle, items, n, lanes, out, done.ctypes.data_as(C.c_void_p))
     return [CompressedShadow(C.c_void_p(out[i]), ctx) for i in range(n)], done[:n]
+
+
+def get_batch_config(ctx: Optional[Context] = None) -> dict:
+    ctx = ctx or default_context()
+    c = _lib.BatchConfig()
+    call("cpvs_ctx_batch_config", ctx.handle, C.byref(c))
+    return c.as_dict()
+
+
+def set_batch_config(config, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    call("cpvs_ctx_set_batch_config", ctx.handle, C.byref(config))
+
+
+def tune_batch_config(depths: Sequence[DeviceArray], candidates, tiles=None, reps: int = 1,
+                      ctx: Optional[Context] = None):
+    """The autotuner (cpvs_tune_batch_config): times every candidate batch_config() on this batch,
+    makes the fastest the context's configuration.  Returns (best index, [ms per candidate])."""
+    ctx = ctx or default_context()
+    items = _batch_items(depths, tiles)
+    arr = (_lib.BatchConfig * len(candidates))(*candidates)
+    ms = np.zeros(len(candidates), np.float64)
+    best = C.c_uint32()
+    call("cpvs_tune_batch_config", ctx.handle, items, len(depths), arr, len(candidates), reps, C.byref(best),
+         ms.ctypes.data_as(C.c_void_p))
+    return int(best.value), [float(m) for m in ms]
 
 
 def _tune_key(ctx: Context, depths) -> str:

@@ -79,6 +79,10 @@ _SIGS = {
     "cpvs_gen_queries_gbuffer": [vp, vp, vp, u32, u64, u32, u32, vp],
     "cpvs_gen_queries_uniform_range": [vp, vp, u64, u64, u64, u64, u32, vp],
     "cpvs_build_batch": [vp, vp, u32, u32, vp, vp],
+    "cpvs_build_batch_config": [vp, vp, u32, vp, vp, vp],
+    "cpvs_ctx_set_batch_config": [vp, vp],
+    "cpvs_ctx_batch_config": [vp, vp],
+    "cpvs_tune_batch_config": [vp, vp, u32, vp, u32, u32, C.POINTER(u32), vp],
     "cpvs_ctx_set_batch_lanes": [vp, u32],
     "cpvs_ctx_batch_lanes": [vp, C.POINTER(u32)],
     "cpvs_ctx_device_name": [vp, C.c_char_p, u64],
@@ -110,6 +114,16 @@ class BatchItem(C.Structure):
     """cpvs_batch_item: one map of a batch build (device depth, N, zTileIndex, zTileNum)."""
     _fields_ = [("depth", C.c_void_p), ("N", C.c_uint32), ("zTileIndex", C.c_uint32), ("zTileNum", C.c_uint32),
                 ("reserved", C.c_uint32)]
+
+class BatchConfig(C.Structure):
+    """cpvs_batch_config: how a batch runs (lanes, forest size in Mtexels (0 = one map per launch
+    chain), leaf kernel columns per wave and waves per workgroup, hipGraph capture)."""
+    _fields_ = [("lanes", C.c_uint32), ("forest_mtexels", C.c_uint32), ("leaf_cols", C.c_uint32),
+                ("leaf_waves", C.c_uint32), ("graph", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k in ("lanes", "forest_mtexels", "leaf_cols", "leaf_waves", "graph")}
+
 
 _lib = None
 

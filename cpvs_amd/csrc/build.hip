@@ -29,56 +29,9 @@
 #include <cstdio>
 #include <cstring>
 
-#include "internal.hpp"
-#include "scan.hpp"
+#include "build_parts.hpp"
 
 namespace cpvs {
-
-constexpr u64 kHashSeed = 0x6A09E667F3BCC909ull;
-constexpr u32 kNoRep = 0xFFFFFFFFu;
-
-__device__ inline u32 fold32(u64 h) { return (u32)(h ^ (h >> 32)); }
-
-__device__ inline u32 fmix32(u32 h) {
-    h ^= h >> 16;
-    h *= 0x85EBCA6Bu;
-    h ^= h >> 13;
-    h *= 0xC2B2AE35u;
-    h ^= h >> 16;
-    return h;
-}
-
-// ---------------------------------------------------------------------------------------
-// 1. Inner classification: createChildmask (src/CompressedShadowUtil.cpp:20-54).
-// ---------------------------------------------------------------------------------------
-__device__ inline u32 classify_node(u64 c, const float2* __restrict__ lv, const float* __restrict__ depth, u32 N, int L,
-                                    u32 T) {
-    const u32 x = unpack_x(c), y = unpack_y(c), z = unpack_z(c);
-    const float h = (float)((N >> L) * T);
-    float mn[4], mx[4];  // texel (x + (q&1), y + (q>>1))
-    if (L > 0) {
-        const u32 w = N >> L;
-        const float4 a = *reinterpret_cast<const float4*>(lv + (size_t)y * w + x);
-        const float4 b = *reinterpret_cast<const float4*>(lv + (size_t)(y + 1) * w + x);
-        mn[0] = a.x; mx[0] = a.y; mn[1] = a.z; mx[1] = a.w;
-        mn[2] = b.x; mx[2] = b.y; mn[3] = b.z; mx[3] = b.w;
-    } else {
-        const float2 a = *reinterpret_cast<const float2*>(depth + (size_t)y * N + x);
-        const float2 b = *reinterpret_cast<const float2*>(depth + (size_t)(y + 1) * N + x);
-        mn[0] = mx[0] = a.x; mn[1] = mx[1] = a.y;
-        mn[2] = mx[2] = b.x; mn[3] = mx[3] = b.y;
-    }
-    u32 mask = 0;
-#pragma unroll
-    for (u32 ch = 0; ch < 8; ++ch) {
-        const u32 q = ch & 3, cz = z + (ch >> 2);
-        u32 bits;
-        if (L > 0) bits = classify_cell(cz, mn[q] * h, mx[q] * h);
-        else bits = classify_mid(cz, mn[q] * h);
-        mask |= bits << (2 * ch);
-    }
-    return mask;
-}
 
 __global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord, u32 count, const float2* __restrict__ lv,
                                                   const float* __restrict__ depth, u32 N, int L, u32 T,
@@ -87,121 +40,6 @@ __global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord,
     if (i >= count) return;
     mask_out[i] = classify_node(coord[i], lv, depth, N, L, T);
 }
-
-// Leaf columns: per level-3 texel one record {partial cells, first leaf's index in column order,
-// first partial z cell, 0}, so a column's scalars come in one load.  The counts (x, z) come from
-// the node-count pass, the starts (y) from a scan of the counts.
-struct LeafColumns {
-    uint4* info;
-    u32* bfs;       // (column start + z - zs) -> breadth-first index of the leaf node
-    u32 w3;         // N >> 3
-    u32 w3_shift;   // log2(w3)
-};
-
-struct ColCountIn {
-    const uint4* info;
-    __device__ u32 operator()(u32 i) const { return info[i].x; }
-};
-struct ColStartOut {
-    uint4* info;
-    __device__ void operator()(u32 i, u32 ex, u32) const { info[i].y = ex; }
-};
-// With several z-tiles most columns have no leaf node in a given tile.  One scan of
-// (column has nodes ? 1 : 0) << 32 | nodes gives the column starts and the list of the columns
-// with nodes, in column order (the halves cannot carry: node counts stay below 2^32).
-struct ColPairIn {
-    const uint4* info;
-    __device__ u64 operator()(u32 i) const {
-        const u32 c = info[i].x;
-        return (c ? (1ull << 32) : 0ull) | c;
-    }
-};
-struct ColListOut {
-    uint4* info;
-    u32* list;   // list[k] = the k-th column with nodes
-    u32* nlist;  // device: the number of such columns
-    u32 n;
-    __device__ void operator()(u32 i, u64 ex, u64 v) const {
-        info[i].y = (u32)ex;
-        if ((u32)v) list[(u32)(ex >> 32)] = i;
-        if (i == n - 1) *nlist = (u32)((ex + v) >> 32);
-    }
-};
-
-// Scan functors: partial-child count in, (first child index, children coordinates) out.
-struct ChildCountIn {
-    const u32* mask;
-    __device__ u32 operator()(u32 i) const { return num_children(mask[i]); }
-};
-// A level's classification inside its emit scan: the input classifies node i, and the output
-// (which classifies it again from the same cached loads) stores the mask.  One launch per level
-// fewer than a separate classify pass.
-struct Classify {
-    const u64* coord;
-    const float2* lv;
-    const float* depth;
-    u32 N;
-    int L;
-    u32 T;
-    __device__ u32 operator()(u32 i) const { return classify_node(coord[i], lv, depth, N, L, T); }
-};
-struct ClassifyCountIn {
-    Classify cls;
-    __device__ u32 operator()(u32 i) const { return num_children(cls(i)); }
-};
-struct EmitChildrenOut {
-    const u32* mask;
-    const u64* coord;
-    u32* cstart;
-    u64* child_coord;   // used when the child level is classified node by node
-    u32 child_cap;      // nodes of the child level (exact count); writes beyond it are dropped
-    LeafColumns cols;   // used when cols.bfs != null (the children are leaves)
-    u32* mask_out = nullptr;  // when set, the node is classified here (cls) and its mask stored
-    Classify cls = {};
-    // The node's mask and cell, loaded by the scan before any output is written.
-    struct Pre {
-        u32 m;
-        u64 c;
-    };
-    __device__ Pre load(u32 i) const { return Pre{mask_out ? cls(i) : mask[i], coord[i]}; }
-    __device__ void operator()(u32 i, u32 first, u32 k, Pre pre) const {
-        cstart[i] = first;
-        if (mask_out) mask_out[i] = pre.m;
-        if (!k) return;
-        const u32 m = pre.m;
-        const u64 c = pre.c;
-        const u32 x = unpack_x(c), y = unpack_y(c), z = unpack_z(c);
-        u32 j = first;
-        // getChildCoordinates (src/CompressedShadowUtil.cpp:101-117): partial children in
-        // ascending child index, child cell doubled.
-        if (cols.bfs) {
-            // Leaf children: children ch and ch + 4 lie in one column, so the node's (at most
-            // four) column records are loaded first, all together, and the stores follow.
-            uint4 ci[4];
-#pragma unroll
-            for (u32 q = 0; q < 4; ++q) {
-                ci[q] = make_uint4(0u, 0u, 0u, 0u);
-                if (m & ((2u << (2 * q)) | (2u << (2 * (q + 4))))) ci[q] = cols.info[(y + (q >> 1)) * cols.w3 + x + (q & 1)];
-            }
-#pragma unroll
-            for (u32 ch = 0; ch < 8; ++ch) {
-                if (!(m & (2u << (2 * ch)))) continue;
-                const uint4 r = ci[ch & 3];
-                const u32 slot = r.y + (z + (ch >> 2) - r.z);
-                if (j < child_cap && slot < child_cap) cols.bfs[slot] = j;
-                ++j;
-            }
-            return;
-        }
-#pragma unroll
-        for (u32 ch = 0; ch < 8; ++ch) {
-            if (!(m & (2u << (2 * ch)))) continue;
-            const u32 cx = x + (ch & 1), cy = y + ((ch >> 1) & 1), cz = z + (ch >> 2);
-            if (j < child_cap) child_coord[j] = pack_xyz(2 * cx, 2 * cy, 2 * cz);
-            ++j;
-        }
-    }
-};
 
 // ---------------------------------------------------------------------------------------
 // 2. Leaves: createChildmask1x1x8 + createLeafmask (src/CompressedShadowUtil.cpp:59-99).
@@ -215,36 +53,6 @@ struct EmitChildrenOut {
 // (mask, partial leafmasks), and the lane placement of the leafmasks for the store.
 // Node z3 equals node z3-1 of the same column unless some u_t lies in [-8, 6].
 // ---------------------------------------------------------------------------------------
-
-// Empty slot of the merge hash tables (see table_insert).
-constexpr u64 kEmptySlot = ~0ull;
-
-// Grid-stride clear of a merge table, folded into a kernel that runs before the table is used.
-__device__ inline void clear_table(u64* tab, u32 slots) {
-    const u32 stride = gridDim.x * blockDim.x;
-    for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < slots; j += stride) tab[j] = kEmptySlot;
-}
-
-// Key hash of a leaf node (wave-uniform): two 32-bit multiply-xorshift streams over the low and
-// high halves of the partial leafmasks, seeded by the childmask and coupled at every step, then
-// a finaliser that is a bijection on the two streams (so it adds no collisions).
-struct LeafHash {
-    u32 a, b;
-    __device__ explicit LeafHash(u32 mask) : a(mask * 0x9E3779B1u + 0x7F4A7C15u), b((mask ^ 0xA5A5u) * 0x85EBCA77u + 1u) {}
-    __device__ void add(u64 m) {
-        a = (a ^ (u32)m) * 0x9E3779B1u;
-        a ^= a >> 15;
-        b = (b ^ (u32)(m >> 32)) * 0x85EBCA77u;
-        b ^= b >> 13;
-        a += b;
-    }
-    __device__ u64 value() const {
-        u32 x = (a ^ (b >> 16)) * 0x2C1B3C6Du;
-        u32 y = (b ^ (x >> 15)) * 0x297A2D39u;
-        x ^= y >> 13;
-        return ((u64)y << 32) | x;
-    }
-};
 
 // ---------------------------------------------------------------------------------------
 // The leaf kernel: kLeafCols level-3 columns per wave, one texel per lane.
@@ -271,9 +79,6 @@ struct LeafHash {
 // XCD x takes the contiguous units [x*unit_chunk, (x+1)*unit_chunk): neighbouring columns, whose
 // 8-texel depth rows share 128-byte lines, are then read through one L2.
 // ---------------------------------------------------------------------------------------
-constexpr u32 kXcds = 8;
-constexpr u32 kLeafCols = 8;
-constexpr u32 kLeafWaves = 1;
 
 // kList (several z-tiles): the wave's columns are col_list[c0 .. c0+7], the z-tile's columns with
 // leaf nodes in column order, *n_list of them; ncol (at most the leaf node count) bounds the grid.
@@ -421,77 +226,6 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
 // ---------------------------------------------------------------------------------------
 // 3. Merge.
 // ---------------------------------------------------------------------------------------
-// Children ranks of inner node i (slots j >= k are 0).  All eight loads are issued before any is
-// used, so a node costs one memory round trip, not one per child.
-__device__ inline void load_child_ranks(const u32* cstart, const u32* child_rank, u32 i, u32 k, u32 r[8]) {
-    const u32 s = k ? cstart[i] : 0u;
-#pragma unroll
-    for (u32 j = 0; j < 8; ++j) r[j] = (j < k) ? child_rank[s + j] : 0u;
-}
-
-__device__ inline u64 inner_key_hash(const u32* mask, const u32* cstart, const u32* child_rank, u32 i) {
-    const u32 m = mask[i];
-    const u32 k = num_children(m);
-    u64 h = hash_step(kHashSeed, m);
-    if (k) {
-        u32 r[8];
-        load_child_ranks(cstart, child_rank, i, k, r);
-#pragma unroll
-        for (u32 j = 0; j < 8; ++j)
-            if (j < k) h = hash_step(h, r[j]);
-    }
-    return h;
-}
-
-struct LeafEq {
-    const u32* mask;
-    const u64* leafm;     // indexed by leafmask slot
-    const u32* leafslot;  // node -> leafmask slot of its run's first node
-    const u32* runhead;   // when set, a node's mask is read from its run's first node
-    __device__ bool operator()(u32 a, u32 b) const {
-        if (runhead) {
-            a = runhead[a];
-            b = runhead[b];
-            if (a == b) return true;
-        }
-        // Masks and leafmask slots are loaded together (one round trip before the leafmasks).
-        const u32 m = mask[a], mb = mask[b];
-        const u32 sa = leafslot[a], sb = leafslot[b];
-        if (m != mb) return false;
-        const u32 k = num_children(m);
-        const u64* pa = leafm + (size_t)sa * 8;
-        const u64* pb = leafm + (size_t)sb * 8;
-        bool eq = true;
-#pragma unroll
-        for (u32 j = 0; j < 8; ++j)
-            if (j < k) eq = eq && (pa[j] == pb[j]);
-        return eq;
-    }
-};
-struct InnerEq {
-    const u32* mask;
-    const u32* cstart;
-    const u32* child_rank;
-    __device__ bool operator()(u32 a, u32 b) const {
-        // Masks and first-child indices are loaded together (one round trip before the ranks).
-        const u32 m = mask[a], mb = mask[b];
-        const u32 ca = cstart[a], cb = cstart[b];
-        if (m != mb) return false;
-        const u32 k = num_children(m);
-        if (!k) return true;
-        u32 ra[8], rb[8];
-#pragma unroll
-        for (u32 j = 0; j < 8; ++j) {
-            ra[j] = (j < k) ? child_rank[ca + j] : 0u;
-            rb[j] = (j < k) ? child_rank[cb + j] : 0u;
-        }
-        bool eq = true;
-#pragma unroll
-        for (u32 j = 0; j < 8; ++j) eq = eq && (ra[j] == rb[j]);
-        return eq;
-    }
-};
-
 // --- (a) hash table --------------------------------------------------------------------------
 
 // One 64-bit word per slot: the key hash's high 32 bits (its tag) above the smallest node index
@@ -504,95 +238,6 @@ __global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ tab, u32 
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < S) tab[i] = kEmptySlot;
 }
-
-// Open addressing, linear probing.  Large groups of equal keys hit one slot, so the slot is read
-// first, and only an empty slot or a larger index costs an atomic.  A slot only ever goes from
-// empty to one tag, and its index only decreases, so a stale read merely leads to an atomic that
-// was not needed.  The read is an agent-scope load, which is served coherently across the XCDs:
-// each XCD's L2 would otherwise keep showing a slot as empty after another XCD's atomic (atomics
-// are performed at memory) had filled it, and every later insert of that key from the XCD would
-// pay an atomic round trip.  Measured at 8192^2: leaf insert 77 -> 55 µs, inner inserts 137 ->
-// 104 µs against a plain load.
-// A probe gives up after kMaxProbes occupied slots (a table too small for its keys: the leaf
-// level's table is sized for a quarter of its nodes), counts that in *full and returns a slot in
-// range; the host then builds again with a larger table.  Node i's representative is then still a
-// node r with rep[r] = r (k_resolve reads whatever the returned slot holds and checks the keys),
-// so every index the rest of the build derives stays in range.
-constexpr u32 kMaxProbes = 256;
-__device__ inline u32 table_insert(u64* tab, u32 S, u64 h, u32 i, u32* full) {
-    const u32 tag = (u32)(h >> 32);
-    const u64 mine = ((u64)tag << 32) | i;
-    u32 slot = (u32)(h ^ (h >> 31)) & (S - 1);
-    for (u32 probe = 0; probe < kMaxProbes; ++probe) {
-        u64 cur = __hip_atomic_load(tab + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cur == kEmptySlot) {
-            cur = atomicCAS(reinterpret_cast<unsigned long long*>(tab + slot), (unsigned long long)kEmptySlot,
-                            (unsigned long long)mine);
-            if (cur == kEmptySlot) return slot;
-        }
-        if ((u32)(cur >> 32) == tag) {
-            if ((u32)cur > i) atomicMin(reinterpret_cast<unsigned long long*>(tab + slot), (unsigned long long)mine);
-            return slot;
-        }
-        slot = (slot + 1) & (S - 1);
-    }
-    atomicAdd(full, 1u);
-    return slot;
-}
-
-// A table slot hit by many threads at once serialises their atomics, and equal keys are common
-// (flat regions repeat the same subtrees all over a level; a level of a few hundred thousand nodes
-// runs almost all at once, before any first occurrence has landed).  So each 256-thread workgroup
-// first merges its own equal hashes in an LDS table: only the smallest index of each distinct
-// hash in the group touches the global table, and the others take its slot.  Every thread of the
-// workgroup calls this (it has barriers).
-// Threads per insert workgroup (the LDS merge window).  Measured at 8192^2: 512 takes the inner
-// inserts 0.142 -> 0.136 ms against 256; 1024 gains nothing more.
-constexpr u32 kInsertBlock = 512;
-constexpr u32 kGroupSlots = 2 * kInsertBlock;
-
-__device__ inline u32 table_insert_group(u64* tab, u32 S, u64 h, u32 i, bool active, u32* full) {
-    __shared__ u64 gh[kGroupSlots];
-    __shared__ u32 gi[kGroupSlots];
-    __shared__ u32 gs[kGroupSlots];
-    for (u32 j = threadIdx.x; j < kGroupSlots; j += blockDim.x) {
-        gh[j] = 0ull;
-        gi[j] = kNoRep;
-    }
-    __syncthreads();
-    if (h == 0ull) h = 1ull;  // 0 marks an empty LDS slot
-    u32 ls = 0;
-    if (active) {
-        ls = (u32)(h ^ (h >> 29)) & (kGroupSlots - 1);
-        for (;;) {
-            const u64 cur = atomicCAS(reinterpret_cast<unsigned long long*>(gh + ls), 0ull, (unsigned long long)h);
-            if (cur == 0ull || cur == h) break;
-            ls = (ls + 1) & (kGroupSlots - 1);
-        }
-        atomicMin(gi + ls, i);
-    }
-    __syncthreads();
-    if (active && gi[ls] == i) gs[ls] = table_insert(tab, S, h, i, full);
-    __syncthreads();
-    return active ? gs[ls] : 0u;
-}
-
-// zero[0 .. nzero): the ranks scan's tile partials that k_resolve adds into next (cleared here, in a
-// launch that runs anyway).
-__device__ inline void clear_partials(u64* zero, u32 nzero) {
-    const u32 stride = gridDim.x * blockDim.x;
-    for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < nzero; j += stride) zero[j] = 0ull;
-}
-
-// The leaf level: only a run's first node is inserted (1.3 M of 6.7 M nodes at 8192^2), so each
-// thread takes kLeafPer nodes and a workgroup merges a window of kLeafWindow nodes in LDS.  The
-// window's distinct hashes are listed as they are found, and the list is spread over the threads
-// for the global inserts: a workgroup waits for about one global insert per thread, not per node.
-constexpr u32 kLeafInsertThreads = 256;
-constexpr u32 kLeafPer = 4;
-constexpr u32 kLeafWindow = kLeafInsertThreads * kLeafPer;
-constexpr u32 kLeafSlots = 2 * kLeafWindow;  // LDS table at most half full, so every probe ends
-static_assert(kLeafSlots <= 65536, "LDS slot numbers are listed as 16-bit values");
 
 __global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u64* __restrict__ hkey,
                                                                     const u32* __restrict__ runhead, u32 n,
@@ -668,48 +313,6 @@ __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __rest
     if (act) tslot[i] = slot;
 }
 
-// Threads per workgroup of the one-thread-per-node merge and write kernels (at 8192^2, 128 and 256
-// tie; 512 and 1024 make resolve and the writes slower by 5-10 µs).
-constexpr u32 kNodeThreads = 256;
-
-// With partials set, each workgroup also reduces the ranks scan's input over its nodes (FirstSizeIn:
-// 1 << 32 | the node's words, for a first occurrence) into the partial of the scan tile it lies in
-// (tiles of `rounds` workgroups: a store for one, an atomic add into a partial the insert kernel
-// cleared otherwise), so the scan that follows needs no reduce pass.
-static_assert(kNodeThreads == kScanThreads, "resolve workgroups are the ranks scan's rounds");
-template <class Eq>
-__global__ __launch_bounds__(kNodeThreads) void k_resolve(const u32* __restrict__ tslot, const u64* __restrict__ tab,
-                                                 const u32* __restrict__ runhead, u32 n, Eq eq, u32* __restrict__ rep,
-                                                 u32* __restrict__ collisions, const u32* __restrict__ mask, u32 leaf,
-                                                 u64* __restrict__ partials, u32 rounds) {
-    __shared__ u64 s_wave[kNodeThreads / kWave];
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    u64 v = 0;
-    if (i < n) {
-        if (runhead && runhead[i] != i) {
-            rep[i] = kNoRep;  // resolved through its run's first node
-        } else {
-            u32 r = (u32)tab[tslot[i]];  // the smallest index with this node's tag
-            if (r >= n) r = i;
-            if (r != i && !eq(i, r)) {
-                atomicAdd(collisions, 1u);  // two keys share a 64-bit hash: the build is redone by sorting
-                r = i;
-            }
-            rep[i] = r;
-            if (partials && r == i) v = (1ull << 32) | (u64)(1u + (num_children(mask[i]) << leaf));
-        }
-    }
-    if (partials) {  // uniform across the launch
-        v = block_reduce(v, s_wave, OpSum64{});
-        if (threadIdx.x == 0) {
-            if (rounds == 1)
-                partials[blockIdx.x] = v;
-            else if (v)
-                atomicAdd(reinterpret_cast<unsigned long long*>(partials + blockIdx.x / rounds), (unsigned long long)v);
-        }
-    }
-}
-
 // --- (b) radix sort ----------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_sortkeys_leaf(const u64* __restrict__ hkey, const u32* __restrict__ runhead, u32 n,
@@ -730,92 +333,7 @@ __global__ __launch_bounds__(256) void k_sortkeys_inner(const u32* __restrict__ 
     sval[i] = i;
 }
 
-// Segment start of each sorted position: max-scan of (key differs from predecessor ? p : 0).
-struct SegHeadIn {
-    const u32* skey;
-    __device__ u32 operator()(u32 p) const { return (p == 0 || skey[p] != skey[p - 1]) ? p : 0u; }
-};
-struct SegStartOut {
-    u32* seg;
-    __device__ void operator()(u32 p, u32 ex, u32 v) const { seg[p] = ex > v ? ex : v; }
-};
-
-// Representative (first occurrence) of each node.  Within a segment of equal 32-bit keys the
-// stable sort keeps node indices ascending, so the head is the smallest index.
-template <class Eq>
-__global__ __launch_bounds__(256) void k_representative(const u32* __restrict__ sval, const u32* __restrict__ seg, u32 count,
-                                                        Eq eq, u32* __restrict__ rep, u32* __restrict__ collisions) {
-    const u32 p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= count) return;
-    const u32 i = sval[p];
-    const u32 s = seg[p];
-    const u32 head = sval[s];
-    u32 r = head;
-    if (head != i && !eq(i, head)) {
-        atomicAdd(collisions, 1u);
-        r = i;
-        for (u32 q = s + 1; q < p; ++q) {
-            const u32 j = sval[q];
-            if (eq(i, j)) {
-                r = j;
-                break;
-            }
-        }
-    }
-    rep[i] = r;
-}
-
 // --- ranks -------------------------------------------------------------------------------------
-
-struct FirstFlagIn {
-    const u32* rep;
-    __device__ u32 operator()(u32 i) const { return rep[i] == i ? 1u : 0u; }
-};
-struct FirstRankOut {
-    u32* rank;
-    u32* uniq;
-    __device__ void operator()(u32 i, u32 ex, u32 first) const {
-        if (first) {
-            rank[i] = ex;
-            uniq[ex] = i;
-        }
-    }
-};
-
-// Ranks and word offsets in one scan: first occurrences in index order are the uniques in rank
-// order, so a scan of (first ? 1 : 0, first ? node size : 0) -- packed as a u64 whose halves
-// cannot carry (size sums stay below 2^32) -- gives each first occurrence its rank and its offset.
-struct FirstSizeIn {
-    const u32* rep;
-    const u32* mask;
-    u32 leaf;  // 1 at the leafmask level (two words per partial child)
-    __device__ u64 operator()(u32 i) const {
-        const u32 r = rep[i], m = mask[i];  // both loads issued together (a select, not a branch)
-        const u64 v = (1ull << 32) | (u64)(1u + (num_children(m) << leaf));
-        return (r == i) ? v : 0ull;
-    }
-};
-struct FirstSizeOut {
-    u32* rank;
-    u32* uniq;
-    u32* off;
-    u32* n_uniq;  // device: the level's unique count
-    u32* words;   // device: the level's word count
-    u32 n;
-    __device__ void operator()(u32 i, u64 ex, u64 v) const {
-        if (v) {
-            const u32 r = (u32)(ex >> 32);
-            rank[i] = r;
-            uniq[r] = i;
-            off[r] = (u32)ex;
-        }
-        if (i == n - 1) {
-            const u64 t = ex + v;
-            *n_uniq = (u32)(t >> 32);
-            *words = (u32)t;
-        }
-    }
-};
 
 // Copies take their first occurrence's rank.  The merge table of the next level (tab_slots > 0)
 // is cleared here too: this level's last table read (k_resolve) ran before.
@@ -847,50 +365,6 @@ __global__ void k_start(u64* root_coord, u64 root, u32* meta, u32 meta_words) {
 // ---------------------------------------------------------------------------------------
 // 4. Compress.
 // ---------------------------------------------------------------------------------------
-// Sizes of the unique nodes of a level whose word count may pass 2^32 (the levels not ranked by
-// the fused 64-bit scan hold at least 2^32 / 17 nodes).  The sums run in 64 bits; an offset or a
-// level total at or above 2^32 is stored saturated, so the host sees the overflow and rejects the
-// DAG before anything is written.
-struct UniqueSizeIn {
-    const u32* uniq;
-    const u32* mask;
-    const u32* n_uniq;  // device count of uniques of the level
-    u32 leaf;           // 1 at the leafmask level (two words per partial child)
-    __device__ u64 operator()(u32 r) const {
-        if (r >= *n_uniq) return 0ull;
-        return 1ull + (u64)(num_children(mask[uniq[r]]) << leaf);
-    }
-};
-
-__device__ inline u32 saturate32(u64 v) { return v > 0xFFFFFFFFull ? 0xFFFFFFFFu : (u32)v; }
-
-// Offsets of the unique nodes only (the scan runs over the level's node count, which the host
-// knows, while the unique count is on the device), and the level's word count.
-struct UniqueOffOut {
-    u32* off;
-    const u32* n_uniq;
-    u32* words;  // device: the level's word count (saturated)
-    u32 n;
-    __device__ void operator()(u32 r, u64 ex, u64 v) const {
-        if (r < *n_uniq) off[r] = saturate32(ex);
-        if (r == n - 1) *words = saturate32(ex + v);
-    }
-};
-
-// Device metadata block (u32 words).
-enum : int {
-    kMetaU = 0,                          // uniques per level
-    kMetaS = kMaxLevels,                 // words per level
-    kMetaB = 2 * kMaxLevels,             // level base offsets
-    kMetaC = 3 * kMaxLevels,             // children emitted per level (scan totals)
-    kMetaW = 4 * kMaxLevels,             // total words
-    kMetaColl = 4 * kMaxLevels + 1,      // key collisions in the hash table (table merge)
-    kMetaRoot = 4 * kMaxLevels + 2,      // root mask
-    kMetaSortColl = 4 * kMaxLevels + 3,  // 32-bit key collisions resolved by a scan (sort merge)
-    kMetaFull = 4 * kMaxLevels + 4,      // table inserts that found no free slot (table merge)
-    kMetaWords = 4 * kMaxLevels + 5,
-};
-
 __global__ void k_level_bases(u32* __restrict__ meta, int R, int bottom, const u32* __restrict__ root_mask,
                               u32* __restrict__ host_meta) {
     // B[R] = 0, B[L-1] = B[L] + S[L]; total = B[bottom] + S[bottom] (levels absent have S = 0).
@@ -903,48 +377,6 @@ __global__ void k_level_bases(u32* __restrict__ meta, int R, int bottom, const u
     meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
     for (u32 i = 0; i < kMetaWords; ++i) host_meta[i] = meta[i];  // the build's readback
     __threadfence_system();
-}
-
-// Unique node r of an inner level: [mask, absolute offsets of its partial children].
-__device__ inline void write_inner_node(u32 r, const u32* __restrict__ uniq, const u32* __restrict__ mask,
-                                        const u32* __restrict__ cstart, const u32* __restrict__ off, u32 base,
-                                        const u32* __restrict__ child_rank, const u32* __restrict__ child_off, u32 child_base,
-                                        u32* __restrict__ out) {
-    const u32 i = uniq[r];
-    const u32 o = base + off[r];
-    const u32 m = mask[i];
-    out[o] = m;
-    const u32 k = num_children(m);
-    if (!k) return;
-    u32 cr[8];
-    load_child_ranks(cstart, child_rank, i, k, cr);
-    u32 co[8];
-#pragma unroll
-    for (u32 j = 0; j < 8; ++j) co[j] = (j < k) ? child_off[cr[j]] : 0u;
-#pragma unroll
-    for (u32 j = 0; j < 8; ++j)
-        if (j < k) out[o + 1 + j] = child_base + co[j];
-}
-
-// Unique leaf node r: [mask, (lo, hi) of each partial slice's leafmask].
-__device__ inline void write_leaf_node(u32 r, const u32* __restrict__ uniq, const u32* __restrict__ mask,
-                                       const u64* __restrict__ leafm, const u32* __restrict__ leafslot,
-                                       const u32* __restrict__ off, u32 base, u32* __restrict__ out) {
-    const u32 i = uniq[r];
-    const u32 o = base + off[r];
-    const u32 m = mask[i];
-    out[o] = m;
-    const u32 k = num_children(m);
-    const u64* p = leafm + (size_t)leafslot[i] * 8;
-    u64 v[8];
-#pragma unroll
-    for (u32 j = 0; j < 8; ++j) v[j] = (j < k) ? p[j] : 0ull;
-#pragma unroll
-    for (u32 j = 0; j < 8; ++j)
-        if (j < k) {
-            out[o + 1 + 2 * j] = (u32)v[j];
-            out[o + 2 + 2 * j] = (u32)(v[j] >> 32);
-        }
 }
 
 // Every level's words in one launch: once the offsets and bases are known the levels' words are
@@ -1298,15 +730,6 @@ constexpr int kRetryWithSort = -1000;
 constexpr int kRetryFullTable = -1001;
 static_assert(sizeof(u32) * kMetaWords <= sizeof(u64) * kPinnedWords, "meta readback fits the pinned staging");
 
-u32 table_slots(u32 n) {
-    u32 S = 1024;
-    while (S < n + n / 2 + 1) S <<= 1;
-    return S;
-}
-// Below 2^20 leaf nodes the table is small anyway, and small maps have more distinct leaves per
-// node (measured: 256^2 and 512^2 maps fill 85-96% of a quarter-size table, 1024^2 40%, 8192^2 4%).
-constexpr u32 kSmallLeafTableMin = 1u << 20;
-u32 leaf_table_slots(u32 c, bool small) { return table_slots((small && c >= kSmallLeafTableMin) ? c / 4 : c); }
 
 // small_leaf: a leaf level of at least kSmallLeafTableMin nodes gets a merge table sized for a
 // quarter of them (only a run's first node is inserted: a fifth of them at 8192^2, with 4% of the

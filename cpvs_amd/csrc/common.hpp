@@ -45,6 +45,22 @@ __device__ inline u32 classify_mid(u32 z, float d_h) {
     return (mid <= d_h) ? 1u : 0u;
 }
 
+// Partial cells of one level-L texel (mn, mx) inside a z-tile's slab [zlo, zhi) of cells, h =
+// (N>>L)*T: cell z is partial iff !(z + 1 <= mn*h) && !(z >= mx*h) (visible(),
+// src/CompressedShadowUtil.h:35-43), i.e. z in [floor(mn*h), ceil(mx*h)).  The bounds are clipped
+// to the slab in float before the conversion: a NaN bound fails its compare and leaves that side
+// open (as it does in the classification), infinities and huge values clip.  *first = the first
+// partial cell (the slab's start when none).
+__device__ inline u32 partial_cells(float2 v, float h, int zlo, int zhi, int* first) {
+    const float lo = floorf(v.x * h), hi = ceilf(v.y * h);
+    const float fz0 = (float)zlo, fz1 = (float)zhi;
+    const float slo = (lo > fz0) ? ((lo < fz1) ? lo : fz1) : fz0;
+    const float shi = (hi < fz1) ? ((hi > fz0) ? hi : fz0) : fz1;
+    const int s = (int)slo, e = (int)shi;
+    *first = s;
+    return (e > s) ? (u32)(e - s) : 0u;
+}
+
 // 64-bit mixers for subtree keys (Stafford variant 13 finaliser of splitmix64).
 __host__ __device__ inline u64 mix64(u64 z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;

@@ -56,11 +56,30 @@ constexpr unsigned kPinnedFlags = hipHostMallocCoherent | hipHostMallocMapped;
 struct Worker {
     hipStream_t stream = nullptr;
     Arena arena;   // build scratch
-    Arena levels;  // pyramid levels of the item being built
+    Arena levels;  // pyramid levels of the item being built (of all maps of a forest)
     u64* pinned = nullptr;
+    void* fpin = nullptr;  // a forest's pinned staging (count partials, tables, readback), grown as needed
+    size_t fpin_bytes = 0;
     int create();  // on the current device
     void destroy();
+    int forest_staging(size_t bytes);  // ensures fpin holds bytes (waits for the stream first)
 };
+
+// forest.hip: many DAGs built as one forest with one launch chain (see there).
+struct ForestIn {
+    const cpvs_pyramid* p;  // levels >= 3 stored, resident until the build returns
+    u32 t, T;               // zTileIndex, zTileNum
+};
+struct ForestOpts {
+    int merge_mode = CPVS_MERGE_TABLE;
+    u32 leaf_cols = 8;    // leaf kernel: level-3 columns per wave (4 or 8)
+    u32 leaf_waves = 1;   // leaf kernel: waves per workgroup (1 or 2)
+    bool graph = false;   // capture the level chain between the two readbacks in a hipGraph
+};
+constexpr int kForestTooLarge = -1002;  // a level's words or nodes exceed 32 bits: split the forest
+// Builds the DAGs of items[0 .. M) on w's stream; out[i] receives item i's DAG.  Pyramids must
+// be complete on w.stream's timeline.  Returns kForestTooLarge when the forest must be split.
+int build_forest(cpvs_ctx* ctx, Worker& w, const ForestIn* items, u32 M, const ForestOpts& o, cpvs_dag** out);
 
 }  // namespace cpvs
 
@@ -100,6 +119,7 @@ private:
 
 struct cpvs_ctx {
     int device = 0;
+    cpvs_batch_config batch{};  // how batches run (cpvs_ctx_set_batch_config); lanes 0 = default
     cpvs::BufferCache buffers;
     int num_cus = 256;  // compute units of the device (persistent grids size themselves to it)
     hipStream_t stream = nullptr;
@@ -111,7 +131,6 @@ struct cpvs_ctx {
     cpvs::Prof prof;
     int merge_mode = CPVS_MERGE_TABLE;
     std::vector<cpvs::Worker*> lanes;  // batch build lanes, created on first use and kept
-    uint32_t batch_workers = 0;        // lanes a batch uses when the caller passes 0 (0 = default)
 };
 
 struct cpvs_pyramid {
@@ -155,8 +174,9 @@ struct cpvs_container {
 namespace cpvs {
 
 // pyramid.hip.  Levels go to a new allocation owned by p, or, when `into` is given, to that
-// arena (p then does not own them; the arena is reset).
-int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* into = nullptr);
+// arena (p then does not own them; the arena is reset, or with append the levels follow what it
+// holds, within a reservation the caller made).
+int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* into = nullptr, bool append = false);
 // Exact node counts per level for z-tile t of T.  Each count workgroup writes its partial count
 // to partials[block] (pinned host memory, kPinnedPartials entries; may be null), and
 // sum_node_counts adds them up per level once the stream has reached that point: cnt[L] = nodes

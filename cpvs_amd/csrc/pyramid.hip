@@ -143,7 +143,7 @@ size_t pyramid_levels_bytes(u32 N, bool store_all) {
     return arena_bytes(total ? total : 1, sizeof(float2));
 }
 
-int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* into) {
+int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* into, bool append) {
     const u32 N = p->N;
     const int n = p->n;
     // Store levels 1-2 only when asked, or when the octree uses them (no leafmasks, N < 16).
@@ -157,10 +157,16 @@ int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* 
         total += w * w;
     }
     if (into) {
-        const int rc = into->reserve(arena_bytes(total ? total : 1, sizeof(float2)), s);
-        if (rc) return rc;
+        if (!append) {
+            const int rc = into->reserve(arena_bytes(total ? total : 1, sizeof(float2)), s);
+            if (rc) return rc;
+        }
         p->levels = into->take<float2>(total ? total : 1);
         p->levels_owned = false;
+        if (into->used > into->cap) {
+            set_error("pyramid: the level arena holds %zu bytes, %zu needed", into->cap, into->used);
+            return CPVS_EINTERNAL;
+        }
     } else if (!(p->levels = static_cast<float2*>(p->ctx->buffers.take(sizeof(float2) * (total ? total : 1))))) {
         set_error("pyramid: hipMalloc of %zu bytes failed", sizeof(float2) * total);
         return CPVS_ENOMEM;
@@ -240,17 +246,8 @@ __global__ __launch_bounds__(256) void k_node_counts(CountArgs a) {
     for (u32 r = 0; r < kCountPerThread; ++r) {
         const u32 j = base + r * 256 + threadIdx.x;
         if (j < w * w) {
-            // Cell z is partial iff !(z + 1 <= min*h) && !(z >= max*h), i.e. z in
-            // [floor(min*h), ceil(max*h)).  The bounds are clipped to the slab in float before the
-            // conversion: a NaN bound fails its compare and leaves that side open (as it does in
-            // the classification), infinities and huge values clip.
-            const float2 v = a.lv[k][j];
-            const float lo = floorf(v.x * h), hi = ceilf(v.y * h);
-            const float fz0 = (float)zlo, fz1 = (float)zhi;
-            const float slo = (lo > fz0) ? ((lo < fz1) ? lo : fz1) : fz0;
-            const float shi = (hi < fz1) ? ((hi > fz0) ? hi : fz0) : fz1;
-            const int s = (int)slo, e = (int)shi;
-            const u32 c = (e > s) ? (u32)(e - s) : 0u;
+            int s;
+            const u32 c = partial_cells(a.lv[k][j], h, zlo, zhi, &s);
             if (colc) colc[j] = make_uint4(c, 0u, (u32)max(s, 0), 0u);
             acc += c;
         }

@@ -211,6 +211,36 @@ int cpvs_ctx_device_name(cpvs_ctx *ctx, char *buf, uint64_t cap);
 int cpvs_tune_batch(cpvs_ctx *ctx, const cpvs_batch_item *items, uint32_t count, const uint32_t *candidates,
                     uint32_t ncand, uint32_t reps, uint32_t *best, double *ms);
 
+/* How a batch runs.  forest_mtexels = 0: each map is its own launch chain (one cpvs_build per map),
+ * `lanes` chains at once.  forest_mtexels > 0: maps are built as forests -- many DAGs through one
+ * launch chain, every octree level one array over all of them, all counts from one launch and one
+ * readback -- each forest holding maps of at most forest_mtexels x 2^20 texels of work (N^2 /
+ * zTileNum per item; the z-tiles of one map stay in one forest and share its pyramid), `lanes`
+ * forests at once.  leaf_cols (4 or 8) and leaf_waves (1 or 2) shape the forest's leaf kernel
+ * (level-3 columns per wave, waves per workgroup); graph = 1 captures each forest's level chain in
+ * a hipGraph and launches it as one.  Words are identical to cpvs_build's in every configuration.
+ * Maps with N < 16 (no leafmask levels) are built one by one. */
+typedef struct cpvs_batch_config {
+    uint32_t lanes;           /* 1..16 */
+    uint32_t forest_mtexels;  /* 0 = one map per launch chain */
+    uint32_t leaf_cols;       /* 4 or 8 */
+    uint32_t leaf_waves;      /* 1 or 2 */
+    uint32_t graph;           /* 0 or 1 */
+    uint32_t reserved[3];     /* 0 */
+} cpvs_batch_config;
+/* The configuration batches with lanes = 0 use (initially: 4 lanes of forests of 256 Mtexels,
+ * 8 columns per wave, 1 wave per workgroup, no graph). */
+int cpvs_ctx_set_batch_config(cpvs_ctx *ctx, const cpvs_batch_config *cfg);
+int cpvs_ctx_batch_config(cpvs_ctx *ctx, cpvs_batch_config *cfg);
+/* cpvs_build_batch with an explicit configuration (NULL: the context's). */
+int cpvs_build_batch_config(cpvs_ctx *ctx, const cpvs_batch_item *items, uint32_t count, const cpvs_batch_config *cfg,
+                            cpvs_dag **out, double *done_ms);
+/* The autotuner: times the batch once per candidate configuration (best of reps runs after one
+ * warm-up of each, DAGs discarded), stores the fastest as the context's configuration and returns
+ * its index in *best; ms[k] (optional) receives candidate k's time. */
+int cpvs_tune_batch_config(cpvs_ctx *ctx, const cpvs_batch_item *items, uint32_t count, const cpvs_batch_config *cands,
+                           uint32_t ncand, uint32_t reps, uint32_t *best, double *ms);
+
 /* A set of DAGs (numLevels >= 5), each looked up as a one-DAG container
  * (CompressedShadowContainer(unique_ptr<CompressedShadow>), src/CompressedShadowContainer.h:27-32)
  * with an identity light matrix: NDC in, visibility out.  The DAGs are borrowed and must outlive
