@@ -238,6 +238,7 @@ def run_c0(cp, np, ctx, N=1024, boxes=32, nq=1 << 20, reps=20):
         mm = cp.MinMaxHierarchy(d, borrow=True)
         cs = cp.CompressedShadow.create(mm)
         e1 = ctx.event(stream)
+        words = cs.num_words()
         c = cp.CompressedShadowContainer(cs)
         c.copyToGPU()
         e2 = ctx.event(stream)
@@ -248,7 +249,6 @@ def run_c0(cp, np, ctx, N=1024, boxes=32, nq=1 << 20, reps=20):
             lms.append(ctx.elapsed_ms(e2, e3))
         for e in (e0, e1, e2, e3):
             ctx.destroy_event(e)
-        words = cs.num_words()
         c.close()
         mm.close()
     d.free()
