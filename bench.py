@@ -442,11 +442,14 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
     vis = cp.DeviceArray((out_bytes,), np.uint8, ctx)
     ctx.synchronize()
 
-    def serve(serial, arrivals, budget=budget):
+    def serve(serial, arrivals, budget=budget, kcap=kmax):
+        # kcap: builds of larger maps build a 2^kcap map instead, and only sizes up to it are warm.
         srv = cp.Server(lanes=lanes, budget_bytes=budget, serial=serial, ctx=ctx)
         try:
             warm = []
             for k in sizes:
+                if k > kcap:
+                    continue
                 for _ in range(warm_per_size):
                     warm.append(srv.build(depth[k]))
             for t in warm:
@@ -465,10 +468,10 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
                             time.sleep(dt - 1e-4)
                     lag.append((time.perf_counter() - t0) - at)
                 if kind == 0:
-                    tickets.append(srv.build(depth[a]))
+                    tickets.append(srv.build(depth[min(a, kcap)]))
                 else:
                     try:
-                        tickets.append(srv.lookup(warm[a], q, vis, count=n, offset=qo, vis_offset=off))
+                        tickets.append(srv.lookup(warm[a % len(warm)], q, vis, count=n, offset=qo, vis_offset=off))
                     except cp.CpvsError:  # its DAG was evicted under a tight budget: not served
                         missed += 1
                         continue
@@ -505,9 +508,11 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
         arr.append(t)
     lat = {m: serve(m == "serial", arr) for m in ("serial", "served")}
     log(f"[bench] serving latency at {rate:.0f} req/s: {lat}")
-    # The same Poisson trace under a budget that forces evictions: the resident DAGs of the builds
-    # outgrow it, so least recently used DAGs are evicted (no device synchronisation on that path).
-    press = {m: serve(m == "serial", arr, pressure_budget) for m in ("serial", "served")}
+    # The same Poisson trace under a budget that forces evictions, with maps of at most 8192^2 (a
+    # 16384^2 map of this generator has about 200 M leaf nodes and alone needs some 35 GB of scratch
+    # and DAG reservation): the resident DAGs of the builds outgrow the budget, so least recently
+    # used DAGs are evicted (no device synchronisation on that path).
+    press = {m: serve(m == "serial", arr, pressure_budget, kcap=min(kmax, 13)) for m in ("serial", "served")}
     log(f"[bench] serving under a {pressure_budget >> 30} GiB budget: {press}")
     vis.free()
     q.free()
@@ -528,7 +533,7 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
             "lookup_launches": {m: v["lookup_batches"] for m, v in lat.items()},
             "evicted": {m: v["evicted"] for m, v in lat.items()},
             "submit_lag_ms_p99": {m: r(v["submit_lag_ms_p99"]) for m, v in lat.items()},
-            "pressure": {"budget_GiB": pressure_budget >> 30,
+            "pressure": {"budget_GiB": pressure_budget >> 30, "max_map": f"{1 << min(kmax, 13)}^2",
                          "latency_ms": {m: {k: r(v[k]) for k in ("build_ms_p50", "build_ms_p99", "lookup_ms_p50",
                                                                   "lookup_ms_p99")} for m, v in press.items()},
                          "req_per_s": {m: round(v["req_per_s"], 1) for m, v in press.items()},

@@ -86,8 +86,9 @@ __global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord,
 template <bool kList>
 __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, u32 ncol, u32 nleaf,
                                                                 const float* __restrict__ depth, u32 N, u32 T,
+                                                                const float2* __restrict__ lv3,
                                                                 u32* __restrict__ mask_out, u64* __restrict__ leafm,
-                                                                u64* __restrict__ hkey, u32* __restrict__ runhead,
+                                                                u32* __restrict__ runhead,
                                                                 u32* __restrict__ leafslot, u32 unit_chunk,
                                                                 u64* __restrict__ tab, u32 tab_slots,
                                                                 const u32* __restrict__ col_list,
@@ -138,12 +139,18 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
         b[k] = lc.bfs[(lane < cnt) ? st + lane : 0u];  // st + lane < nleaf: the column counts sum to it
     }
     const float HT = (float)(N * T);
+    // The columns' minimum and maximum f from their level-3 pyramid texels (the 8x8 blocks' depth
+    // minimum and maximum): lane k < kLeafCols holds column k's, lane k + kLeafCols its maximum.
+    // A block holding a NaN takes them from its texels instead (the pyramid keeps a NaN only as a
+    // first operand, so its extremes do not tell).
+    const float2 ext = lv3[my_col];
+    const int fext = leaf_f(((lane & kLeafCols) ? ext.y : ext.x) * HT);
+    u32 nan_cols = 0;
 #pragma unroll
     for (u32 k = 0; k < kLeafCols; ++k) {
         const float D = d[k] * HT;
-        int f = -(1 << 30);  // NaN is never lit
-        if (D == D) f = (int)floorf(fminf(fmaxf(D, -2.0f), 4194304.0f) - 0.5f);  // exact for |D| < 2^22
-        s_f[wave][k][lane] = f;
+        if (__ballot(!(D == D))) nan_cols |= 1u << k;
+        s_f[wave][k][lane] = leaf_f(D);
         s_b[wave][k][lane] = b[k];
     }
 
@@ -154,7 +161,14 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
         const u32 base = __builtin_amdgcn_readlane(ci.y, k);
         const int zs = (int)__builtin_amdgcn_readlane(ci.z, k);
         const int f = s_f[wave][k][lane];
-        const int fmin = __ockl_wfred_min_i32(f), fmax = __ockl_wfred_max_i32(f);
+        int fmin, fmax;
+        if ((nan_cols >> k) & 1u) {
+            fmin = __ockl_wfred_min_i32(f);
+            fmax = __ockl_wfred_max_i32(f);
+        } else {
+            fmin = __builtin_amdgcn_readlane(fext, k);
+            fmax = __builtin_amdgcn_readlane(fext, k + kLeafCols);
+        }
         const int h = f + 8 - 8 * zs;
         const int qa = h >> 3;  // floor(h / 8)
         const bool below = (h & 7) != 7;
@@ -171,9 +185,9 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
             u64 chg = ((u64)chg_hi << 32) | chg_lo;  // wave-uniform
             if (q0 == 0) chg |= 1ull;                // a column's first node starts a run
             if (qn < 64) chg &= (1ull << qn) - 1ull;
-            // The run heads of this chunk, one after the other (wave-uniform): mask, leafmasks, hash.
+            // The run heads of this chunk, one after the other (wave-uniform): mask and leafmasks (their
+            // key hash is computed lane-parallel by the leaf insert, which reads them anyway).
             u32 my_mask = 0;
-            u64 my_hash = 0;
             for (u64 pend = chg; pend; pend &= pend - 1ull) {
                 const u32 j = (u32)__builtin_ctzll(pend);
                 const int z8 = 8 * (zs + (int)(q0 + j));
@@ -182,19 +196,15 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
                 const u32 cmax = (u32)min(max(fmax - z8 + 1, 0), 8);
                 const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
                 const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min);
-                LeafHash hs(mask);
                 u64 mine = 0;  // lane s gets the leafmask of partial slice cmin + s
                 const u32 kp = cmax - cmin;
                 for (u32 sl = 0; sl < kp; ++sl) {
                     const u64 m = __ballot(u >= (int)(cmin + sl));
-                    hs.add(m);
                     mine = (lane == sl) ? m : mine;
                 }
                 const u32 slot = base + q0 + j;  // < nleaf
                 if (lane < kp && slot < nleaf) leafm[(size_t)slot * 8 + lane] = mine;
-                const u64 hv = hs.value();
                 my_mask = (lane == j) ? mask : my_mask;
-                my_hash = (lane == j) ? hv : my_hash;
             }
             // Node q0 + lane's run head: the last changed node at or before it.
             const u64 upto = chg & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
@@ -208,10 +218,7 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
             if (lane < qn && bfs < nleaf) {
                 runhead[bfs] = my_head_bfs;
                 leafslot[bfs] = base + my_head_q;
-                if ((chg >> lane) & 1ull) {
-                    mask_out[bfs] = my_mask;
-                    hkey[bfs] = my_hash;
-                }
+                if ((chg >> lane) & 1ull) mask_out[bfs] = my_mask;
             }
         };
         // The first chunk's indices are staged; a longer column (box walls) loads the rest here.
@@ -239,7 +246,8 @@ __global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ tab, u32 
     if (i < S) tab[i] = kEmptySlot;
 }
 
-__global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u64* __restrict__ hkey,
+__global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u32* __restrict__ mask, const u64* __restrict__ leafm,
+                                                                    const u32* __restrict__ leafslot,
                                                                     const u32* __restrict__ runhead, u32 n,
                                                                     u64* __restrict__ tab, u32 S, u32* __restrict__ tslot,
                                                                     u32* __restrict__ full, u64* __restrict__ zero,
@@ -261,11 +269,21 @@ __global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u64* _
         const u32 i = i0 + k * kLeafInsertThreads;
         rh[k] = (i < n) ? runhead[i] : kNoRep;
     }
+    // The key hash of each run's first node (the run's smallest index), from its mask and
+    // leafmasks: computed here, lane-parallel, rather than wave-uniform in the leaf kernel.
+    u32 hm[kLeafPer], hs[kLeafPer];
+#pragma unroll
+    for (u32 k = 0; k < kLeafPer; ++k) {
+        const u32 i = i0 + k * kLeafInsertThreads;
+        const bool head = rh[k] == i;
+        hm[k] = head ? mask[i] : 0u;
+        hs[k] = head ? leafslot[i] : 0u;
+    }
     u64 h[kLeafPer];
 #pragma unroll
     for (u32 k = 0; k < kLeafPer; ++k) {
         const u32 i = i0 + k * kLeafInsertThreads;
-        h[k] = (rh[k] == i) ? hkey[i] : 0ull;  // a run's first node has the run's smallest index
+        h[k] = (rh[k] == i) ? leaf_key_hash(hm[k], leafm + (size_t)hs[k] * 8) : 0ull;
         if (h[k] == 0ull && rh[k] == i) h[k] = 1ull;
     }
     __syncthreads();
@@ -315,12 +333,14 @@ __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __rest
 
 // --- (b) radix sort ----------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_sortkeys_leaf(const u64* __restrict__ hkey, const u32* __restrict__ runhead, u32 n,
-                                                       u32* __restrict__ skey, u32* __restrict__ sval) {
+__global__ __launch_bounds__(256) void k_sortkeys_leaf(const u32* __restrict__ mask, const u64* __restrict__ leafm,
+                                                       const u32* __restrict__ leafslot, const u32* __restrict__ runhead,
+                                                       u32 n, u32* __restrict__ skey, u32* __restrict__ sval) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const u32 h = runhead[i];
-    skey[i] = fold32(hkey[h < n ? h : i]);
+    u32 h = runhead[i];  // the key is the run's first node's
+    h = (h < n) ? h : i;
+    skey[i] = fold32(leaf_key_hash(mask[h], leafm + (size_t)leafslot[h] * 8));
     sval[i] = i;
 }
 
@@ -390,6 +410,7 @@ struct WriteLevel {
     const u32* child_rank;  // null at the deepest level
     const u32* child_off;
     u32 U, base, child_base, leaf;
+    int L;                  // the octree level (k_write_levels_dev reads U and the bases of meta)
 };
 struct WriteArgs {
     int nlev;
@@ -409,6 +430,44 @@ __global__ __launch_bounds__(kNodeThreads) void k_write_levels(WriteArgs a, u32*
         write_leaf_node(r, l.uniq, l.mask, a.leafm, a.leafslot, l.off, l.base, out);
     else
         write_inner_node(r, l.uniq, l.mask, l.cstart, l.off, l.base, l.child_rank, l.child_off, l.child_base, out);
+}
+
+// The same with the unique counts and level bases read from the metadata block, so that it runs
+// right behind the merge with no host readback between: a grid-stride loop over the levels'
+// blocks (their number is known on the device only), the words into a staging area of the
+// pre-merge word bound.
+__global__ __launch_bounds__(kNodeThreads) void k_write_levels_dev(WriteArgs a, const u32* __restrict__ meta,
+                                                                  u32* __restrict__ out, u64 cap) {
+    __shared__ u32 s_first[kMaxLevels + 1], s_U[kMaxLevels], s_base[kMaxLevels], s_cbase[kMaxLevels];
+    if (threadIdx.x == 0) {
+        u32 acc = 0;
+        for (int k = 0; k < a.nlev; ++k) {
+            const int L = a.lv[k].L;
+            s_U[k] = meta[kMetaU + L];
+            s_base[k] = meta[kMetaB + L];
+            s_cbase[k] = a.lv[k].child_rank ? meta[kMetaB + L - 1] : 0u;
+            s_first[k] = acc;
+            acc += (s_U[k] + kNodeThreads - 1) / kNodeThreads;
+        }
+        s_first[a.nlev] = acc;
+    }
+    __syncthreads();
+    const u32 nb = s_first[a.nlev];
+    for (u32 vb = blockIdx.x; vb < nb; vb += gridDim.x) {
+        int k = 0;
+        while (k + 1 < a.nlev && vb >= s_first[k + 1]) ++k;
+        const WriteLevel& l = a.lv[k];
+        const u32 r = (vb - s_first[k]) * blockDim.x + threadIdx.x;
+        if (r >= s_U[k]) continue;
+        // Offsets lie below the word count, itself below the staging's pre-merge bound; a node
+        // that would not fit (only if the build were inconsistent, which the host then reports)
+        // is not written.
+        if ((u64)s_base[k] + l.off[r] + 17 > cap) continue;
+        if (l.leaf)
+            write_leaf_node(r, l.uniq, l.mask, a.leafm, a.leafslot, l.off, s_base[k], out);
+        else
+            write_inner_node(r, l.uniq, l.mask, l.cstart, l.off, s_base[k], l.child_rank, l.child_off, s_cbase[k], out);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -738,6 +797,19 @@ static_assert(sizeof(u32) * kMetaWords <= sizeof(u64) * kPinnedWords, "meta read
 // the build runs again.
 __global__ void k_store_word(u32* dst, u32 v) { *dst = v; }
 
+// The staged words into the DAG's own buffer (16-byte vectors where both are aligned).
+__global__ __launch_bounds__(256) void k_copy_words(const u32* __restrict__ src, u32* __restrict__ dst, u64 n) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 i0 = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if ((((uintptr_t)src | (uintptr_t)dst) & 15) == 0) {
+        const u64 n4 = n / 4;
+        for (u64 i = i0; i < n4; i += stride) reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
+        for (u64 i = 4 * n4 + i0; i < n; i += stride) dst[i] = src[i];
+    } else {
+        for (u64 i = i0; i < n; i += stride) dst[i] = src[i];
+    }
+}
+
 int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, int merge_mode,
                cpvs_dag** out, const Admit* admit, bool small_leaf, DagOut* into) {
     const int n = p->n;
@@ -825,7 +897,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         need += arena_bytes(c, 8) + 5 * arena_bytes(c, 4);
     }
     const size_t c2 = leaf ? (cnt[2] ? cnt[2] : 1) : 1;
-    if (leaf) need += arena_bytes(c2 * 8, 8) + arena_bytes(c2, 8) + 3 * arena_bytes(c2, 4);
+    if (leaf) need += arena_bytes(c2 * 8, 8) + 3 * arena_bytes(c2, 4);
     need += 2 * arena_bytes(cmax, 4);
     const bool col_list = leaf && T > 1;  // compact the z-tile's columns with leaf nodes
     if (col_list) need += arena_bytes(ncol, 4) + arena_bytes(1, 4);
@@ -833,13 +905,17 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     else need += 4 * arena_bytes(cmax, 4) + arena_bytes(sort_scratch_words(cmax), 4);
     need += arena_bytes(scan_scratch_words(scan_n), 4);
     need += 1 << 20;
+    // Pre-merge words bound the DAG: mask and child pointers per node, two words per partial slice
+    // at the leaves.  The words are written into a staging area of that size right behind the
+    // merge (no host readback before the write), then copied to an exact allocation; a caller's
+    // output buffer (into) is written directly after the readback instead.
+    u64 bound_words = 0;
+    for (int L = bottom; L <= R; ++L)
+        bound_words += cnt[L] + ((L > bottom) ? cnt[L - 1] : 0) + ((leaf && L == 2) ? 16 * cnt[L] : 0);
+    const bool staged = !into;
+    if (staged) need += arena_bytes(bound_words + 17, 4);  // 17 words of slack for the bound check
     if (admit) {
-        // Pre-merge words bound the DAG: mask and child pointers per node, two words per
-        // partial slice at the leaves.
-        u64 words = 0;
-        for (int L = bottom; L <= R; ++L)
-            words += cnt[L] + ((L > bottom) ? cnt[L - 1] : 0) + ((leaf && L == 2) ? 16 * cnt[L] : 0);
-        rc = admit->fn(admit->user, A, need, 4 * words);
+        rc = admit->fn(admit->user, A, need, 4 * bound_words);
         if (rc) return rc;
     }
     bool moved = false;
@@ -865,7 +941,6 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         lv[L].off = A.take<u32>(c);
     }
     u64* leafm = leaf ? A.take<u64>(c2 * 8) : nullptr;
-    u64* hkey = leaf ? A.take<u64>(c2) : nullptr;
     u32* runhead = leaf ? A.take<u32>(c2) : nullptr;
     u32* leafslot = leaf ? A.take<u32>(c2) : nullptr;
     u32* colbfs = leaf ? A.take<u32>(c2) : nullptr;
@@ -884,6 +959,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         sort_scratch = A.take<u32>(sort_scratch_words(cmax));
     }
     u32* scan_scratch = A.take<u32>(scan_scratch_words(scan_n));
+    u32* staging = staged ? A.take<u32>(bound_words + 17) : nullptr;
     if (A.used > A.cap) {
         set_error("build: scratch layout exceeds its reservation (%zu > %zu)", A.used, A.cap);
         return CPVS_EINTERNAL;
@@ -941,12 +1017,12 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             // The leaf level is the first merged: its table is cleared by this kernel.
             if (col_list)
                 CPVS_LAUNCH("leaf_waves", k_leaf_waves<true>, dim3(unit_chunk * kXcds), dim3(64 * kLeafWaves), 0, s, lc,
-                            bound, c, p->depth, N, T, lv[2].mask, leafm, hkey, runhead, leafslot, unit_chunk, tab, slots,
-                            clist, nclist);
+                            bound, c, p->depth, N, T, p->lv[3], lv[2].mask, leafm, runhead, leafslot, unit_chunk, tab,
+                            slots, clist, nclist);
             else
                 CPVS_LAUNCH("leaf_waves", k_leaf_waves<false>, dim3(unit_chunk * kXcds), dim3(64 * kLeafWaves), 0, s, lc,
-                            bound, c, p->depth, N, T, lv[2].mask, leafm, hkey, runhead, leafslot, unit_chunk, tab, slots,
-                            nullptr, nullptr);
+                            bound, c, p->depth, N, T, p->lv[3], lv[2].mask, leafm, runhead, leafslot, unit_chunk, tab,
+                            slots, nullptr, nullptr);
             break;
         }
         // A level is classified inside its emit scan when that scan loads its outputs' prefetch
@@ -999,8 +1075,8 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             // Cleared by the leaf kernel (leaf level) or the previous level's k_rank_copies.
             if (L == deepest && !is_leaf) CPVS_LAUNCH("table_clear", k_table_clear, grid1d(Sl), dim3(256), 0, s, tab, Sl);
             if (is_leaf) {
-                CPVS_LAUNCH("insert_leaf", k_insert_leaf, dim3(ceil_div<u32>(c, kLeafWindow)), dim3(kLeafInsertThreads), 0, s, hkey,
-                            runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
+                CPVS_LAUNCH("insert_leaf", k_insert_leaf, dim3(ceil_div<u32>(c, kLeafWindow)), dim3(kLeafInsertThreads), 0, s,
+                            lv[2].mask, leafm, leafslot, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
                 CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, runhead, c,
                             LeafEq{lv[2].mask, leafm, leafslot, nullptr}, rep, meta + kMetaColl, lv[2].mask, 1u, rpart, rrounds);
             } else {
@@ -1012,7 +1088,8 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             }
         } else {
             if (is_leaf)
-                CPVS_LAUNCH("sortkeys", k_sortkeys_leaf, grid1d(c), dim3(256), 0, s, hkey, runhead, c, skey, sval);
+                CPVS_LAUNCH("sortkeys", k_sortkeys_leaf, grid1d(c), dim3(256), 0, s, lv[2].mask, leafm, leafslot, runhead, c, skey,
+                            sval);
             else
                 CPVS_LAUNCH("sortkeys", k_sortkeys_inner, grid1d(c), dim3(256), 0, s, lv[L].mask, lv[L].cstart, child_rank, c,
                             skey, sval);
@@ -1059,8 +1136,29 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         CPVS_LAUNCH("level_bases", k_level_bases, dim3(1), dim3(1), 0, s, meta, R, deepest, lv[R].mask,
                     reinterpret_cast<u32*>(pinned));
     }
+    // The word writer's per-level arrays (the counts and bases come from meta or the readback).
+    WriteArgs wa{};
+    u32 wmax = 0;  // blocks of the largest possible write (every node unique)
+    for (int L = R; L >= deepest; --L) {
+        WriteLevel& w = wa.lv[wa.nlev++];
+        const bool kids = L > deepest;
+        w.uniq = lv[L].uniq;
+        w.mask = lv[L].mask;
+        w.cstart = lv[L].cstart;
+        w.off = lv[L].off;
+        w.child_rank = kids ? lv[L - 1].rank : nullptr;
+        w.child_off = kids ? lv[L - 1].off : nullptr;
+        w.leaf = (leaf && L == 2) ? 1u : 0u;
+        w.L = L;
+        wmax += ceil_div<u32>(lv[L].count ? lv[L].count : 1, kNodeThreads);
+    }
+    wa.leafm = leafm;
+    wa.leafslot = leafslot;
+    if (staged)
+        CPVS_LAUNCH("write_levels", k_write_levels_dev, dim3(std::min<u32>(wmax, 2048)), dim3(kNodeThreads), 0, s, wa, meta,
+                    staging, bound_words + 17);
     CPVS_HIP_CHECK(hipGetLastError());
-    CPVS_HIP_CHECK(hipStreamSynchronize(s));  // the last kernel copied meta into pinned
+    CPVS_HIP_CHECK(hipStreamSynchronize(s));  // the bases kernel copied meta into pinned
     u32 hm[kMetaWords];
     std::memcpy(hm, pinned, sizeof(hm));
 
@@ -1093,28 +1191,22 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         set_error("build: hipMalloc of %llu DAG words failed", (unsigned long long)total);
         return CPVS_ENOMEM;
     }
-    WriteArgs wa{};
-    u32 wblocks = 0;
-    for (int L = R; L >= deepest; --L) {
-        WriteLevel& w = wa.lv[wa.nlev];
-        const bool kids = L > deepest;
-        w.uniq = lv[L].uniq;
-        w.mask = lv[L].mask;
-        w.cstart = lv[L].cstart;
-        w.off = lv[L].off;
-        w.child_rank = kids ? lv[L - 1].rank : nullptr;
-        w.child_off = kids ? lv[L - 1].off : nullptr;
-        w.U = hm[kMetaU + L];
-        w.base = hm[kMetaB + L];
-        w.child_base = kids ? hm[kMetaB + L - 1] : 0u;
-        w.leaf = (leaf && L == 2) ? 1u : 0u;
-        wa.first[wa.nlev++] = wblocks;
-        wblocks += ceil_div<u32>(w.U, kNodeThreads);
+    if (staged) {
+        if (total) CPVS_LAUNCH("copy_words", k_copy_words, dim3(std::min<u64>(ceil_div<u64>(total, 1024), 4096)),
+                               dim3(256), 0, s, staging, d_words, total);
+    } else {
+        u32 wblocks = 0;
+        for (int k = 0; k < wa.nlev; ++k) {
+            WriteLevel& w = wa.lv[k];
+            w.U = hm[kMetaU + w.L];
+            w.base = hm[kMetaB + w.L];
+            w.child_base = w.child_rank ? hm[kMetaB + w.L - 1] : 0u;
+            wa.first[k] = wblocks;
+            wblocks += ceil_div<u32>(w.U, kNodeThreads);
+        }
+        wa.first[wa.nlev] = wblocks;
+        if (wblocks) CPVS_LAUNCH("write_levels", k_write_levels, dim3(wblocks), dim3(kNodeThreads), 0, s, wa, d_words);
     }
-    wa.first[wa.nlev] = wblocks;
-    wa.leafm = leafm;
-    wa.leafslot = leafslot;
-    if (wblocks) CPVS_LAUNCH("write_levels", k_write_levels, dim3(wblocks), dim3(kNodeThreads), 0, s, wa, d_words);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {

@@ -207,6 +207,30 @@ struct LeafHash {
 };
 
 
+// f = floor(D - 0.5) of a texel's scaled depth D = d * N * T: slice z of its column is lit iff
+// z <= f (the midpoint test float(z) + 0.5 <= D, exact for |D| < 2^22).  NaN is never lit.  f is
+// non-decreasing in d, so the f of a block's minimum (maximum) depth is the minimum (maximum) of
+// its texels' f when the block holds no NaN.
+__device__ inline int leaf_f(float D) {
+    int f = -(1 << 30);
+    if (D == D) f = (int)floorf(fminf(fmaxf(D, -2.0f), 4194304.0f) - 0.5f);
+    return f;
+}
+
+// Key hash of a leaf node from its mask and its partial slices' leafmasks (slot8: the node's
+// leafmask slot, partial slices in ascending order).  All loads are issued before the hash.
+__device__ inline u64 leaf_key_hash(u32 m, const u64* __restrict__ slot8) {
+    const u32 k = num_children(m);
+    u64 v[8];
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j) v[j] = (j < k) ? slot8[j] : 0ull;
+    LeafHash hs(m);
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j)
+        if (j < k) hs.add(v[j]);
+    return hs.value();
+}
+
 constexpr u32 kXcds = 8;
 constexpr u32 kLeafCols = 8;
 constexpr u32 kLeafWaves = 1;
@@ -603,7 +627,8 @@ inline u32 leaf_table_slots(u32 c, bool small) { return table_slots((small && c 
 
 // Non-template kernels defined in build.hip.
 __global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ tab, u32 S);
-__global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u64* __restrict__ hkey,
+__global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u32* __restrict__ mask, const u64* __restrict__ leafm,
+                                                                    const u32* __restrict__ leafslot,
                                                                     const u32* __restrict__ runhead, u32 n,
                                                                     u64* __restrict__ tab, u32 S, u32* __restrict__ tslot,
                                                                     u32* __restrict__ full, u64* __restrict__ zero,
@@ -612,8 +637,9 @@ __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __rest
                                                                const u32* __restrict__ child_rank, u32 n, u64* __restrict__ tab,
                                                                u32 S, u32* __restrict__ tslot, u32* __restrict__ full,
                                                                u64* __restrict__ zero, u32 nzero);
-__global__ __launch_bounds__(256) void k_sortkeys_leaf(const u64* __restrict__ hkey, const u32* __restrict__ runhead, u32 n,
-                                                       u32* __restrict__ skey, u32* __restrict__ sval);
+__global__ __launch_bounds__(256) void k_sortkeys_leaf(const u32* __restrict__ mask, const u64* __restrict__ leafm,
+                                                       const u32* __restrict__ leafslot, const u32* __restrict__ runhead,
+                                                       u32 n, u32* __restrict__ skey, u32* __restrict__ sval);
 __global__ __launch_bounds__(256) void k_sortkeys_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
                                                         const u32* __restrict__ child_rank, u32 n, u32* __restrict__ skey,
                                                         u32* __restrict__ sval);

@@ -494,7 +494,7 @@ int cpvs_estimate_build(cpvs_ctx* ctx, const cpvs_pyramid* p, uint32_t t, uint32
     }
     const u64 c2 = (bottom == 2) ? (cnt[2] ? cnt[2] : 1) : 0;
     const u64 ncol = (bottom == 2) ? (u64)(p->N >> 3) * (p->N >> 3) : 0;
-    const u64 leaf = (bottom == 2) ? arena_bytes(c2 * 8, 8) + arena_bytes(c2, 8) + 3 * arena_bytes(c2, 4) +
+    const u64 leaf = (bottom == 2) ? arena_bytes(c2 * 8, 8) + 3 * arena_bytes(c2, 4) +
                                          arena_bytes(ncol, 16)
                                    : 0;
     u64 slots = 1024;
@@ -506,7 +506,8 @@ int cpvs_estimate_build(cpvs_ctx* ctx, const cpvs_pyramid* p, uint32_t t, uint32
                       (1 << 20);
     if (nodes)
         for (int L = 0; L < n; ++L) nodes[L] = cnt[L];
-    if (scratch_bytes) *scratch_bytes = per_level + leaf + temps;
+    // The words are staged at their pre-merge bound before the exact allocation.
+    if (scratch_bytes) *scratch_bytes = per_level + leaf + temps + arena_bytes(words + 17, 4);
     if (max_dag_bytes) *max_dag_bytes = 4 * words;
     return CPVS_OK;
     GUARD_END

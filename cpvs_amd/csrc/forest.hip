@@ -223,7 +223,7 @@ template <u32 kCols, u32 kWaves>
 __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __restrict__ colinfo, const u32* __restrict__ colbfs,
                                                              const FItem* __restrict__ items, u32 nleaf,
                                                              u32* __restrict__ mask_out, u64* __restrict__ leafm,
-                                                             u64* __restrict__ hkey, u32* __restrict__ runhead,
+                                                             u32* __restrict__ runhead,
                                                              u32* __restrict__ leafslot, u64* __restrict__ tab, u32 tab_slots,
                                                              const u32* __restrict__ col_list, const u32* __restrict__ n_list,
                                                              u32 list_cap) {
@@ -254,12 +254,21 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
         const u32 cnt = __builtin_amdgcn_readlane(ci.x, k), st = __builtin_amdgcn_readlane(ci.y, k);
         b[k] = colbfs[(lane < cnt) ? st + lane : 0u];
     }
+    // The columns' minimum and maximum f from their DAGs' level-3 pyramid texels (lane k < kCols:
+    // column k's minimum, lane k + kCols its maximum), or from the texels for a block with a NaN.
+    int fext;
+    {
+        const uint4 cj = colinfo[my_col];  // lanes k and k + kCols read column k's record
+        const FItem& it = items[cj.w];
+        const float2 ext = it.lv[3][my_col - it.col_base];
+        fext = leaf_f(((lane & kCols) ? ext.y : ext.x) * (float)(it.N * it.T));
+    }
+    u32 nan_cols = 0;
 #pragma unroll
     for (u32 k = 0; k < kCols; ++k) {
         const float D = d[k] * HT[k];
-        int f = -(1 << 30);  // NaN is never lit
-        if (D == D) f = (int)floorf(fminf(fmaxf(D, -2.0f), 4194304.0f) - 0.5f);  // exact for |D| < 2^22
-        s_f[wave][k][lane] = f;
+        if (__ballot(!(D == D))) nan_cols |= 1u << k;
+        s_f[wave][k][lane] = leaf_f(D);
         s_b[wave][k][lane] = b[k];
     }
 
@@ -271,7 +280,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
         const int zs = (int)__builtin_amdgcn_readlane(ci.z, k);
         const u32 gtag = __builtin_amdgcn_readlane(ci.w, k) << 16;
         const int f = s_f[wave][k][lane];
-        const int fmin = __ockl_wfred_min_i32(f), fmax = __ockl_wfred_max_i32(f);
+        int fmin, fmax;
+        if ((nan_cols >> k) & 1u) {
+            fmin = __ockl_wfred_min_i32(f);
+            fmax = __ockl_wfred_max_i32(f);
+        } else {
+            fmin = __builtin_amdgcn_readlane(fext, k);
+            fmax = __builtin_amdgcn_readlane(fext, k + kCols);
+        }
         const int h = f + 8 - 8 * zs;
         const int qa = h >> 3;
         const bool below = (h & 7) != 7;
@@ -288,7 +304,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
             if (q0 == 0) chg |= 1ull;
             if (qn < 64) chg &= (1ull << qn) - 1ull;
             u32 my_mask = 0;
-            u64 my_hash = 0;
             for (u64 pend = chg; pend; pend &= pend - 1ull) {
                 const u32 j = (u32)__builtin_ctzll(pend);
                 const int z8 = 8 * (zs + (int)(q0 + j));
@@ -297,19 +312,15 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
                 const u32 cmax = (u32)min(max(fmax - z8 + 1, 0), 8);
                 const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
                 const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min) | gtag;
-                LeafHash hs(mask);
                 u64 mine = 0;
                 const u32 kp = cmax - cmin;
                 for (u32 sl = 0; sl < kp; ++sl) {
                     const u64 m = __ballot(u >= (int)(cmin + sl));
-                    hs.add(m);
                     mine = (lane == sl) ? m : mine;
                 }
                 const u32 slot = base + q0 + j;
                 if (lane < kp && slot < nleaf) leafm[(size_t)slot * 8 + lane] = mine;
-                const u64 hv = hs.value();
                 my_mask = (lane == j) ? mask : my_mask;
-                my_hash = (lane == j) ? hv : my_hash;
             }
             const u64 upto = chg & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
             const int src = upto ? 63 - __builtin_clzll(upto) : 0;
@@ -321,10 +332,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
             if (lane < qn && bfs < nleaf) {
                 runhead[bfs] = my_head_bfs;
                 leafslot[bfs] = base + my_head_q;
-                if ((chg >> lane) & 1ull) {
-                    mask_out[bfs] = my_mask;
-                    hkey[bfs] = my_hash;
-                }
+                if ((chg >> lane) & 1ull) mask_out[bfs] = my_mask;
             }
         };
         chunk(0u, s_b[wave][k][lane]);
@@ -445,22 +453,22 @@ constexpr int kRetryFull = -1011;
 
 // Launches the leaf kernel in the shape the options name.
 void launch_leaf(const ForestOpts& o, u32 units_cols, hipStream_t s, const uint4* colinfo, const u32* colbfs,
-                 const FItem* items, u32 nleaf, u32* mask, u64* leafm, u64* hkey, u32* runhead, u32* leafslot, u64* tab,
+                 const FItem* items, u32 nleaf, u32* mask, u64* leafm, u32* runhead, u32* leafslot, u64* tab,
                  u32 slots, const u32* clist, const u32* nclist, u32 cap) {
     const u32 per = o.leaf_cols * o.leaf_waves;
     const dim3 grid((units_cols + per - 1) / per);
     if (o.leaf_cols == 4 && o.leaf_waves == 1)
-        CPVS_LAUNCH("forest_leaf", (k_forest_leaf<4, 1>), grid, dim3(64), 0, s, colinfo, colbfs, items, nleaf, mask, leafm, hkey,
+        CPVS_LAUNCH("forest_leaf", (k_forest_leaf<4, 1>), grid, dim3(64), 0, s, colinfo, colbfs, items, nleaf, mask, leafm,
                     runhead, leafslot, tab, slots, clist, nclist, cap);
     else if (o.leaf_cols == 4)
         CPVS_LAUNCH("forest_leaf", (k_forest_leaf<4, 2>), grid, dim3(128), 0, s, colinfo, colbfs, items, nleaf, mask, leafm,
-                    hkey, runhead, leafslot, tab, slots, clist, nclist, cap);
+                    runhead, leafslot, tab, slots, clist, nclist, cap);
     else if (o.leaf_waves == 1)
-        CPVS_LAUNCH("forest_leaf", (k_forest_leaf<8, 1>), grid, dim3(64), 0, s, colinfo, colbfs, items, nleaf, mask, leafm, hkey,
+        CPVS_LAUNCH("forest_leaf", (k_forest_leaf<8, 1>), grid, dim3(64), 0, s, colinfo, colbfs, items, nleaf, mask, leafm,
                     runhead, leafslot, tab, slots, clist, nclist, cap);
     else
         CPVS_LAUNCH("forest_leaf", (k_forest_leaf<8, 2>), grid, dim3(128), 0, s, colinfo, colbfs, items, nleaf, mask, leafm,
-                    hkey, runhead, leafslot, tab, slots, clist, nclist, cap);
+                    runhead, leafslot, tab, slots, clist, nclist, cap);
 }
 
 // Pinned staging of one forest build.
@@ -636,7 +644,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
         const size_t c = C[L] ? C[L] : 1;
         need += (L > 2 ? arena_bytes(c, 8) + arena_bytes(c, 4) : 0) + 4 * arena_bytes(c, 4);  // no coords, starts at leaves
     }
-    need += arena_bytes(c2 * 8, 8) + arena_bytes(c2, 8) + 3 * arena_bytes(c2, 4);
+    need += arena_bytes(c2 * 8, 8) + 3 * arena_bytes(c2, 4);
     need += arena_bytes(ncol, 4) + arena_bytes(1, 4);
     need += 2 * arena_bytes(cmax, 4);
     if (use_table) need += arena_bytes(S, 8);
@@ -662,7 +670,6 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
         lv.off[L] = A.take<u32>(c);
     }
     u64* leafm = A.take<u64>(c2 * 8);
-    u64* hkey = A.take<u64>(c2);
     u32* runhead = A.take<u32>(c2);
     u32* leafslot = A.take<u32>(c2);
     u32* colbfs = A.take<u32>(c2);
@@ -719,7 +726,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
         const u32 slots2 = use_table ? leaf_table_slots((u32)C[2], small_leaf) : 0u;
         if (C[2]) {
             const u32 bound = (u32)std::min<u64>(ncol, C[2]);
-            launch_leaf(o, bound, s, colinfo, colbfs, items, (u32)C[2], lv.mask[2], leafm, hkey, runhead, leafslot, tab, slots2,
+            launch_leaf(o, bound, s, colinfo, colbfs, items, (u32)C[2], lv.mask[2], leafm, runhead, leafslot, tab, slots2,
                         clist, nclist, bound);
         }
         // Bottom-up merge over every level with nodes (roots included: a root is alone in its DAG).
@@ -741,7 +748,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
                 if (!is_leaf && prev < 0) CPVS_LAUNCH("table_clear", k_table_clear, grid_of(Sl, 256), dim3(256), 0, s, tab, Sl);
                 if (is_leaf) {
                     CPVS_LAUNCH("insert_leaf", k_insert_leaf, dim3(ceil_div<u32>(c, kLeafWindow)), dim3(kLeafInsertThreads), 0, s,
-                                hkey, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
+                                lv.mask[2], leafm, leafslot, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
                     CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid_of(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab,
                                 runhead, c, LeafEq{lv.mask[2], leafm, leafslot, nullptr}, rep, meta + kMetaColl, lv.mask[2], 1u,
                                 rpart, rrounds);
@@ -758,7 +765,8 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
                                           c, OpSum64{}, nullptr, rpart, s);
             } else {
                 if (is_leaf)
-                    CPVS_LAUNCH("sortkeys", k_sortkeys_leaf, grid_of(c, 256), dim3(256), 0, s, hkey, runhead, c, skey, sval);
+                    CPVS_LAUNCH("sortkeys", k_sortkeys_leaf, grid_of(c, 256), dim3(256), 0, s, lv.mask[2], leafm, leafslot, runhead, c,
+                                skey, sval);
                 else
                     CPVS_LAUNCH("sortkeys", k_sortkeys_inner, grid_of(c, 256), dim3(256), 0, s, lv.mask[L], lv.cstart[L],
                                 child_rank, c, skey, sval);
