@@ -543,6 +543,80 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
                          "requests_failed": {m: v["failed"] for m, v in press.items()}}}
 
 
+def run_busy_phases(cp, np, ctx, maps=256, kmin=8, kmax=13, config=None):
+    """For a kernel trace (rocprofv3 --kernel-trace): the batched configuration's 256 maps built
+    plainly (one map after another on one stream), then as one batch (forests), each phase after
+    a 0.3 s pause so that --busy-from can tell them apart.  Prints the host times."""
+    seed = SEED0 + 3
+    rng = splitmix64(seed)
+    sizes = [1 << (kmin + next(rng) % (kmax - kmin + 1)) for _ in range(maps)]
+    depths = [cp.gen_depth(N, seed + 1000 * (i + 1), max(1, (32 * N * N) // (1024 * 1024)), True)
+              for i, N in enumerate(sizes)]
+    ctx.synchronize()
+    cfg = config or cp.batch_config(lanes=4, forest_mtexels=1024)
+
+    def plain():
+        out = []
+        for d in depths:
+            mm = cp.MinMaxHierarchy(d, borrow=True)
+            out.append(cp.CompressedShadow.create(mm))
+            mm.close()
+        ctx.synchronize()
+        return out
+
+    def batch():
+        return cp.build_batch(depths, config=cfg)[0]
+
+    for f in (plain, batch):  # warm-up: arenas, lanes
+        for sh in f():
+            sh.close()
+    res = {}
+    for name, f in (("plain", plain), ("batched", batch)):
+        time.sleep(0.3)
+        t0 = time.perf_counter()
+        out = f()
+        res[name + "_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+        for sh in out:
+            sh.close()
+    time.sleep(0.3)
+    res["config"] = cfg.as_dict()
+    return res
+
+
+def busy_from(db_path, gap_ms=100.0):
+    """GPU-busy fraction of each phase of a rocprofv3 kernel trace (its SQLite results): phases are
+    separated by idle gaps of at least gap_ms; busy = the union of kernel intervals over the span
+    from a phase's first kernel start to its last kernel end."""
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    rows = con.execute("select start, end from kernels order by start").fetchall()
+    phases, cur = [], []
+    last_end = None
+    for st, en in rows:
+        if last_end is not None and st - last_end > gap_ms * 1e6:
+            phases.append(cur)
+            cur = []
+        cur.append((st, en))
+        last_end = en if last_end is None else max(last_end, en)
+    if cur:
+        phases.append(cur)
+    out = []
+    for ph in phases:
+        busy, s0, e0 = 0, None, None
+        for st, en in ph:  # union of intervals (sorted by start)
+            if e0 is None or st > e0:
+                if e0 is not None:
+                    busy += e0 - s0
+                s0, e0 = st, en
+            else:
+                e0 = max(e0, en)
+        busy += e0 - s0
+        span = max(en for _, en in ph) - ph[0][0]
+        out.append({"kernels": len(ph), "span_ms": round(span / 1e6, 3), "busy_ms": round(busy / 1e6, 3),
+                    "busy_fraction": round(busy / span, 3)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -560,7 +634,14 @@ def main():
     ap.add_argument("--no-c0", action="store_true", help="skip the 1024^2 configuration")
     ap.add_argument("--faithful-cap", type=float, default=20.0,
                     help="seconds the faithful (linear-search) create() may run at the headline map")
+    ap.add_argument("--busy-phases", action="store_true",
+                    help="only build the 256-map batch plainly and batched, paused apart, for a kernel trace")
+    ap.add_argument("--busy-from", metavar="RESULTS_DB",
+                    help="print the GPU-busy fraction of each phase of a rocprofv3 kernel trace, and exit")
     args = ap.parse_args()
+    if args.busy_from:
+        print(json.dumps(busy_from(args.busy_from)))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -574,6 +655,9 @@ def main():
     import cpvs_amd as cp
 
     ctx = cp.default_context()
+    if args.busy_phases:
+        print(json.dumps(run_busy_phases(cp, np, ctx)), flush=True)
+        return
     N = args.N
     log(f"[bench] generating {N}^2 depth map and {2 * args.width * args.height} queries on the device")
     depth = cp.gen_depth(N, SEED0 + 1, args.boxes, True)

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
     // only ever reads back its own entries, so the arrays need no barrier).
     __shared__ int s_f[kLeafWaves][kLeafCols][64];
     __shared__ u32 s_b[kLeafWaves][kLeafCols][64];
-    const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // The wave index is uniform across the wave; saying so keeps the column bases in scalar registers.
+    const u32 lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // Workgroups are dealt to the XCDs in turn; XCD x takes the contiguous units [x*unit_chunk, ...).
     // A column list fills only its first units, so there consecutive units go to different XCDs.
     const u32 unit = kList ? blockIdx.x : (blockIdx.x % kXcds) * unit_chunk + blockIdx.x / kXcds;
@@ -111,8 +112,30 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
     const u32 my_col = kList ? (nvalid ? col_list[my_slot] : 0u) : my_slot;
     uint4 ci = lc.info[my_col];
     if (lane >= kLeafCols || c0 + lane >= nvalid) ci = make_uint4(0u, 0u, 0u, 0u);
+    // Addresses are a wave-uniform base (the column's block, its run of indices) plus a per-lane
+    // offset computed once, so every load takes its address from scalar registers and one vector
+    // offset, with no per-column vector arithmetic.
+    // Byte offset of texel (lane & 7, lane >> 3) of a column's block: below 2^32 (N <= 2^21), so a
+    // load takes it as its 32-bit vector offset from the block's scalar base.
+    const u32 loff = 4u * ((lane >> 3) * N + (lane & 7));
     float d[kLeafCols];
-    if (T > 1) {
+    // Without a list, a wave's columns are c0 .. c0+7 of one row of level-3 texels when the row
+    // holds a multiple of 8 columns: their blocks are 32 bytes apart, one base and immediate offsets.
+    const bool row_run = !kList && lc.w3 >= kLeafCols && c0 + kLeafCols <= ncol;
+    if (row_run) {
+        const u32 x3 = c0 & (lc.w3 - 1), y3 = c0 >> lc.w3_shift;
+        const char* blk = reinterpret_cast<const char*>(depth + ((size_t)(8 * y3) * N + 8 * x3)) + loff;
+#pragma unroll
+        for (u32 k = 0; k < kLeafCols; ++k) {
+            const float* p = reinterpret_cast<const float*>(blk + 32 * k);
+            if (T > 1) {
+                d[k] = 0.0f;
+                if (__builtin_amdgcn_readlane(ci.x, k) != 0u) d[k] = *p;
+            } else {
+                d[k] = __builtin_nontemporal_load(p);
+            }
+        }
+    } else if (T > 1) {
         // A column with no leaf node in this z-tile (most columns of a tile) is not read: its f
         // values are never used.  With one z-tile every column has nodes, and the check would only
         // cost issue slots.
@@ -120,23 +143,26 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
         for (u32 k = 0; k < kLeafCols; ++k) {
             const u32 c = kList ? __builtin_amdgcn_readlane(my_col, k) : min(c0 + k, clast);
             const u32 x3 = c & (lc.w3 - 1), y3 = c >> lc.w3_shift;
+            const float* blk = depth + ((size_t)(8 * y3) * N + 8 * x3);
             d[k] = 0.0f;
-            if (__builtin_amdgcn_readlane(ci.x, k) != 0u) d[k] = depth[(size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7)];
+            if (__builtin_amdgcn_readlane(ci.x, k) != 0u) d[k] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(blk) + loff);
         }
     } else {
 #pragma unroll
         for (u32 k = 0; k < kLeafCols; ++k) {
             const u32 c = kList ? __builtin_amdgcn_readlane(my_col, k) : min(c0 + k, clast);
             const u32 x3 = c & (lc.w3 - 1), y3 = c >> lc.w3_shift;
+            const float* blk = depth + ((size_t)(8 * y3) * N + 8 * x3);
             // Nontemporal: the map's last read (the merge kernels that follow keep more of the L2).
-            d[k] = __builtin_nontemporal_load(depth + (size_t)(8 * y3 + (lane >> 3)) * N + 8 * x3 + (lane & 7));
+            d[k] = __builtin_nontemporal_load(reinterpret_cast<const float*>(reinterpret_cast<const char*>(blk) + loff));
         }
     }
     u32 b[kLeafCols];
 #pragma unroll
     for (u32 k = 0; k < kLeafCols; ++k) {
         const u32 cnt = __builtin_amdgcn_readlane(ci.x, k), st = __builtin_amdgcn_readlane(ci.y, k);
-        b[k] = lc.bfs[(lane < cnt) ? st + lane : 0u];  // st + lane < nleaf: the column counts sum to it
+        const u32* run = lc.bfs + (cnt ? st : 0u);  // st + lane < nleaf for lane < cnt: the counts sum to it
+        b[k] = run[(lane < cnt) ? lane : 0u];
     }
     const float HT = (float)(N * T);
     // The columns' minimum and maximum f from their level-3 pyramid texels (the 8x8 blocks' depth
@@ -203,7 +229,8 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
                     mine = (lane == sl) ? m : mine;
                 }
                 const u32 slot = base + q0 + j;  // < nleaf
-                if (lane < kp && slot < nleaf) leafm[(size_t)slot * 8 + lane] = mine;
+                u64* lslot = leafm + (size_t)slot * 8;
+                if (lane < kp && slot < nleaf) lslot[lane] = mine;
                 my_mask = (lane == j) ? mask : my_mask;
             }
             // Node q0 + lane's run head: the last changed node at or before it.
@@ -212,8 +239,8 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
             const u32 hb = (u32)__shfl((int)bfs, src, kWave);
             const u32 my_head_bfs = upto ? hb : head_bfs;
             const u32 my_head_q = upto ? q0 + (u32)src : head_q;
-            head_bfs = (u32)__shfl((int)my_head_bfs, (int)(qn - 1), kWave);
-            head_q = (u32)__shfl((int)my_head_q, (int)(qn - 1), kWave);
+            head_bfs = __builtin_amdgcn_readlane(my_head_bfs, qn - 1);
+            head_q = __builtin_amdgcn_readlane(my_head_q, qn - 1);
             // nleaf bounds every index (only exceeded if the column map were inconsistent).
             if (lane < qn && bfs < nleaf) {
                 runhead[bfs] = my_head_bfs;

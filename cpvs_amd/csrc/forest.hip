@@ -229,7 +229,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
                                                              u32 list_cap) {
     __shared__ int s_f[kWaves][kCols][64];
     __shared__ u32 s_b[kWaves][kCols][64];
-    const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const u32 lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 c0 = (blockIdx.x * kWaves + wave) * kCols;
     const u32 nvalid = min(*n_list, list_cap);
     const u32 clast = nvalid ? nvalid - 1 : 0u;
@@ -245,14 +245,18 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
         const u32 local = c - it.col_base;
         const u32 x3 = local & (it.w3 - 1), y3 = local >> it.w3_shift;
         HT[k] = (float)(it.N * it.T);
+        const float* blk = it.depth + ((size_t)(8 * y3) * it.N + 8 * x3);  // uniform base, per-lane offset
         d[k] = 0.0f;
-        if (__builtin_amdgcn_readlane(ci.x, k) != 0u) d[k] = it.depth[(size_t)(8 * y3 + (lane >> 3)) * it.N + 8 * x3 + (lane & 7)];
+        const u32 loff = 4u * ((lane >> 3) * it.N + (lane & 7));  // bytes, below 2^32 (N <= 65536)
+        if (__builtin_amdgcn_readlane(ci.x, k) != 0u)
+            d[k] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(blk) + loff);
     }
     u32 b[kCols];
 #pragma unroll
     for (u32 k = 0; k < kCols; ++k) {
         const u32 cnt = __builtin_amdgcn_readlane(ci.x, k), st = __builtin_amdgcn_readlane(ci.y, k);
-        b[k] = colbfs[(lane < cnt) ? st + lane : 0u];
+        const u32* run = colbfs + (cnt ? st : 0u);
+        b[k] = run[(lane < cnt) ? lane : 0u];
     }
     // The columns' minimum and maximum f from their DAGs' level-3 pyramid texels (lane k < kCols:
     // column k's minimum, lane k + kCols its maximum), or from the texels for a block with a NaN.
@@ -319,7 +323,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
                     mine = (lane == sl) ? m : mine;
                 }
                 const u32 slot = base + q0 + j;
-                if (lane < kp && slot < nleaf) leafm[(size_t)slot * 8 + lane] = mine;
+                u64* lslot = leafm + (size_t)slot * 8;
+                if (lane < kp && slot < nleaf) lslot[lane] = mine;
                 my_mask = (lane == j) ? mask : my_mask;
             }
             const u64 upto = chg & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
@@ -327,8 +332,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
             const u32 hb = (u32)__shfl((int)bfs, src, kWave);
             const u32 my_head_bfs = upto ? hb : head_bfs;
             const u32 my_head_q = upto ? q0 + (u32)src : head_q;
-            head_bfs = (u32)__shfl((int)my_head_bfs, (int)(qn - 1), kWave);
-            head_q = (u32)__shfl((int)my_head_q, (int)(qn - 1), kWave);
+            head_bfs = __builtin_amdgcn_readlane(my_head_bfs, qn - 1);
+            head_q = __builtin_amdgcn_readlane(my_head_q, qn - 1);
             if (lane < qn && bfs < nleaf) {
                 runhead[bfs] = my_head_bfs;
                 leafslot[bfs] = base + my_head_q;
