@@ -16,6 +16,7 @@ from __future__ import annotations
 import ctypes as C
 import json
 import os
+import weakref
 from typing import Optional, Sequence
 
 import numpy as np
@@ -36,6 +37,7 @@ class Context:
         call("cpvs_ctx_create", device, C.byref(h))
         self.handle = h
         self.device = device
+        self._objects = weakref.WeakSet()  # objects made on this context that are still open
 
     def synchronize(self):
         call("cpvs_ctx_synchronize", self.handle)
@@ -77,6 +79,17 @@ class Context:
         """'table' (default) or 'sort': how builds merge common subtrees (same words)."""
         call("cpvs_ctx_set_merge", self.handle, {"table": _lib.CPVS_MERGE_TABLE, "sort": _lib.CPVS_MERGE_SORT}[mode])
 
+    def set_leafmasks(self, on: bool):
+        """The reference's LEAFMASKS compile switch (src/CompressedShadow.cpp:17, shader/traverse.cs:8-9)
+        as a mode of this context, on by default.  Off, builds of maps of N >= 16 go down to level 0
+        and pyramids made afterwards keep levels 1 and 2; DAGs keep the mode they were built in."""
+        call("cpvs_ctx_set_leafmasks", self.handle, int(bool(on)))
+
+    def leafmasks(self) -> bool:
+        v = C.c_int()
+        call("cpvs_ctx_leafmasks", self.handle, C.byref(v))
+        return bool(v.value)
+
     def event(self, stream=None):
         e = C.c_void_p()
         call("cpvs_event_record", self.handle, stream, C.byref(e))
@@ -93,9 +106,21 @@ class Context:
         call("cpvs_event_destroy", e)
 
     def close(self):
+        """Destroys the context after closing every object still open on it (DAG sets and servers
+        first, then containers, DAGs, pyramids and device arrays): none of them may outlive it."""
         if self.handle:
+            live = list(self._objects)
+            for kind in (DagSet, Server, CompressedShadowContainer, CompressedShadow, MinMaxHierarchy, DeviceArray):
+                for o in live:
+                    if type(o) is kind:
+                        o.close()
             _lib.load().cpvs_ctx_destroy(self.handle)
             self.handle = None
+
+
+def _track(obj):
+    obj.ctx._objects.add(obj)
+    return obj
 
 
 _default: Optional[Context] = None
@@ -119,6 +144,7 @@ class DeviceArray:
         p = C.c_void_p()
         call("cpvs_device_alloc", self.ctx.handle, self.nbytes, C.byref(p))
         self.ptr = p
+        _track(self)
 
     @classmethod
     def from_numpy(cls, a: np.ndarray, ctx: Optional[Context] = None) -> "DeviceArray":
@@ -138,6 +164,8 @@ class DeviceArray:
         if self.ptr:
             call("cpvs_device_free", self.ctx.handle, self.ptr)
             self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:
@@ -171,6 +199,7 @@ class MinMaxHierarchy:
                  C.byref(h))
         self.handle = h
         self.N = N
+        _track(self)
 
     def getNumLevels(self) -> int:
         v = C.c_int()
@@ -224,6 +253,7 @@ class CompressedShadow:
     def __init__(self, handle, ctx: Context):
         self.handle = handle
         self.ctx = ctx
+        _track(self)
 
     @staticmethod
     def create(minMax: MinMaxHierarchy, zTileIndex: int = 0, zTileNum: int = 1, stream=None) -> "CompressedShadow":
@@ -268,6 +298,13 @@ class CompressedShadow:
         mode, coll = C.c_int(), C.c_uint64()
         call("cpvs_dag_merge_info", self._h(), C.byref(mode), C.byref(coll))
         return ("table" if mode.value == _lib.CPVS_MERGE_TABLE else "sort"), coll.value
+
+    def has_leafmasks(self) -> bool:
+        """True when level 2 holds leafmask nodes (useLeafmasks, src/CompressedShadow.cpp:20-27, in the
+        mode the DAG was built in)."""
+        v = C.c_int()
+        call("cpvs_dag_leafmasks", self._h(), C.byref(v))
+        return bool(v.value)
 
     def traverse_many(self, ndc: np.ndarray, tryLeafmasks: bool = True) -> np.ndarray:
         ndc = np.ascontiguousarray(np.asarray(ndc, np.float32).reshape(-1, 3))
@@ -350,6 +387,7 @@ class CompressedShadowContainer:
         call("cpvs_container_create", self.ctx.handle, length, C.byref(h))
         self.handle = h
         self.length = length
+        _track(self)
 
     def set(self, shadow: CompressedShadow, x: int, y: int, z: int):
         call("cpvs_container_set", self.handle, shadow._h(), x, y, z)
@@ -453,6 +491,7 @@ class DagSet:
         h = C.c_void_p()
         call("cpvs_dagset_create", self.ctx.handle, arr, len(self.shadows), C.byref(h))
         self.handle = h
+        _track(self)
 
     def __len__(self):
         return len(self.shadows)
@@ -493,6 +532,7 @@ class Server:
         h = C.c_void_p()
         call("cpvs_server_create", self.ctx.handle, lanes, budget_bytes, int(serial), C.byref(h))
         self.handle = h
+        _track(self)
 
     def build(self, depth: DeviceArray, zTileIndex: int = 0, zTileNum: int = 1) -> int:
         t = C.c_uint64()

@@ -38,6 +38,9 @@ _SIGS = {
     "cpvs_ctx_profile_read": [vp, C.c_char_p, u64],
     "cpvs_ctx_set_merge": [vp, i32],
     "cpvs_dag_merge_info": [vp, C.POINTER(i32), C.POINTER(u64)],
+    "cpvs_ctx_set_leafmasks": [vp, i32],
+    "cpvs_ctx_leafmasks": [vp, C.POINTER(i32)],
+    "cpvs_dag_leafmasks": [vp, C.POINTER(i32)],
     "cpvs_pyramid_build": [vp, vp, i32, u32, u32, vp, PP],
     "cpvs_pyramid_num_levels": [vp, C.POINTER(i32)],
     "cpvs_pyramid_get": [vp, u32, u32, u32, C.POINTER(C.c_float), C.POINTER(C.c_float)],

@@ -87,7 +87,7 @@ int build_pyramid(cpvs_ctx* ctx, Worker& w, const cpvs_batch_item& it, cpvs_pyra
     p->n = levels_of(it.N);
     p->depth = const_cast<float*>(it.depth);
     p->borrowed = true;
-    return pyramid_build_levels(p, false, w.stream, &w.levels);
+    return pyramid_build_levels(p, ctx->leafmasks.load() == 0, w.stream, &w.levels);
 }
 
 int ensure_lanes(cpvs_ctx* ctx, uint32_t lanes) {
@@ -418,10 +418,11 @@ int check_config(const cpvs_batch_config& c) {
     return CPVS_OK;
 }
 
+// Forests build leafmask DAGs only: without leafmasks the batch runs one map per launch chain.
 int run_config(cpvs_ctx* ctx, const cpvs_batch_item* items, uint32_t count, const cpvs_batch_config& c, cpvs_dag** out,
                double* done_ms) {
-    return c.forest_mtexels ? run_forests(ctx, items, count, c, out, done_ms)
-                            : run_batch(ctx, items, count, c.lanes, out, done_ms);
+    return (c.forest_mtexels && ctx->leafmasks.load()) ? run_forests(ctx, items, count, c, out, done_ms)
+                                                       : run_batch(ctx, items, count, c.lanes, out, done_ms);
 }
 
 }  // namespace
@@ -605,7 +606,9 @@ int cpvs_dagset_create(cpvs_ctx* ctx, const cpvs_dag* const* dags, uint32_t coun
         const cpvs_dag* d = dags[i];
         CHECK_ARG(d, "dagset_create: DAG %u is NULL", i);
         CHECK_ARG(d->ctx == ctx, "dagset_create: DAG %u belongs to another context", i);
-        CHECK_ARG(d->n >= 5, "dagset_create: DAG %u has %d levels; the lookup walks leafmask DAGs (>= 5)", i, d->n);
+        // One lookup for the whole set, as one shader (traverse.cs:8-9 compiles LEAFMASKS in or out).
+        CHECK_ARG(d->leafmasks == dags[0]->leafmasks, "dagset_create: DAG %u %s leafmasks, DAG 0 %s", i,
+                  d->leafmasks ? "has" : "has no", dags[0]->leafmasks ? "has" : "has none");
         CHECK_ARG(d->words < 0xFFFFFFFEull, "dagset_create: DAG %u exceeds 32-bit offsets", i);
         h[i] = dag_desc(d);
     }
@@ -650,7 +653,6 @@ int cpvs_dagset_destroy(cpvs_dagset* s) {
 int cpvs_dag_evaluate(const cpvs_dag* d, const float* ndc, uint32_t stride, uint64_t count, uint8_t* vis, void* stream) {
     CHECK_ARG(d && ((ndc && vis) || !count), "dag_evaluate: NULL argument");
     CHECK_ARG(stride >= 3, "dag_evaluate: stride %u < 3", stride);
-    CHECK_ARG(d->n >= 5, "dag_evaluate: the lookup walks leafmask DAGs (numLevels >= 5), got %d", d->n);
     CHECK_ARG(d->words < 0xFFFFFFFEull, "dag_evaluate: DAG exceeds 32-bit offsets");
     CPVS_HIP_CHECK(hipSetDevice(d->ctx->device));
     return launch_dagset_eval(nullptr, 0, dag_desc(d), nullptr, ndc, stride, count, vis, pick_stream(d->ctx, stream));

@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u32* _
         const u32 i = i0 + k * kLeafInsertThreads;
         const bool head = rh[k] == i;
         hm[k] = head ? mask[i] : 0u;
-        hs[k] = head ? leafslot[i] : 0u;
+        hs[k] = head ? min(leafslot[i], n - 1) : 0u;  // bounded as in LeafEq
     }
     u64 h[kLeafPer];
 #pragma unroll
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(256) void k_sortkeys_leaf(const u32* __restrict__ m
     if (i >= n) return;
     u32 h = runhead[i];  // the key is the run's first node's
     h = (h < n) ? h : i;
-    skey[i] = fold32(leaf_key_hash(mask[h], leafm + (size_t)leafslot[h] * 8));
+    skey[i] = fold32(leaf_key_hash(mask[h], leafm + (size_t)min(leafslot[h], n - 1) * 8));  // bounded as in LeafEq
     sval[i] = i;
 }
 
@@ -436,6 +436,7 @@ struct WriteLevel {
     const u32* off;
     const u32* child_rank;  // null at the deepest level
     const u32* child_off;
+    u32 child_n;            // nodes of the child level
     u32 U, base, child_base, leaf;
     int L;                  // the octree level (k_write_levels_dev reads U and the bases of meta)
 };
@@ -445,6 +446,7 @@ struct WriteArgs {
     WriteLevel lv[kMaxLevels];
     const u64* leafm;
     const u32* leafslot;
+    u32 nleaf;  // nodes of the leaf level (at least 1)
 };
 
 __global__ __launch_bounds__(kNodeThreads) void k_write_levels(WriteArgs a, u32* __restrict__ out) {
@@ -454,9 +456,10 @@ __global__ __launch_bounds__(kNodeThreads) void k_write_levels(WriteArgs a, u32*
     const u32 r = (blockIdx.x - a.first[k]) * blockDim.x + threadIdx.x;
     if (r >= l.U) return;
     if (l.leaf)
-        write_leaf_node(r, l.uniq, l.mask, a.leafm, a.leafslot, l.off, l.base, out);
+        write_leaf_node(r, l.uniq, l.mask, a.leafm, a.leafslot, a.nleaf, l.off, l.base, out);
     else
-        write_inner_node(r, l.uniq, l.mask, l.cstart, l.off, l.base, l.child_rank, l.child_off, l.child_base, out);
+        write_inner_node(r, l.uniq, l.mask, l.cstart, l.off, l.base, l.child_rank, l.child_off, l.child_base, l.child_n,
+                         out);
 }
 
 // The same with the unique counts and level bases read from the metadata block, so that it runs
@@ -491,9 +494,10 @@ __global__ __launch_bounds__(kNodeThreads) void k_write_levels_dev(WriteArgs a, 
         // is not written.
         if ((u64)s_base[k] + l.off[r] + 17 > cap) continue;
         if (l.leaf)
-            write_leaf_node(r, l.uniq, l.mask, a.leafm, a.leafslot, l.off, s_base[k], out);
+            write_leaf_node(r, l.uniq, l.mask, a.leafm, a.leafslot, a.nleaf, l.off, s_base[k], out);
         else
-            write_inner_node(r, l.uniq, l.mask, l.cstart, l.off, s_base[k], l.child_rank, l.child_off, s_cbase[k], out);
+            write_inner_node(r, l.uniq, l.mask, l.cstart, l.off, s_base[k], l.child_rank, l.child_off, s_cbase[k],
+                             l.child_n, out);
     }
 }
 
@@ -814,6 +818,7 @@ inline dim3 grid1d(u32 n, u32 block = 256) { return dim3(ceil_div<u32>(n ? n : 1
 
 constexpr int kRetryWithSort = -1000;
 constexpr int kRetryFullTable = -1001;
+constexpr int kRetryExactCounts = -1002;  // the counts exceed the emitted nodes (a map with NaN depths)
 static_assert(sizeof(u32) * kMetaWords <= sizeof(u64) * kPinnedWords, "meta readback fits the pinned staging");
 
 
@@ -838,10 +843,10 @@ __global__ __launch_bounds__(256) void k_copy_words(const u32* __restrict__ src,
 }
 
 int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s, int merge_mode,
-               cpvs_dag** out, const Admit* admit, bool small_leaf, DagOut* into) {
+               cpvs_dag** out, const Admit* admit, bool small_leaf, bool exact, DagOut* into) {
     const int n = p->n;
     const int R = n - 2;
-    const bool leaf = n >= 5;
+    const bool leaf = n >= 5 && ctx->leafmasks != 0;  // useLeafmasks (src/CompressedShadow.cpp:20-27)
     const int bottom = leaf ? 2 : 0;
     const u32 N = p->N;
     const bool use_table = merge_mode == CPVS_MERGE_TABLE;
@@ -855,7 +860,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     if (rc) return rc;
     uint4* colinfo = leaf ? A.take<uint4>(ncol) : nullptr;
     CountLayout lay;
-    rc = pyramid_node_counts(p, t, T, pinned_partials(pinned), colinfo, s, &lay);
+    rc = pyramid_node_counts(p, t, T, leaf, pinned_partials(pinned), colinfo, s, &lay, exact);
     if (rc) return rc;
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
     u64 cnt[kMaxLevels] = {};
@@ -901,6 +906,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         d->zTileIndex = t;
         d->zTileNum = T;
         d->merge_mode = merge_mode;
+        d->leafmasks = leaf;
         d->nodes[R] = 1;
         d->uniques[R] = 1;
         *out = d;
@@ -951,7 +957,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     colinfo = leaf ? A.take<uint4>(ncol) : nullptr;
     u32* meta = A.take<u32>(kMetaWords);
     if (leaf && moved) {  // the arena was reallocated: count the columns again
-        rc = pyramid_node_counts(p, t, T, nullptr, colinfo, s);
+        rc = pyramid_node_counts(p, t, T, leaf, nullptr, colinfo, s, nullptr, exact);
         if (rc) return rc;
     }
     // meta is zeroed by the first kernel below (k_small_topdown or k_start).
@@ -1105,7 +1111,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
                 CPVS_LAUNCH("insert_leaf", k_insert_leaf, dim3(ceil_div<u32>(c, kLeafWindow)), dim3(kLeafInsertThreads), 0, s,
                             lv[2].mask, leafm, leafslot, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
                 CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, runhead, c,
-                            LeafEq{lv[2].mask, leafm, leafslot, nullptr}, rep, meta + kMetaColl, lv[2].mask, 1u, rpart, rrounds);
+                            LeafEq{lv[2].mask, leafm, leafslot, nullptr, c}, rep, meta + kMetaColl, lv[2].mask, 1u, rpart, rrounds);
             } else {
                 CPVS_LAUNCH("insert_inner", k_insert_inner, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, lv[L].mask,
                             lv[L].cstart, child_rank, c,
@@ -1124,7 +1130,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             device_scan(SegHeadIn{skey}, SegStartOut{seg}, c, OpMax{}, nullptr, scan_scratch, s);
             if (is_leaf)
                 CPVS_LAUNCH("representative", k_representative<LeafEq>, grid1d(c), dim3(256), 0, s, sval, seg, c,
-                            LeafEq{lv[2].mask, leafm, leafslot, runhead}, rep, meta + kMetaSortColl);
+                            LeafEq{lv[2].mask, leafm, leafslot, runhead, c}, rep, meta + kMetaSortColl);
             else
                 CPVS_LAUNCH("representative", k_representative<InnerEq>, grid1d(c), dim3(256), 0, s, sval, seg, c,
                             InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaSortColl);
@@ -1175,12 +1181,14 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         w.off = lv[L].off;
         w.child_rank = kids ? lv[L - 1].rank : nullptr;
         w.child_off = kids ? lv[L - 1].off : nullptr;
+        w.child_n = kids ? std::max<u32>(lv[L - 1].count, 1u) : 1u;
         w.leaf = (leaf && L == 2) ? 1u : 0u;
         w.L = L;
         wmax += ceil_div<u32>(lv[L].count ? lv[L].count : 1, kNodeThreads);
     }
     wa.leafm = leafm;
     wa.leafslot = leafslot;
+    wa.nleaf = (u32)c2;
     if (staged)
         CPVS_LAUNCH("write_levels", k_write_levels_dev, dim3(std::min<u32>(wmax, 2048)), dim3(kNodeThreads), 0, s, wa, meta,
                     staging, bound_words + 17);
@@ -1189,14 +1197,16 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     u32 hm[kMetaWords];
     std::memcpy(hm, pinned, sizeof(hm));
 
-    if (use_table && hm[kMetaFull] != 0) return small_leaf ? kRetryFullTable : kRetryWithSort;
-    if (use_table && hm[kMetaColl] != 0) return kRetryWithSort;
+    // Counts first: a level whose count exceeds its emitted children merged unwritten nodes too.
     for (int L = deepest + 1; L <= R; ++L) {
         if (hm[kMetaC + L] != lv[L - 1].count) {
+            if (!exact) return kRetryExactCounts;  // cells under a NaN texel were counted: count exactly
             set_error("build: level %d emitted %u children, expected %u", L, hm[kMetaC + L], lv[L - 1].count);
             return CPVS_EINTERNAL;
         }
     }
+    if (use_table && hm[kMetaFull] != 0) return small_leaf ? kRetryFullTable : kRetryWithSort;
+    if (use_table && hm[kMetaColl] != 0) return kRetryWithSort;
     if (deepest > bottom && hm[kMetaC + deepest] != 0) {
         set_error("build: level %d emitted children below the deepest level", deepest);
         return CPVS_EINTERNAL;
@@ -1257,6 +1267,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     d->zTileNum = T;
     d->merge_mode = merge_mode;
     d->key_collisions = hm[kMetaSortColl];
+    d->leafmasks = leaf;
     for (int L = 0; L < n && L < kMaxLevels; ++L) {
         d->nodes[L] = (L >= bottom && L <= R) ? cnt[L] : 0;
         d->uniques[L] = (L >= deepest && L <= R) ? hm[kMetaU + L] : 0;
@@ -1267,11 +1278,24 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
 
 }  // namespace
 
+// A build that finds its merge table too small runs again with a full-size table, one with a key
+// collision again with the sort merge, and one whose counts exceed the emitted nodes again with
+// exact counts; each switch is made at most once.
 int build_dag(cpvs_ctx* ctx, Arena& A, u64* pinned, int merge_mode, const cpvs_pyramid* p, u32 t, u32 T, hipStream_t s,
               cpvs_dag** out, const Admit* admit, DagOut* into) {
-    int rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out, admit, true, into);
-    if (rc == kRetryFullTable) rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out, admit, false, into);
-    if (rc == kRetryWithSort) rc = build_once(ctx, A, pinned, p, t, T, s, CPVS_MERGE_SORT, out, admit, false, into);
+    bool small_leaf = true, exact = false;
+    int rc = CPVS_OK;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        rc = build_once(ctx, A, pinned, p, t, T, s, merge_mode, out, admit, small_leaf, exact, into);
+        if (rc == kRetryFullTable && small_leaf) small_leaf = false;
+        else if (rc == kRetryWithSort && merge_mode != CPVS_MERGE_SORT) merge_mode = CPVS_MERGE_SORT, small_leaf = false;
+        else if (rc == kRetryExactCounts && !exact) exact = true;
+        else break;
+    }
+    if (rc == kRetryFullTable || rc == kRetryWithSort || rc == kRetryExactCounts) {
+        set_error("build: no consistent build after every fallback (status %d)", rc);
+        return CPVS_EINTERNAL;
+    }
     return rc;
 }
 

@@ -30,13 +30,15 @@ __device__ inline u32 fmix32(u32 h) {
 // ---------------------------------------------------------------------------------------
 // 1. Inner classification: createChildmask (src/CompressedShadowUtil.cpp:20-54).
 // ---------------------------------------------------------------------------------------
+// The cell is clamped into the level: a node past a level's emitted children (counts that
+// exceed them, which the host then reports) still reads inside the pyramid.
 __device__ inline u32 classify_node(u64 c, const float2* __restrict__ lv, const float* __restrict__ depth, u32 N, int L,
                                     u32 T) {
-    const u32 x = unpack_x(c), y = unpack_y(c), z = unpack_z(c);
+    const u32 w = N >> L;
+    const u32 x = min(unpack_x(c), w - 2) & ~1u, y = min(unpack_y(c), w - 2), z = unpack_z(c);
     const float h = (float)((N >> L) * T);
     float mn[4], mx[4];  // texel (x + (q&1), y + (q>>1))
     if (L > 0) {
-        const u32 w = N >> L;
         const float4 a = *reinterpret_cast<const float4*>(lv + (size_t)y * w + x);
         const float4 b = *reinterpret_cast<const float4*>(lv + (size_t)(y + 1) * w + x);
         mn[0] = a.x; mx[0] = a.y; mn[1] = a.z; mx[1] = a.w;
@@ -147,13 +149,16 @@ struct EmitChildrenOut {
         // getChildCoordinates (src/CompressedShadowUtil.cpp:101-117): partial children in
         // ascending child index, child cell doubled.
         if (cols.bfs) {
+            // (the column records read lie inside the level even for a node past its level's
+            // emitted children)
+            const u32 xc = min(x, cols.w3 - 2) & ~1u, yc = min(y, cols.w3 - 2);
             // Leaf children: children ch and ch + 4 lie in one column, so the node's (at most
             // four) column records are loaded first, all together, and the stores follow.
             uint4 ci[4];
 #pragma unroll
             for (u32 q = 0; q < 4; ++q) {
                 ci[q] = make_uint4(0u, 0u, 0u, 0u);
-                if (m & ((2u << (2 * q)) | (2u << (2 * (q + 4))))) ci[q] = cols.info[(y + (q >> 1)) * cols.w3 + x + (q & 1)];
+                if (m & ((2u << (2 * q)) | (2u << (2 * (q + 4))))) ci[q] = cols.info[(yc + (q >> 1)) * cols.w3 + xc + (q & 1)];
             }
 #pragma unroll
             for (u32 ch = 0; ch < 8; ++ch) {
@@ -257,20 +262,25 @@ __device__ inline u64 inner_key_hash(const u32* mask, const u32* cstart, const u
     return h;
 }
 
+// Run heads and leafmask slots are bounded by the level's n nodes where they are used as indices:
+// past a level's emitted children (counts that exceed them, which the host then reports) the
+// entries were never written.
 struct LeafEq {
     const u32* mask;
     const u64* leafm;     // indexed by leafmask slot
     const u32* leafslot;  // node -> leafmask slot of its run's first node
     const u32* runhead;   // when set, a node's mask is read from its run's first node
+    u32 n;                // nodes of the leaf level
     __device__ bool operator()(u32 a, u32 b) const {
         if (runhead) {
-            a = runhead[a];
-            b = runhead[b];
+            const u32 ha = runhead[a], hb = runhead[b];
+            a = (ha < n) ? ha : a;
+            b = (hb < n) ? hb : b;
             if (a == b) return true;
         }
         // Masks and leafmask slots are loaded together (one round trip before the leafmasks).
         const u32 m = mask[a], mb = mask[b];
-        const u32 sa = leafslot[a], sb = leafslot[b];
+        const u32 sa = min(leafslot[a], n - 1), sb = min(leafslot[b], n - 1);
         if (m != mb) return false;
         const u32 k = num_children(m);
         const u64* pa = leafm + (size_t)sa * 8;
@@ -573,10 +583,12 @@ enum : int {
 
 
 // Unique node r of an inner level: [mask, absolute offsets of its partial children].
+// child_n (the child level's nodes) bounds the child ranks read as indices: ranks past a level's
+// emitted children were never written (the host then rejects the build's words).
 __device__ inline void write_inner_node(u32 r, const u32* __restrict__ uniq, const u32* __restrict__ mask,
                                         const u32* __restrict__ cstart, const u32* __restrict__ off, u32 base,
                                         const u32* __restrict__ child_rank, const u32* __restrict__ child_off, u32 child_base,
-                                        u32* __restrict__ out) {
+                                        u32 child_n, u32* __restrict__ out) {
     const u32 i = uniq[r];
     const u32 o = base + off[r];
     const u32 m = mask[i];
@@ -587,7 +599,7 @@ __device__ inline void write_inner_node(u32 r, const u32* __restrict__ uniq, con
     load_child_ranks(cstart, child_rank, i, k, cr);
     u32 co[8];
 #pragma unroll
-    for (u32 j = 0; j < 8; ++j) co[j] = (j < k) ? child_off[cr[j]] : 0u;
+    for (u32 j = 0; j < 8; ++j) co[j] = (j < k) ? child_off[min(cr[j], child_n - 1)] : 0u;
 #pragma unroll
     for (u32 j = 0; j < 8; ++j)
         if (j < k) out[o + 1 + j] = child_base + co[j];
@@ -595,14 +607,14 @@ __device__ inline void write_inner_node(u32 r, const u32* __restrict__ uniq, con
 
 // Unique leaf node r: [mask, (lo, hi) of each partial slice's leafmask].
 __device__ inline void write_leaf_node(u32 r, const u32* __restrict__ uniq, const u32* __restrict__ mask,
-                                       const u64* __restrict__ leafm, const u32* __restrict__ leafslot,
+                                       const u64* __restrict__ leafm, const u32* __restrict__ leafslot, u32 nleaf,
                                        const u32* __restrict__ off, u32 base, u32* __restrict__ out) {
     const u32 i = uniq[r];
     const u32 o = base + off[r];
     const u32 m = mask[i];
     out[o] = m;
     const u32 k = num_children(m);
-    const u64* p = leafm + (size_t)leafslot[i] * 8;
+    const u64* p = leafm + (size_t)min(leafslot[i], nleaf - 1) * 8;  // bounded as in LeafEq
     u64 v[8];
 #pragma unroll
     for (u32 j = 0; j < 8; ++j) v[j] = (j < k) ? p[j] : 0ull;

@@ -321,6 +321,25 @@ int cpvs_ctx_set_merge(cpvs_ctx* ctx, int mode) {
     return CPVS_OK;
 }
 
+int cpvs_ctx_set_leafmasks(cpvs_ctx* ctx, int on) {
+    CHECK_ARG(ctx, "ctx_set_leafmasks: NULL ctx");
+    CHECK_ARG(on == 0 || on == 1, "ctx_set_leafmasks: %d (0 or 1)", on);
+    ctx->leafmasks.store(on);
+    return CPVS_OK;
+}
+
+int cpvs_ctx_leafmasks(const cpvs_ctx* ctx, int* on) {
+    CHECK_ARG(ctx && on, "ctx_leafmasks: NULL argument");
+    *on = ctx->leafmasks.load();
+    return CPVS_OK;
+}
+
+int cpvs_dag_leafmasks(const cpvs_dag* d, int* on) {
+    CHECK_ARG(d && on, "dag_leafmasks: NULL argument");
+    *on = d->leafmasks ? 1 : 0;
+    return CPVS_OK;
+}
+
 int cpvs_dag_merge_info(const cpvs_dag* d, int* mode, uint64_t* key_collisions) {
     CHECK_ARG(d, "dag_merge_info: NULL dag");
     if (mode) *mode = d->merge_mode;
@@ -373,7 +392,8 @@ int cpvs_pyramid_build(cpvs_ctx* ctx, const float* depth, int depth_on_device, u
             return hip_fail(e, "pyramid depth copy", __FILE__, __LINE__);
         }
     }
-    int rc = pyramid_build_levels(p, (flags & CPVS_PYR_ALL_LEVELS) != 0, s);
+    // Without leafmasks the octree reads levels 1 and 2 as well.
+    int rc = pyramid_build_levels(p, (flags & CPVS_PYR_ALL_LEVELS) != 0 || ctx->leafmasks.load() == 0, s);
     // A borrowed device depth map on the context's own stream needs no wait here: builds on that
     // stream run after the levels, and everything else waits for them (pyramid_wait).  A copied
     // input is waited for, so the caller may reuse its buffer, and so is a caller's stream, which
@@ -476,11 +496,12 @@ int cpvs_estimate_build(cpvs_ctx* ctx, const cpvs_pyramid* p, uint32_t t, uint32
     hipStream_t s = ctx->stream;
     CPVS_HIP_CHECK(pyramid_wait(p, s));
     CountLayout lay;
-    rc = pyramid_node_counts(p, t, T, pinned_partials(ctx->pinned), nullptr, s, &lay);
+    const int n = p->n, R = n - 2;
+    const bool leafm = n >= 5 && ctx->leafmasks.load() != 0;
+    rc = pyramid_node_counts(p, t, T, leafm, pinned_partials(ctx->pinned), nullptr, s, &lay);
     if (rc) return rc;
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
-    const int n = p->n, R = n - 2;
-    const int bottom = (n >= 5) ? 2 : 0;
+    const int bottom = leafm ? 2 : 0;
     u64 cnt[kMaxLevels] = {};
     sum_node_counts(lay, pinned_partials(ctx->pinned), cnt);
     cnt[R] = 1;
@@ -562,11 +583,11 @@ int cpvs_dag_device_words(const cpvs_dag* d, const uint32_t** dptr) {
 
 int cpvs_dag_traverse(cpvs_dag* d, const float* ndc, uint64_t count, int try_leaf, int* vis) {
     CHECK_ARG(d && (ndc || !count) && (vis || !count), "dag_traverse: NULL argument");
-    // Leafmask DAGs (numLevels >= 5) are only walkable with leafmasks and plain ones without
-    // (src/CompressedShadow.cpp:20-27,404-463); the other combinations read leaf payload as
-    // pointers in the reference.
-    CHECK_ARG((try_leaf != 0) == (d->n >= 5), "dag_traverse: tryLeafmasks=%d invalid for a DAG of %d levels", try_leaf,
-              d->n);
+    // Leafmask DAGs are only walkable with leafmasks and plain ones (numLevels < 5, or built without
+    // leafmasks) without (src/CompressedShadow.cpp:20-27,404-463); the other combinations read leaf
+    // payload as pointers in the reference.
+    CHECK_ARG((try_leaf != 0) == d->leafmasks, "dag_traverse: tryLeafmasks=%d invalid for a DAG %s leafmasks", try_leaf,
+              d->leafmasks ? "with" : "without");
     if (!count) return CPVS_OK;
     GUARD_BEGIN
     cpvs_ctx* ctx = d->ctx;
@@ -684,11 +705,16 @@ int cpvs_container_copy_to_gpu(cpvs_container* c) {
             set_error("container_copy_to_gpu: DAG %zu has %d levels, DAG 0 has %d", i, d->n, dl);
             return CPVS_EINVAL;
         }
+        // One lookup walks every cell (traverse.cs:8-9 compiles LEAFMASKS in or out).
+        if (d->leafmasks != c->dags[0]->leafmasks) {
+            set_error("container_copy_to_gpu: DAG %zu %s leafmasks, DAG 0 %s", i, d->leafmasks ? "has" : "has no",
+                      c->dags[0]->leafmasks ? "has" : "has none");
+            return CPVS_EINVAL;
+        }
         // createTopLevelGrid (src/CompressedShadowContainer.cpp:71-91).
         grid[i] = (d->root_mask == 0u) ? 0xFFFFFFFFu : (d->root_mask == 0x5555u) ? 0xFFFFFFFEu : (u32)total;
         total += d->words;
     }
-    CHECK_ARG(dl >= 5, "container: the lookup walks leafmask DAGs (numLevels >= 5), got %d", dl);
     CHECK_ARG(total < 0xFFFFFFFEull, "container: %llu combined words exceed 32-bit offsets", (unsigned long long)total);
     cpvs_ctx* ctx = c->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -715,6 +741,7 @@ int cpvs_container_copy_to_gpu(cpvs_container* c) {
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
     c->dag_words = total;
     c->dag_levels = dl;
+    c->leafmasks = c->dags[0]->leafmasks;
     int gl = 0;
     while ((1u << gl) < c->length) ++gl;
     c->grid_levels = gl;  // (uint)log8(length^3) (src/CompressedShadowContainer.cpp:42)
@@ -782,7 +809,7 @@ int cpvs_container_evaluate(cpvs_container* c, const float* pos, uint32_t stride
     }
     CPVS_HIP_CHECK(hipSetDevice(c->ctx->device));
     ProfInstall prof(c->ctx->prof_on ? &c->ctx->prof : nullptr);
-    return launch_evaluate(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, pos, stride, count,
+    return launch_evaluate(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, c->leafmasks, pos, stride, count,
                            to_mat(m), vis, pick_stream(c->ctx, stream));
 }
 
@@ -798,7 +825,7 @@ int cpvs_container_evaluate_pcf(cpvs_container* c, const float* pos, uint32_t st
     }
     CPVS_HIP_CHECK(hipSetDevice(c->ctx->device));
     ProfInstall prof(c->ctx->prof_on ? &c->ctx->prof : nullptr);
-    return launch_evaluate_pcf(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, pos, stride, count,
+    return launch_evaluate_pcf(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, c->leafmasks, pos, stride, count,
                                to_mat(m), c->filter_size, vis, pick_stream(c->ctx, stream));
 }
 
@@ -811,7 +838,7 @@ int cpvs_container_evaluate_image(cpvs_container* c, const float* pos, uint32_t 
     }
     CPVS_HIP_CHECK(hipSetDevice(c->ctx->device));
     ProfInstall prof(c->ctx->prof_on ? &c->ctx->prof : nullptr);
-    return launch_evaluate_image(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, pos, width, height,
+    return launch_evaluate_image(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, c->leafmasks, pos, width, height,
                                  to_mat(m), vis, pick_stream(c->ctx, stream));
 }
 

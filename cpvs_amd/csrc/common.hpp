@@ -51,14 +51,43 @@ __device__ inline u32 classify_mid(u32 z, float d_h) {
 // to the slab in float before the conversion: a NaN bound fails its compare and leaves that side
 // open (as it does in the classification), infinities and huge values clip.  *first = the first
 // partial cell (the slab's start when none).
-__device__ inline u32 partial_cells(float2 v, float h, int zlo, int zhi, int* first) {
-    const float lo = floorf(v.x * h), hi = ceilf(v.y * h);
+__device__ inline u32 clip_cells(float lo, float hi, int zlo, int zhi, int* first) {
     const float fz0 = (float)zlo, fz1 = (float)zhi;
     const float slo = (lo > fz0) ? ((lo < fz1) ? lo : fz1) : fz0;
     const float shi = (hi < fz1) ? ((hi > fz0) ? hi : fz0) : fz1;
     const int s = (int)slo, e = (int)shi;
     *first = s;
     return (e > s) ? (u32)(e - s) : 0u;
+}
+__device__ inline u32 partial_cells(float2 v, float h, int zlo, int zhi, int* first) {
+    return clip_cells(floorf(v.x * h), ceilf(v.y * h), zlo, zhi, first);
+}
+
+// Interval bounds where NaN means open (no bound): the tighter of a and b.
+__device__ inline float open_max(float a, float b) { return (b > a || a != a) ? b : a; }
+__device__ inline float open_min(float a, float b) { return (b < a || a != a) ? b : a; }
+
+// The cells of texel (x, y) of level L that are nodes of the octree: partial, and under a partial
+// cell at every level above, up to the root's children at level R (lvl[M] = level M, M in
+// [L, R]).  A cell at level L+d lies over the cells [2^d z, 2^d (z+1)) of level L, so the
+// reachable cells stay one interval: the texel's own partial interval cut by each ancestor's,
+// scaled.  Without NaN this equals partial_cells (a parent's minimum is at most, and its maximum
+// at least, its children's); a NaN, which the pyramid keeps only from a block's first texel
+// (std::min/max order, src/MinMaxHierarchy.cpp:29-49), makes a cell partial at every z while its
+// parent's range may be narrower.
+__device__ inline u32 reachable_cells(const float2* const* lvl, u32 x, u32 y, int L, int R, u32 N, u32 T, int zlo,
+                                      int zhi, int* first) {
+    const float2 v = lvl[L][(size_t)y * (N >> L) + x];
+    const float h = (float)((N >> L) * T);
+    float lo = floorf(v.x * h), hi = ceilf(v.y * h);
+    for (int M = L + 1; M <= R; ++M) {
+        const int d = M - L;
+        const float2 a = lvl[M][(size_t)(y >> d) * (N >> M) + (x >> d)];
+        const float hm = (float)((N >> M) * T), sc = (float)(1u << d);
+        lo = open_max(lo, floorf(a.x * hm) * sc);
+        hi = open_min(hi, ceilf(a.y * hm) * sc);
+    }
+    return clip_cells(lo, hi, zlo, zhi, first);
 }
 
 // 64-bit mixers for subtree keys (Stafford variant 13 finaliser of splitmix64).

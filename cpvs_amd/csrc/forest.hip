@@ -63,9 +63,11 @@ struct FCountRec {
 constexpr u32 kFCountPer = 8;
 constexpr u32 kFCountTile = 256 * kFCountPer;
 
+// exact: only the cells under a partial cell at every level above (reachable_cells; maps with NaN
+// depths, whose plain counts can exceed the emitted nodes).
 __global__ __launch_bounds__(256) void k_forest_counts(const FCountRec* __restrict__ recs, u32 nrec,
                                                        const FItem* __restrict__ items, u32* __restrict__ partials,
-                                                       uint4* __restrict__ colinfo, u32 root_slot0) {
+                                                       uint4* __restrict__ colinfo, u32 root_slot0, u32 exact) {
     __shared__ u32 s_wave[4];
     u32 lo = 0, hi = nrec;  // recs[lo].block0 <= blockIdx.x < recs[hi].block0
     while (hi - lo > 1) {
@@ -88,7 +90,8 @@ __global__ __launch_bounds__(256) void k_forest_counts(const FCountRec* __restri
         const u32 j = base + k * 256 + threadIdx.x;
         if (j < w * w) {
             int s;
-            const u32 c = partial_cells(lv[j], h, zlo, zhi, &s);
+            const u32 c = exact ? reachable_cells(it.lv, j % w, j / w, L, (int)it.R, it.N, it.T, zlo, zhi, &s)
+                                : partial_cells(lv[j], h, zlo, zhi, &s);
             if (colc) colc[j] = make_uint4(c, 0u, (u32)max(s, 0), r.item);
             acc += c;
         }
@@ -135,24 +138,28 @@ __global__ void k_forest_roots(const FItem* __restrict__ items, u32 M, const u32
 }
 
 // createChildmask of a forest node: the single build's classification on the node's own pyramid,
-// with the DAG's index in the mask's upper half.
-__device__ inline u32 fclassify(u64 c, int L, const FItem* __restrict__ items) {
-    const u32 g = fg(c);
+// with the DAG's index in the mask's upper half.  The index is clamped to the forest's M items (as
+// classify_node clamps the cell): a node past its level's emitted children (counts that exceed
+// them, which the host then reports) still reads inside the forest's arrays.
+__device__ inline u32 fclassify(u64 c, int L, const FItem* __restrict__ items, u32 M) {
+    const u32 g = min(fg(c), M - 1);
     const FItem& it = items[g];
     return classify_node(pack_xyz(fx(c), fy(c), fz(c)), it.lv[L], it.depth, it.N, L, it.T) | (g << 16);
 }
 
 __global__ __launch_bounds__(256) void k_forest_classify(const u64* __restrict__ coord, u32 count,
-                                                         const FItem* __restrict__ items, int L, u32* __restrict__ mask) {
+                                                         const FItem* __restrict__ items, u32 M, int L,
+                                                         u32* __restrict__ mask) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) mask[i] = fclassify(coord[i], L, items);
+    if (i < count) mask[i] = fclassify(coord[i], L, items, M);
 }
 
 struct FClassify {
     const u64* coord;
     const FItem* items;
+    u32 M;
     int L;
-    __device__ u32 operator()(u32 i) const { return fclassify(coord[i], L, items); }
+    __device__ u32 operator()(u32 i) const { return fclassify(coord[i], L, items, M); }
 };
 struct FClassifyCountIn {
     FClassify cls;
@@ -169,6 +176,7 @@ struct FEmitOut {
     u64* child_coord;       // inner children
     u32 child_cap;          // emitted nodes of the child level; writes beyond it are dropped
     const FItem* items;
+    u32 M;                  // items in the forest
     const uint4* colinfo;   // leaf children: column records ...
     u32* colbfs;            // ... and (column slot -> breadth-first index), when set
     u32* mask_out = nullptr;  // when set, the node is classified here and its mask stored
@@ -184,16 +192,17 @@ struct FEmitOut {
         if (!k) return;
         const u32 m = pre.m;
         const u64 c = pre.c;
-        const u32 x = fx(c), y = fy(c), z = fz(c), g = fg(c);
+        const u32 x = fx(c), y = fy(c), z = fz(c), g = min(fg(c), M - 1);
         u32 j = first;
         if (colbfs) {
             const FItem& it = items[g];
             const uint4* info = colinfo + it.col_base;
+            const u32 xc = min(x, it.w3 - 2) & ~1u, yc = min(y, it.w3 - 2);  // inside the DAG's columns
             uint4 ci[4];
 #pragma unroll
             for (u32 q = 0; q < 4; ++q) {
                 ci[q] = make_uint4(0u, 0u, 0u, 0u);
-                if (m & ((2u << (2 * q)) | (2u << (2 * (q + 4))))) ci[q] = info[(y + (q >> 1)) * it.w3 + x + (q & 1)];
+                if (m & ((2u << (2 * q)) | (2u << (2 * (q + 4))))) ci[q] = info[(yc + (q >> 1)) * it.w3 + xc + (q & 1)];
             }
 #pragma unroll
             for (u32 ch = 0; ch < 8; ++ch) {
@@ -372,10 +381,13 @@ __global__ void k_forest_bases(const FItem* __restrict__ items, u32 M, FLevels l
             const u32 s = seg[k];
             u32 S = 0, U = 0, o0 = 0;
             if (s != kNone) {
-                const u32 r0 = lv.rank[L][s];
+                // Ranks bounded by the level's unique nodes: a rank past the level's emitted children
+                // was never written (the host then rejects the forest's words).
+                const u32 Ul = meta[kMetaU + L], rmax = Ul ? Ul - 1 : 0u;
+                const u32 r0 = min(lv.rank[L][s], rmax);
                 o0 = lv.off[L][r0];
                 const u32 e = nxt[k];
-                const u32 r1 = (e != kNone) ? lv.rank[L][e] : meta[kMetaU + L];
+                const u32 r1 = (e != kNone) ? min(lv.rank[L][e], rmax) : Ul;
                 const u32 o1 = (e != kNone) ? lv.off[L][r1] : meta[kMetaS + L];
                 S = o1 - o0;
                 U = r1 - r0;
@@ -455,6 +467,7 @@ inline dim3 grid_of(u64 n, u32 block) { return dim3((u32)std::max<u64>(1, (n + b
 
 constexpr int kRetrySort = -1010;
 constexpr int kRetryFull = -1011;
+constexpr int kRetryExact = -1012;  // the counts exceed the emitted nodes (a map with NaN depths)
 
 // Launches the leaf kernel in the shape the options name.
 void launch_leaf(const ForestOpts& o, u32 units_cols, hipStream_t s, const uint4* colinfo, const u32* colbfs,
@@ -495,7 +508,7 @@ T* carve(char*& p, size_t n) {
 }
 
 int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const ForestOpts& o, int merge_mode, bool small_leaf,
-                cpvs_dag** out) {
+                bool exact, cpvs_dag** out) {
     const hipStream_t s = w.stream;
     const bool use_table = merge_mode == CPVS_MERGE_TABLE;
 
@@ -585,7 +598,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
         CPVS_HIP_CHECK(hipMemcpyAsync(items, st.items, sizeof(FItem) * M, hipMemcpyHostToDevice, s));
         CPVS_HIP_CHECK(hipMemcpyAsync(drecs, st.recs, sizeof(FCountRec) * recs.size(), hipMemcpyHostToDevice, s));
         CPVS_LAUNCH("forest_counts", k_forest_counts, dim3(nblocks), dim3(256), 0, s, drecs, (u32)recs.size(), items, partials,
-                    colinfo, nblocks);
+                    colinfo, nblocks, exact ? 1u : 0u);
         CPVS_HIP_CHECK(hipGetLastError());
         return CPVS_OK;
     };
@@ -716,15 +729,15 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
             if (!c) continue;
             const bool leaf_kids = (L == 3);
             FEmitOut emit{lv.mask[L], lv.coord[L], lv.cstart[L], leaf_kids ? nullptr : lv.coord[L - 1],
-                          (u32)(leaf_kids ? C[2] : E[L - 1]), items, colinfo, leaf_kids ? colbfs : nullptr};
+                          (u32)(leaf_kids ? C[2] : E[L - 1]), items, M, colinfo, leaf_kids ? colbfs : nullptr};
             if (scan_prefetches(c)) {
-                const FClassify cls{lv.coord[L], items, L};
+                const FClassify cls{lv.coord[L], items, M, L};
                 emit.mask_out = lv.mask[L];
                 emit.cls = cls;
                 device_scan(FClassifyCountIn{cls}, emit, c, OpSum{}, meta + kMetaC + L, scan_scratch, s);
             } else {
-                CPVS_LAUNCH("forest_classify", k_forest_classify, grid_of(c, 256), dim3(256), 0, s, lv.coord[L], c, items, L,
-                            lv.mask[L]);
+                CPVS_LAUNCH("forest_classify", k_forest_classify, grid_of(c, 256), dim3(256), 0, s, lv.coord[L], c, items, M,
+                            L, lv.mask[L]);
                 device_scan(ChildCountIn{lv.mask[L]}, emit, c, OpSum{}, meta + kMetaC + L, scan_scratch, s);
             }
         }
@@ -755,7 +768,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
                     CPVS_LAUNCH("insert_leaf", k_insert_leaf, dim3(ceil_div<u32>(c, kLeafWindow)), dim3(kLeafInsertThreads), 0, s,
                                 lv.mask[2], leafm, leafslot, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
                     CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid_of(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab,
-                                runhead, c, LeafEq{lv.mask[2], leafm, leafslot, nullptr}, rep, meta + kMetaColl, lv.mask[2], 1u,
+                                runhead, c, LeafEq{lv.mask[2], leafm, leafslot, nullptr, c}, rep, meta + kMetaColl, lv.mask[2], 1u,
                                 rpart, rrounds);
                 } else {
                     CPVS_LAUNCH("insert_inner", k_insert_inner, grid_of(c, kInsertBlock), dim3(kInsertBlock), 0, s, lv.mask[L],
@@ -779,7 +792,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
                 device_scan(SegHeadIn{skey}, SegStartOut{segs}, c, OpMax{}, nullptr, scan_scratch, s);
                 if (is_leaf)
                     CPVS_LAUNCH("representative", k_representative<LeafEq>, grid_of(c, 256), dim3(256), 0, s, sval, segs, c,
-                                LeafEq{lv.mask[2], leafm, leafslot, runhead}, rep, meta + kMetaSortColl);
+                                LeafEq{lv.mask[2], leafm, leafslot, runhead, c}, rep, meta + kMetaSortColl);
                 else
                     CPVS_LAUNCH("representative", k_representative<InnerEq>, grid_of(c, 256), dim3(256), 0, s, sval, segs, c,
                                 InnerEq{lv.mask[L], lv.cstart[L], child_rank}, rep, meta + kMetaSortColl);
@@ -817,16 +830,18 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
     if (e != hipSuccess) return hip_fail(e, "forest level chain", __FILE__, __LINE__);
 
     const u32* hm = st.back;
-    if (use_table && hm[kMetaFull] != 0) return small_leaf ? kRetryFull : kRetrySort;
-    if (use_table && hm[kMetaColl] != 0) return kRetrySort;
+    // Counts first: a level whose count exceeds its emitted children merged unwritten nodes too.
     for (int L = std::max(Lmin, 3); L <= Rmax; ++L) {
         if (!C[L]) continue;
         const u64 want = (L == 3) ? C[2] : E[L - 1];
         if (hm[kMetaC + L] != want) {
+            if (!exact) return kRetryExact;  // cells under a NaN texel were counted: count exactly
             set_error("forest: level %d emitted %u children, expected %llu", L, hm[kMetaC + L], (unsigned long long)want);
             return CPVS_EINTERNAL;
         }
     }
+    if (use_table && hm[kMetaFull] != 0) return small_leaf ? kRetryFull : kRetrySort;
+    if (use_table && hm[kMetaColl] != 0) return kRetrySort;
     const u32* totals = st.back + kMetaWords;
     const u32* roots = totals + M;
     const u32* uq = roots + M;
@@ -885,6 +900,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
         d->zTileNum = x.T;
         d->merge_mode = merge_mode;
         d->key_collisions = hm[kMetaSortColl];
+        d->leafmasks = true;
         const int R = (int)hi[f].R;
         for (int L = 0; L < kMaxLevels; ++L) {
             d->nodes[L] = (L >= 2 && L <= R) ? cnt[(size_t)f * kKL + L] : 0;
@@ -901,6 +917,10 @@ int build_forest(cpvs_ctx* ctx, Worker& w, const ForestIn* items, u32 M, const F
     for (u32 i = 0; i < M; ++i) out[i] = nullptr;
     if (!M) return CPVS_OK;
     if (M > kMaxForest) return kForestTooLarge;
+    if (!ctx->leafmasks.load()) {
+        set_error("forest: forests build leafmask DAGs; the context builds without leafmasks");
+        return CPVS_EINVAL;
+    }
     for (u32 i = 0; i < M; ++i) {
         const cpvs_pyramid* p = items[i].p;
         if (p->n < 5 || p->N > 65536 || (u64)p->N * items[i].T > (1ull << 21) || items[i].t >= items[i].T) {
@@ -914,9 +934,21 @@ int build_forest(cpvs_ctx* ctx, Worker& w, const ForestIn* items, u32 M, const F
                 return CPVS_EINTERNAL;
             }
     }
-    int rc = forest_once(ctx, w, items, M, o, o.merge_mode, true, out);
-    if (rc == kRetryFull) rc = forest_once(ctx, w, items, M, o, o.merge_mode, false, out);
-    if (rc == kRetrySort) rc = forest_once(ctx, w, items, M, o, CPVS_MERGE_SORT, false, out);
+    // As build_dag: a full table, a key collision and counts above the emitted nodes each switch to
+    // their fallback once.
+    int mode = o.merge_mode, rc = CPVS_OK;
+    bool small_leaf = true, exact = false;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        rc = forest_once(ctx, w, items, M, o, mode, small_leaf, exact, out);
+        if (rc == kRetryFull && small_leaf) small_leaf = false;
+        else if (rc == kRetrySort && mode != CPVS_MERGE_SORT) mode = CPVS_MERGE_SORT, small_leaf = false;
+        else if (rc == kRetryExact && !exact) exact = true;
+        else break;
+    }
+    if (rc == kRetryFull || rc == kRetrySort || rc == kRetryExact) {
+        set_error("forest: no consistent build after every fallback (status %d)", rc);
+        return CPVS_EINTERNAL;
+    }
     return rc;
 }
 

@@ -14,7 +14,8 @@
 //       24     4  zTileIndex
 //       28     4  zTileNum
 //       32     4  root mask (= word 0)
-//       36     4  flags (0)
+//       36     4  flags: bit 0 = no leafmasks (a DAG of numLevels >= 5 built down to level 0,
+//                  the reference without LEAFMASKS, src/CompressedShadow.cpp:17); other bits 0
 //       40     8  word count W
 //       48     8  checksum of the words (below)
 //       56     8  reserved (0)
@@ -51,6 +52,10 @@ static_assert(sizeof(FileHeader) == 64, "DAG file header is 64 bytes");
 
 constexpr char kMagic[8] = {'C', 'P', 'V', 'S', 'D', 'A', 'G', '\0'};
 constexpr uint32_t kVersion = 1;
+constexpr uint32_t kFlagNoLeafmasks = 1u;
+
+// useLeafmasks (src/CompressedShadow.cpp:20-27) unless the DAG was built without them.
+bool leaf_dag(const FileHeader& h) { return h.num_levels >= 5 && !(h.flags & kFlagNoLeafmasks); }
 
 uint64_t words_checksum(const uint32_t* w, uint64_t n) {
     uint64_t h = 0x243F6A8885A308D3ull;
@@ -69,9 +74,8 @@ uint64_t words_checksum(const uint32_t* w, uint64_t n) {
 
 // Decodes the DAG level by level (root at word 0, levels contiguous, every child pointer the
 // start of a node of the next level, every node below the root referenced, nothing after the
-// last level).  nodes[L] receives the node count of level L.
-int check_structure(const uint32_t* w, uint64_t W, int n, uint64_t* nodes) {
-    const bool leaf = n >= 5;
+// last level).  nodes[L] receives the node count of level L.  leaf: level 2 holds leafmask nodes.
+int check_structure(const uint32_t* w, uint64_t W, int n, bool leaf, uint64_t* nodes) {
     const int R = n - 2, bottom = leaf ? 2 : 0;
     for (int L = 0; L < kMaxLevels; ++L) nodes[L] = 0;
     if (W == 0) {
@@ -169,6 +173,7 @@ int cpvs_dag_serialize(const cpvs_dag* d, void* buf, uint64_t cap, uint64_t* siz
     h.z_tile_index = d->zTileIndex;
     h.z_tile_num = d->zTileNum;
     h.root_mask = d->root_mask;
+    h.flags = (d->n >= 5 && !d->leafmasks) ? kFlagNoLeafmasks : 0u;
     h.words = d->words;
     h.checksum = words_checksum(w, d->words);
     std::memcpy(buf, &h, sizeof(h));
@@ -190,6 +195,7 @@ int cpvs_dag_check(const void* buf, uint64_t size, uint32_t* num_levels, uint64_
     CHECK_ARG(h.z_tile_num >= 1 && h.z_tile_index < h.z_tile_num, "dag_check: z tile %u of %u", h.z_tile_index,
               h.z_tile_num);
     CHECK_ARG(h.words >= 1 && h.words < 0xFFFFFFFEull, "dag_check: %llu words", (unsigned long long)h.words);
+    CHECK_ARG((h.flags & ~kFlagNoLeafmasks) == 0, "dag_check: unknown flags 0x%x", h.flags);
     CHECK_ARG(size == sizeof(FileHeader) + 4 * h.words, "dag_check: %llu bytes for %llu words", (unsigned long long)size,
               (unsigned long long)h.words);
     GUARD_BEGIN
@@ -199,7 +205,7 @@ int cpvs_dag_check(const void* buf, uint64_t size, uint32_t* num_levels, uint64_
     CHECK_ARG(words_checksum(w.data(), h.words) == h.checksum, "dag_check: checksum mismatch");
     CHECK_ARG(w[0] == h.root_mask, "dag_check: root mask 0x%x, header says 0x%x", w[0], h.root_mask);
     uint64_t nodes[kMaxLevels];
-    const int rc = check_structure(w.data(), h.words, (int)h.num_levels, nodes);
+    const int rc = check_structure(w.data(), h.words, (int)h.num_levels, leaf_dag(h), nodes);
     if (rc) return rc;
     if (num_levels) *num_levels = h.num_levels;
     if (words) *words = h.words;
@@ -239,6 +245,7 @@ int cpvs_dag_deserialize(cpvs_ctx* ctx, const void* buf, uint64_t size, cpvs_dag
     d->N = h.N;
     d->zTileIndex = h.z_tile_index;
     d->zTileNum = h.z_tile_num;
+    d->leafmasks = leaf_dag(h);
     for (int L = 0; L < kMaxLevels; ++L) d->uniques[L] = nodes[L];  // pre-merge counts are not stored
     *out = d;
     return CPVS_OK;

@@ -2,6 +2,7 @@
 // each translation unit provides.
 #pragma once
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -130,6 +131,10 @@ struct cpvs_ctx {
     bool prof_on = false;
     cpvs::Prof prof;
     int merge_mode = CPVS_MERGE_TABLE;
+    // The reference's LEAFMASKS switch (src/CompressedShadow.cpp:17, shader/traverse.cs:8-9), on by
+    // default: builds of maps of numLevels >= 5 end in 17-word leafmask nodes at level 2; off, they
+    // go down to level 0 (cpvs_ctx_set_leafmasks).
+    std::atomic<int> leafmasks{1};
     std::vector<cpvs::Worker*> lanes;  // batch build lanes, created on first use and kept
 };
 
@@ -156,6 +161,7 @@ struct cpvs_dag {
     uint64_t uniques[cpvs::kMaxLevels] = {};  // nodes per level after merging
     int merge_mode = CPVS_MERGE_TABLE;        // merge path that produced the words
     uint32_t key_collisions = 0;              // 32-bit sort-key collisions resolved by scanning
+    bool leafmasks = false;                   // level 2 holds leafmask nodes (lookups stop there)
 };
 
 struct cpvs_container {
@@ -167,6 +173,7 @@ struct cpvs_container {
     uint64_t dag_words = 0;
     uint32_t* d_grid = nullptr;  // length^3 entries
     int dag_levels = 0, grid_levels = 0;
+    bool leafmasks = true;       // the DAGs' mode (all alike)
     bool on_gpu = false;
     bool freed_cpu = false;
 };
@@ -193,8 +200,12 @@ struct CountLayout {
     int level[kMaxLevels] = {};            // pyramid level of each block range
     u32 block_start[kMaxLevels + 1] = {};  // first block of each range
 };
-int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, u32* partials, uint4* col_info, hipStream_t s,
-                        CountLayout* layout = nullptr);
+// leaf: the octree ends in leafmask nodes at level 2 (counts from level 3 up) rather than at level 0
+// (counts from level 1 up, which needs pyramid levels 1 and 2).
+// exact: count only the cells under a partial cell at every level above (slower; needed only
+// for maps with NaN depths, whose plain counts can exceed the octree's nodes).
+int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, bool leaf, u32* partials, uint4* col_info, hipStream_t s,
+                        CountLayout* layout = nullptr, bool exact = false);
 void sum_node_counts(const CountLayout& lay, const u32* partials, u64* cnt);
 
 // Admission hook of a build: called once the exact node counts are known and before any scratch
@@ -235,20 +246,21 @@ int noalloc_setup(int device);
 struct Mat4 {
     float m[16];
 };
-int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 stride, u64 count,
-                    const Mat4& m, uint8_t* vis, hipStream_t s);
-int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 stride, u64 count,
-                        const Mat4& m, u32 f, float* vis, hipStream_t s);
-int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 width,
-                          u32 height, const Mat4& m, uint8_t* vis, hipStream_t s);
+// leaf: the DAGs end in leafmask nodes at level 2; 0: they go down to level 0.
+int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos, u32 stride,
+                    u64 count, const Mat4& m, uint8_t* vis, hipStream_t s);
+int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos, u32 stride,
+                        u64 count, const Mat4& m, u32 f, float* vis, hipStream_t s);
+int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos,
+                          u32 width, u32 height, const Mat4& m, uint8_t* vis, hipStream_t s);
 // One DAG of a set, walked as a one-cell container.  base is 0, or the grid sentinel
 // 0xFFFFFFFF / 0xFFFFFFFE when the root is all shadow / all lit.
 struct DagDesc {
     const u32* words;
     u32 nwords;
     u32 base;
-    u32 dl;  // numLevels
-    u32 pad;
+    u32 dl;    // numLevels
+    u32 leaf;  // 1: leafmask nodes at level 2; 0: the DAG goes down to level 0
 };
 // The descriptor of a built DAG (createTopLevelGrid sentinels for a one-cell grid,
 // src/CompressedShadowContainer.cpp:71-91).
@@ -258,6 +270,7 @@ inline DagDesc dag_desc(const cpvs_dag* d) {
     r.nwords = (u32)d->words;
     r.base = (d->root_mask == 0u) ? 0xFFFFFFFFu : (d->root_mask == 0x5555u) ? 0xFFFFFFFEu : 0u;
     r.dl = (u32)d->n;
+    r.leaf = d->leafmasks ? 1u : 0u;
     return r;
 }
 constexpr uint8_t kBadId = 0xFF;  // visibility written for a query whose DAG id is outside the set

@@ -17,6 +17,7 @@ struct GridDag {
     const u32* __restrict__ grid;
     int dl, gl;
     u32 words;  // DAG word count: every read is bounds-checked against it
+    u32 leaf;   // 1: leafmask nodes at level 2; 0: the DAG goes down to level 0
 };
 
 // Workgroups are dealt to the XCDs in turn (block b to XCD b mod kXcds), each XCD with its own L2.
@@ -44,19 +45,27 @@ __device__ inline u32 quantise(float v, u32 R) {
 }
 
 // traverse.cs:89-133: the walk below one grid cell whose DAG starts at word `base` of `dag`
-// (pointers inside a DAG are relative to its first word, combineDAGs keeps them so).
-__device__ inline u32 walk_dag(const u32* __restrict__ dag, u32 words, int dl, u32 base, u32 px, u32 py, u32 pz) {
+// (pointers inside a DAG are relative to its first word, combineDAGs keeps them so).  With leaf
+// the inner levels end at 3 and level 2 holds leafmask nodes (MIN_LEVEL 3, traverse.cs:34-38);
+// without, the inner levels go down to level 0, whose children are single voxels, lit or shadow
+// (absoluteVisible, src/CompressedShadowUtil.cpp:34-40), never partial; a walk that passed level 0
+// would end visible (traverse.cs:130-132).
+__device__ inline u32 walk_dag(const u32* __restrict__ dag, u32 words, int dl, u32 leaf, u32 base, u32 px, u32 py,
+                               u32 pz) {
     u32 off = base;
-    for (int level = dl - 2; level >= 3; --level) {
+    const int last = leaf ? 3 : 0;
+    for (int level = dl - 2; level >= last; --level) {
         const u32 ci = (((px >> level) & 1u) << 1) | (((py >> level) & 1u) << 2) | (((pz >> level) & 1u) << 3);
         if (off >= words) return 0u;
         const u32 m = dag[off];
         const u32 v = (m >> ci) & 3u;
         if (v != 2u) return v;  // 0 shadow, 1 visible (3 never occurs)
+        if (level == 0) return 1u;  // a partial voxel does not occur
         const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
         if (off + 1 + co >= words) return 0u;
         off = base + dag[off + 1 + co];
     }
+    if (!leaf) return 1u;
     if (off >= words) return 0u;
     const u32 ci = (pz & 7u) * 2u;
     const u32 m = dag[off];
@@ -77,7 +86,7 @@ __device__ inline u32 walk(const GridDag& g, u32 px, u32 py, u32 pz) {
     const u32 base = g.grid[((pz >> sh) * G + (py >> sh)) * G + (px >> sh)];
     if (base == 0xFFFFFFFFu) return 0u;
     if (base == 0xFFFFFFFEu) return 1u;
-    return walk_dag(g.dag, g.words, g.dl, base, px, py, pz);
+    return walk_dag(g.dag, g.words, g.dl, g.leaf, base, px, py, pz);
 }
 
 // traverse.cs:141-144: p = lightViewProj * (pos, 1); p /= p.w; then the path.
@@ -127,10 +136,10 @@ __global__ __launch_bounds__(kLookupThreads) void k_evaluate_pcf(GridDag g, Mat4
     vis[i] = (float)lit / (float)(f * f);
 }
 
-int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 stride, u64 count,
-                        const Mat4& m, u32 f, float* vis, hipStream_t s) {
+int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos, u32 stride,
+                        u64 count, const Mat4& m, u32 f, float* vis, hipStream_t s) {
     if (count == 0) return CPVS_OK;
-    GridDag g{dag, grid, dl, gl, words};
+    GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u};
     const u64 blocks = (count + kLookupThreads - 1) / kLookupThreads;
     if (blocks > 0x7FFFFFF0ull) {
         set_error("evaluate_pcf: %llu points exceed one launch", (unsigned long long)count);
@@ -177,10 +186,10 @@ __global__ __launch_bounds__(256) void k_evaluate_image(GridDag g, Mat4 M, const
     vis[i] = (uint8_t)lookup_point(g, M, q.x, q.y, q.z);
 }
 
-int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 stride, u64 count,
-                    const Mat4& m, uint8_t* vis, hipStream_t s) {
+int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos, u32 stride,
+                    u64 count, const Mat4& m, uint8_t* vis, hipStream_t s) {
     if (count == 0) return CPVS_OK;
-    GridDag g{dag, grid, dl, gl, words};
+    GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u};
     const u64 blocks = (count + kLookupThreads - 1) / kLookupThreads;
     if (blocks > 0x7FFFFFF0ull) {
         set_error("evaluate: %llu points exceed one launch", (unsigned long long)count);
@@ -196,10 +205,10 @@ int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, 
     return CPVS_OK;
 }
 
-int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, const float* pos, u32 width,
-                          u32 height, const Mat4& m, uint8_t* vis, hipStream_t s) {
+int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos,
+                          u32 width, u32 height, const Mat4& m, uint8_t* vis, hipStream_t s) {
     if (width == 0 || height == 0) return CPVS_OK;
-    GridDag g{dag, grid, dl, gl, words};
+    GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u};
     const u64 gx = ceil_div<u32>(width, 32), nblocks = gx * ceil_div<u32>(height, 8);
     if (nblocks > 0x7FFFFFF0ull) {
         set_error("evaluate_image: %ux%u pixels exceed one launch", width, height);
@@ -283,7 +292,7 @@ __device__ inline uint8_t dagset_lookup(const DagDesc& d, const float* __restric
     const u32 px = quantise(q[0], R), py = quantise(q[1], R), pz = quantise(q[2], R);
     if (d.base == 0xFFFFFFFFu) return 0;
     if (d.base == 0xFFFFFFFEu) return 1;
-    return (uint8_t)walk_dag(d.words, d.nwords, (int)d.dl, 0u, px, py, pz);
+    return (uint8_t)walk_dag(d.words, d.nwords, (int)d.dl, d.leaf, 0u, px, py, pz);
 }
 
 // Query i names its DAG by ids[i] (or uses `one` when set is null).

@@ -211,9 +211,12 @@ int pyramid_build_levels(cpvs_pyramid* p, bool store_all, hipStream_t s, Arena* 
 // ------------------------------------------------------------------------------------------
 // Exact node counts.  A cell at level L is partial iff float(z+1) > mn*h and float(z) < mx*h
 // (src/CompressedShadowUtil.h:35-43 with h = (N>>L)*T, src/CompressedShadowUtil.cpp:16-18),
-// i.e. z in [floor(mn*h), ceil(mx*h)).  A partial cell implies a partial parent cell, so the
-// nodes of level L-1 are exactly the partial level-L cells inside the z-tile's slab
-// [t*(N>>L), (t+1)*(N>>L)).
+// i.e. z in [floor(mn*h), ceil(mx*h)).  Without NaN a partial cell implies a partial parent
+// cell, so the nodes of level L-1 are exactly the partial level-L cells inside the z-tile's slab
+// [t*(N>>L), (t+1)*(N>>L)).  A NaN texel can be partial where its parent's cell is not; such a
+// map's counts exceed what the classification emits, the build sees the difference after its
+// readback, and counts again with `exact`, which cuts every cell by its ancestors' cells
+// (reachable_cells).
 // ------------------------------------------------------------------------------------------
 
 struct CountArgs {
@@ -226,6 +229,9 @@ struct CountArgs {
     u32* partials;               // partials[block] = the block's partial cells (host memory; may be null)
     uint4* col_info;             // per level-3 texel (leaf column): x = partial cells, z = first partial z
                                  // (y is left for the column's start); may be null
+    const float2* by_level[kMaxLevels];  // exact: level M's texels, M in [level[0], R]
+    int R;
+    u32 exact;                   // count only cells under partial cells at every level above
 };
 
 constexpr u32 kCountPerThread = 8;
@@ -247,7 +253,8 @@ __global__ __launch_bounds__(256) void k_node_counts(CountArgs a) {
         const u32 j = base + r * 256 + threadIdx.x;
         if (j < w * w) {
             int s;
-            const u32 c = partial_cells(a.lv[k][j], h, zlo, zhi, &s);
+            const u32 c = a.exact ? reachable_cells(a.by_level, j % w, j / w, L, a.R, a.N, a.T, zlo, zhi, &s)
+                                  : partial_cells(a.lv[k][j], h, zlo, zhi, &s);
             if (colc) colc[j] = make_uint4(c, 0u, (u32)max(s, 0), 0u);
             acc += c;
         }
@@ -285,17 +292,21 @@ void sum_node_counts(const CountLayout& lay, const u32* partials, u64* cnt) {
     }
 }
 
-int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, u32* partials, uint4* col_info, hipStream_t s,
-                        CountLayout* layout) {
+int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, bool leaf, u32* partials, uint4* col_info, hipStream_t s,
+                        CountLayout* layout, bool exact) {
     CountArgs a = {};
     const int n = p->n;
     const int R = n - 2;
-    const int Lmin = (n >= 5) ? 3 : 1;
+    leaf = leaf && n >= 5;
+    const int Lmin = leaf ? 3 : 1;
     a.N = p->N;
     a.T = T;
     a.t = t;
+    a.R = R;
+    a.exact = exact ? 1u : 0u;
+    for (int L = Lmin; L <= R; ++L) a.by_level[L] = p->lv[L];
     a.partials = partials;
-    a.col_info = (n >= 5) ? col_info : nullptr;
+    a.col_info = leaf ? col_info : nullptr;
     u32 blocks = 0;
     for (int L = Lmin; L <= R; ++L) {
         const u32 w = p->N >> L;
@@ -304,7 +315,8 @@ int pyramid_node_counts(const cpvs_pyramid* p, u32 t, u32 T, u32* partials, uint
         a.level[a.nlev] = L;
         a.lv[a.nlev] = p->lv[L];
         if (!p->lv[L]) {
-            set_error("node counts: pyramid level %d not stored", L);
+            set_error("node counts: pyramid level %d not stored (a build without leafmasks needs a pyramid made with "
+                      "every level: create it in that mode or with all levels)", L);
             return CPVS_EINTERNAL;
         }
         blocks += ceil_div<u32>(w * w, kCountTile);

@@ -207,9 +207,11 @@ int run_build(cpvs_server* s, uint32_t li, Req* r) {
     AdmitState a{s, li, 0};
     int rc = CPVS_OK;
     // The lane's pyramid-level arena grows before the build's admission: reserve that growth first.
-    const size_t lv_bytes = pyramid_levels_bytes(r->N, false);
+    // Without leafmasks the octree reads pyramid levels 1 and 2 as well.
+    const bool all_levels = s->ctx->leafmasks.load() == 0;
+    const size_t lv_bytes = pyramid_levels_bytes(r->N, all_levels);
     if (lv_bytes > w.levels.cap) rc = reserve_bytes(&a, lv_bytes + (lv_bytes >> 3) - w.levels.cap);
-    if (rc == CPVS_OK) rc = pyramid_build_levels(&p, false, w.stream, &w.levels);
+    if (rc == CPVS_OK) rc = pyramid_build_levels(&p, all_levels, w.stream, &w.levels);
     if (rc == CPVS_OK && a.reserved) {
         std::lock_guard<std::mutex> lk(s->mu);  // the growth is lane scratch from here on
         s->reserved_bytes -= a.reserved;

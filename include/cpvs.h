@@ -38,7 +38,7 @@ extern "C" {
 #define CPVS_MERGE_SORT 1  /* LSD radix sort of (32-bit key hash, node index) */
 
 /* cpvs_pyramid_build flags. */
-#define CPVS_PYR_ALL_LEVELS 1u   /* also store levels 1 and 2, which the build never reads */
+#define CPVS_PYR_ALL_LEVELS 1u   /* also store levels 1 and 2 (read only by builds without leafmasks) */
 #define CPVS_PYR_BORROW_DEPTH 2u /* depth is device memory the caller keeps alive; not copied */
 
 typedef struct cpvs_ctx cpvs_ctx;
@@ -65,6 +65,16 @@ int cpvs_ctx_profile(cpvs_ctx *ctx, int mode);
  * different keys that the table path cannot tell apart (equal hash tags in one probe chain) are
  * detected by the full-key compare, and the build is redone with the sort path. */
 int cpvs_ctx_set_merge(cpvs_ctx *ctx, int mode);
+/* The reference's compile-time LEAFMASKS switch (#define LEAFMASKS, src/CompressedShadow.cpp:17;
+ * shader/traverse.cs:8-9, MIN_LEVEL 34-38), as a run-time mode of the context, on (1) by default.
+ * Off (0), builds of maps of numLevels >= 5 go down to level 0 with 1+k-word nodes (useLeafmasks
+ * false, src/CompressedShadow.cpp:20-41), pyramids made afterwards keep levels 1 and 2, and
+ * forests are not used (batches build map by map).  Every DAG records the mode it was built in;
+ * lookups, traversals and the file format follow the DAG's mode, and a container or DAG set takes
+ * DAGs of one mode only.  Set it before builds start: a build racing a change may fail with an
+ * error. */
+int cpvs_ctx_set_leafmasks(cpvs_ctx *ctx, int on);
+int cpvs_ctx_leafmasks(const cpvs_ctx *ctx, int *on);
 int cpvs_ctx_profile_read(cpvs_ctx *ctx, char *buf, uint64_t cap);
 
 /* MinMaxHierarchy::MinMaxHierarchy(const ImageF&) (src/MinMaxHierarchy.h:23,
@@ -118,6 +128,8 @@ int cpvs_dag_info(const cpvs_dag *d, uint32_t *num_levels, uint64_t *num_words, 
 int cpvs_dag_level_stats(const cpvs_dag *d, uint64_t *nodes, uint64_t *uniques, int cap);
 /* Merge path that produced the DAG and the 32-bit sort-key collisions it resolved. */
 int cpvs_dag_merge_info(const cpvs_dag *d, int *mode, uint64_t *key_collisions);
+/* 1 when level 2 holds leafmask nodes (numLevels >= 5 built with leafmasks), else 0. */
+int cpvs_dag_leafmasks(const cpvs_dag *d, int *on);
 /* CompressedShadow::getDAG (src/CompressedShadow.h:75-77): copy the words to host memory. */
 int cpvs_dag_copy_words(const cpvs_dag *d, uint32_t *host, uint64_t cap_words);
 int cpvs_dag_device_words(const cpvs_dag *d, const uint32_t **device_words);

@@ -48,6 +48,7 @@ def _load():
         "or_last_node_counts": (None, [u64p, C.c_int]),
         "or_total_visibility": (C.c_int, [u32p]),
         "or_traverse_cpu": (C.c_int, [u32p, C.c_int, f32p, C.c_int]),
+        "or_set_leafmasks": (None, [C.c_int]),
         "or_traverse_path_cs": (C.c_int, [u32p, u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
         "or_path_from_ndc": (None, [f32p, C.c_int, C.c_int, i32p]),
         "or_evaluate": (None, [u32p, u32p, C.c_int, C.c_int, f32p, C.c_uint32, C.c_size_t, C.c_void_p, u8p, C.c_int]),
@@ -135,13 +136,31 @@ def merge_level(words: np.ndarray, node_size: int, mode: int = 0):
     return out[: u * node_size], {i * node_size: int(mapping[i]) for i in range(n)}
 
 
-def create(pyr: Pyramid, zTileIndex: int = 0, zTileNum: int = 1, merge_mode: int = 1):
-    """CompressedShadow::create -> (words uint32 array, numLevels)."""
+class leafmasks_off:
+    """The reference built without LEAFMASKS (CompressedShadow.cpp:17, traverse.cs:8-9): inside the
+    block create() and evaluate() build and walk DAGs that go down to level 0."""
+
+    def __enter__(self):
+        lib().or_set_leafmasks(0)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_leafmasks(1)
+        return False
+
+
+def create(pyr: Pyramid, zTileIndex: int = 0, zTileNum: int = 1, merge_mode: int = 1, leafmasks: bool = True):
+    """CompressedShadow::create -> (words uint32 array, numLevels); leafmasks=False: the reference
+    compiled without LEAFMASKS."""
     ptr = C.POINTER(C.c_uint32)()
     ln = C.c_size_t(0)
     nl = C.c_int(0)
-    rc = lib().or_create(pyr.depth, pyr.P, pyr.N, zTileIndex, zTileNum, merge_mode,
-                         C.byref(ptr), C.byref(ln), C.byref(nl))
+    lib().or_set_leafmasks(int(leafmasks))
+    try:
+        rc = lib().or_create(pyr.depth, pyr.P, pyr.N, zTileIndex, zTileNum, merge_mode,
+                             C.byref(ptr), C.byref(ln), C.byref(nl))
+    finally:
+        lib().or_set_leafmasks(1)
     if rc != 0:
         raise ValueError(f"or_create failed rc={rc}")
     words = np.ctypeslib.as_array(ptr, shape=(ln.value,)).copy()
@@ -174,15 +193,21 @@ def traverse_path_cs(dag, grid, dag_levels, grid_levels, path) -> int:
                                      dag_levels, grid_levels, int(path[0]), int(path[1]), int(path[2]))
 
 
-def evaluate(dag, grid, dag_levels, grid_levels, positions: np.ndarray, matrix=None, nthreads: int = 1):
-    """traverse.cs main() over an (count, 3|4) float32 array -> uint8 visibilities."""
+def evaluate(dag, grid, dag_levels, grid_levels, positions: np.ndarray, matrix=None, nthreads: int = 1,
+             leafmasks: bool = True):
+    """traverse.cs main() over an (count, 3|4) float32 array -> uint8 visibilities (leafmasks=False:
+    the shader compiled without LEAFMASKS)."""
     pos = _f32(positions)
     stride = pos.shape[1]
     vis = np.zeros(pos.shape[0], np.uint8)
     m = None if matrix is None else _f32(matrix).reshape(16)
     mp = None if m is None else m.ctypes.data_as(C.c_void_p)
-    lib().or_evaluate(np.ascontiguousarray(dag, np.uint32), np.ascontiguousarray(grid, np.uint32),
-                      dag_levels, grid_levels, pos, stride, pos.shape[0], mp, vis, nthreads)
+    lib().or_set_leafmasks(int(leafmasks))
+    try:
+        lib().or_evaluate(np.ascontiguousarray(dag, np.uint32), np.ascontiguousarray(grid, np.uint32),
+                          dag_levels, grid_levels, pos, stride, pos.shape[0], mp, vis, nthreads)
+    finally:
+        lib().or_set_leafmasks(1)
     return vis
 
 

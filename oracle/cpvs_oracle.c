@@ -308,8 +308,13 @@ static int vu_push(vu32 *v, u32 x) {
     return 0;
 }
 
-/* useLeafmasks / getMinLevel / getNodeSize (CompressedShadow.cpp:20-41). */
-static int use_leafmasks(int n) { return (n - 3) >= 2; }
+/* useLeafmasks / getMinLevel / getNodeSize (CompressedShadow.cpp:20-41).  The reference compiles
+ * leafmasks in or out (#define LEAFMASKS, CompressedShadow.cpp:17, traverse.cs:8-9); g_leafmasks
+ * is that switch here (on by default, or_set_leafmasks), for create() and the traverse.cs
+ * transcription alike. */
+static int g_leafmasks = 1;
+void or_set_leafmasks(int on) { g_leafmasks = on != 0; }
+static int use_leafmasks(int n) { return g_leafmasks && (n - 3) >= 2; }
 static int min_level(int n) { return use_leafmasks(n) ? 2 : 0; }
 static u32 node_size(int n, int level) { return (use_leafmasks(n) && level == 2) ? LEAF_SIZE : NODE_SIZE; }
 
@@ -668,16 +673,22 @@ int or_traverse_path_cs(const u32 *dag, const u32 *grid, int dl, int gl, int px,
     if (base == 0xFFFFFFFFu) return 0;
     if (base == 0xFFFFFFFEu) return 1;
     size_t off = base;
-    for (int level = dl - 2; level >= 3; --level) {
+    /* Without leafmasks (MIN_LEVEL 0, traverse.cs:34-38) the walk goes down to level 0, whose
+     * masks hold no partial child (absoluteVisible, CompressedShadowUtil.cpp:34-40); a walk that
+     * passed it would return visible (traverse.cs:130-132). */
+    const int last = use_leafmasks(dl) ? 3 : 0;
+    for (int level = dl - 2; level >= last; --level) {
         const u32 b = 1u << level;
         const u32 ci = (((u32)px & b) ? 2u : 0u) + (((u32)py & b) ? 4u : 0u) + (((u32)pz & b) ? 8u : 0u);
         const u32 m = dag[off];
         const u32 v = 3u & (m >> ci);
         if (v == 0) return 0;
         if (v == 1) return 1;
+        if (level == 0) return 1; /* the shader's fall-through: no partial voxel exists */
         const u32 co = (u32)__builtin_popcount(m & (ci ? (0xAAAAu >> (16 - ci)) : 0u));
         off = (size_t)base + dag[off + 1 + co];
     }
+    if (!use_leafmasks(dl)) return 1;
     {
         const u32 ci = ((u32)pz & 7u) * 2u;
         const u32 m = dag[off];

@@ -65,6 +65,8 @@ uint32_t or_merge_level(const uint32_t *level_words, size_t num_nodes, uint32_t 
  * array (free with or_free) and its length; numLevels in *num_levels.
  * merge_mode: 0 = faithful O(n*u) mergeLevel, 1 = hashed (identical words).
  * Returns 0 on success, nonzero on bad arguments / allocation failure. */
+/* The reference's compile-time LEAFMASKS switch (CompressedShadow.cpp:17): on by default. */
+void or_set_leafmasks(int on);
 int or_create(const float *D, const float *P, int N, uint32_t zTileIndex, uint32_t zTileNum,
               int merge_mode, uint32_t **out_words, size_t *out_len, int *num_levels);
 void or_free(void *p);

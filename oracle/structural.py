@@ -12,11 +12,21 @@ This module derives the same words from a different definition, for small maps o
 * words are written level by level from the root, node = [mask, absolute child offsets]
   or, at the leafmask level, [mask, lo, hi, ...] (src/CompressedShadow.cpp:326-392).
 
-The pyramid comes from numpy reshapes (exact min/max), not from the C oracle.
+The pyramid comes from numpy (exact min/max), not from the C oracle: each level pairs the four
+texels of a block as pred(pred(a, b), pred(c, d)) with std::min / std::max, a and b the upper row
+(find/setRow, src/MinMaxHierarchy.cpp:29-49), so a NaN survives only from the first operand.
 """
 from __future__ import annotations
 
 import numpy as np
+
+
+def _smin(a, b):
+    return np.where(b < a, b, a)  # std::min(a, b)
+
+
+def _smax(a, b):
+    return np.where(a < b, b, a)  # std::max(a, b)
 
 
 def _pyramid(D: np.ndarray):
@@ -24,18 +34,18 @@ def _pyramid(D: np.ndarray):
     n = N.bit_length()  # numLevels = log2(N) + 1
     mins, maxs = [D], [D]
     for L in range(1, n):
-        w = N >> L
-        blk = D.reshape(w, N // w, w, N // w)
-        mins.append(blk.min(axis=(1, 3)))
-        maxs.append(blk.max(axis=(1, 3)))
+        m, M = mins[-1], maxs[-1]
+        mins.append(_smin(_smin(m[0::2, 0::2], m[0::2, 1::2]), _smin(m[1::2, 0::2], m[1::2, 1::2])))
+        maxs.append(_smax(_smax(M[0::2, 0::2], M[0::2, 1::2]), _smax(M[1::2, 0::2], M[1::2, 1::2])))
     return n, mins, maxs
 
 
-def build(D: np.ndarray, zTileIndex: int = 0, zTileNum: int = 1):
+def build(D: np.ndarray, zTileIndex: int = 0, zTileNum: int = 1, leafmasks: bool = True):
+    """leafmasks=False: the reference compiled without LEAFMASKS (src/CompressedShadow.cpp:17)."""
     D = np.ascontiguousarray(D, np.float32)
     N = D.shape[0]
     n, mins, maxs = _pyramid(D)
-    leaf_mode = n - 3 >= 2
+    leaf_mode = leafmasks and n - 3 >= 2
     bottom = 2 if leaf_mode else 0
     f32 = np.float32
 
@@ -122,12 +132,13 @@ def build(D: np.ndarray, zTileIndex: int = 0, zTileNum: int = 1):
                 uniq.append(c)
         order.append((L, uniq, seen))
 
-    # Sizes and offsets, then words.
+    # Sizes and offsets, then words (keyed by level too: a node without partial children has the
+    # same key as an equal-mask node of another level).
     offsets = {}
     pos = 0
     for (L, uniq, seen) in order:
         for c in uniq:
-            offsets[key(L, c)] = pos
+            offsets[(L, key(L, c))] = pos
             if leaf_mode and L == 2:
                 pos += 1 + 2 * len(info[(L,) + c][1])
             else:
@@ -135,7 +146,7 @@ def build(D: np.ndarray, zTileIndex: int = 0, zTileNum: int = 1):
     words = np.zeros(pos, np.uint32)
     for (L, uniq, seen) in order:
         for c in uniq:
-            o = offsets[key(L, c)]
+            o = offsets[(L, key(L, c))]
             if leaf_mode and L == 2:
                 m, payload = info[(L,) + c]
                 words[o] = m
@@ -146,5 +157,5 @@ def build(D: np.ndarray, zTileIndex: int = 0, zTileNum: int = 1):
                 m, kids = info[(L,) + c]
                 words[o] = m
                 for j, ch in enumerate(kids):
-                    words[o + 1 + j] = offsets[key(L - 1, ch)]
+                    words[o + 1 + j] = offsets[(L - 1, key(L - 1, ch))]
     return words, n

@@ -25,11 +25,12 @@ class DagError(AssertionError):
     pass
 
 
-def check_dag(words: np.ndarray, num_levels: int):
-    """Raises DagError on a violation; returns per-level node counts (root first)."""
+def check_dag(words: np.ndarray, num_levels: int, leafmasks: bool = True):
+    """Raises DagError on a violation; returns per-level node counts (root first).  leafmasks=False:
+    a DAG built without leafmasks (inner nodes down to level 0)."""
     w = np.asarray(words, np.uint32)
     n = num_levels
-    leaf = (n - 3) >= 2
+    leaf = leafmasks and (n - 3) >= 2
     bottom = 2 if leaf else 0
     R = n - 2
     if w.size == 0:

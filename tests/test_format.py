@@ -29,12 +29,13 @@ def checksum(words):
     return h ^ len(w)
 
 
-def write(words, num_levels, t=0, T=1, *, magic=b"CPVSDAG\0", version=1, header=64, N=None, root=None, ck=None):
+def write(words, num_levels, t=0, T=1, *, magic=b"CPVSDAG\0", version=1, header=64, N=None, root=None, ck=None,
+          flags=0):
     words = np.asarray(words, np.uint32)
     N = (1 << (num_levels - 1)) if N is None else N
     root = int(words[0]) if root is None else root
     ck = checksum(words) if ck is None else ck
-    h = struct.pack("<8sIIIIIIIIQQQ", magic, version, header, num_levels, N, t, T, root, 0, len(words), ck, 0)
+    h = struct.pack("<8sIIIIIIIIQQQ", magic, version, header, num_levels, N, t, T, root, flags, len(words), ck, 0)
     assert len(h) == 64
     return h + words.astype("<u4").tobytes()
 
@@ -109,3 +110,37 @@ def test_structure_damage_is_rejected(dag64):
     _rejects(write(bad, n), "bits above 16")
     # the words end inside the leaf level
     _rejects(write(w[:-1], n), "past the end")
+
+
+@pytest.mark.parametrize("N,t,T", [(16, 0, 1), (64, 1, 2), (128, 0, 1)])
+def test_dags_without_leafmasks_carry_their_flag(N, t, T):
+    """Flag bit 0 marks a DAG built without leafmasks (nodes down to level 0): accepted with it,
+    rejected without it (level 2 then reads as leafmask nodes), and the leafmask DAG of the same
+    map the other way round."""
+    D = O.gen_depth(N, SEED0 + 5 * N + t, max(1, N // 16), True)
+    p = O.Pyramid(D)
+    w0, n = O.create(p, t, T, leafmasks=False)
+    nl, nw, nodes = cp.dag_check(write(w0, n, t, T, flags=1))
+    assert (nl, nw) == (n, len(w0))
+    expect = check_dag(w0, n, leafmasks=False)
+    assert [int(nodes[L]) for L in range(n - 2, -1, -1) if nodes[L]] == [int(c) for c in expect if c]
+    if T == 1:
+        assert nodes[0] > 0 and nodes[1] > 0  # the octree reaches level 0
+    if nodes[1] > 0:  # read as leafmask nodes, level 2's pointers become payload
+        with pytest.raises(cp.CpvsError):
+            cp.dag_check(write(w0, n, t, T))
+    w1, _ = O.create(p, t, T)
+    cp.dag_check(write(w1, n, t, T))
+    # A leafmask DAG whose leaf nodes hold partial slices fails when read as plain nodes (its
+    # leafmask words read as pointers); one without partial slices is the same either way.
+    noise = np.random.default_rng(N + t).random((N, N), dtype=np.float32)
+    w2, _ = O.create(O.Pyramid(noise), t, T)
+    cp.dag_check(write(w2, n, t, T))
+    with pytest.raises(cp.CpvsError):
+        cp.dag_check(write(w2, n, t, T, flags=1))
+
+
+def test_unknown_flags_are_rejected(dag64):
+    w, n = dag64
+    _rejects(write(w, n, flags=2), "unknown flags")
+    _rejects(write(w, n, flags=0x80000000), "unknown flags")

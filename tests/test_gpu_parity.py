@@ -524,3 +524,24 @@ def test_tiled_65536_container_properties(ctx):
         bf = (mid <= D[p[:, 1] - ty * tile, p[:, 0] - tx * tile] * np.float32(full)).astype(np.uint8)
         assert np.array_equal(v, bf), (tx, ty)
         assert 0.05 < v.mean() < 0.95
+
+
+def test_context_close_closes_its_objects():
+    """Closing a context closes the pyramids, DAGs, containers, DAG sets, servers and device arrays
+    still open on it, so none of them later reaches into the destroyed context."""
+    c2 = cp.Context(0)
+    D = O.gen_depth(64, SEED0 + 8, 4, True)
+    d = cp.DeviceArray.from_numpy(D, ctx=c2)
+    mm = cp.MinMaxHierarchy(d, ctx=c2)
+    a = cp.CompressedShadow.create(mm)
+    b = cp.CompressedShadow.create(mm)
+    s = cp.DagSet([a, b])
+    cont = cp.CompressedShadowContainer(cp.CompressedShadow.create(mm))
+    srv = cp.Server(lanes=1, ctx=c2)
+    c2.close()
+    assert not any(getattr(o, "handle", None) for o in (mm, a, b, s, cont, srv)) and not d.ptr
+    for o in (mm, a, b, s, cont, srv, d):
+        o.close()  # a second close is a no-op
+    del mm, a, b, s, cont, srv, d
+    import gc
+    gc.collect()

@@ -218,6 +218,97 @@ def test_tiled_container_matches_brute_force():
     assert np.array_equal(vis, exp2)
 
 
+# --- without leafmasks (the reference built without #define LEAFMASKS) ------------------------
+
+@pytest.mark.parametrize("T", [1, 2])
+def test_leafmasks_off_structural_builder_agrees(T):
+    """CompressedShadow.cpp:17-41 with LEAFMASKS undefined: 1+k-word nodes down to level 0."""
+    for N, D in _maps():
+        if N > 64:
+            continue
+        p = O.Pyramid(D)
+        for t in range(T):
+            w, n = O.create(p, t, T, merge_mode=0, leafmasks=False)
+            ws, ns = structural.build(D, t, T, leafmasks=False)
+            assert n == ns
+            assert np.array_equal(w, ws), (N, T, t)
+            assert np.array_equal(O.create(p, t, T, merge_mode=1, leafmasks=False)[0], w), (N, T, t)
+            counts = check_dag(w, n, leafmasks=False)
+            if N >= 16 and len(w) > 1:
+                assert not np.array_equal(w, O.create(p, t, T)[0])  # the leafmask DAG differs
+
+
+@pytest.mark.parametrize("N,T", [(16, 1), (32, 1), (32, 2)])
+def test_leafmasks_off_every_voxel_matches_brute_force(N, T):
+    """traverse.cs with MIN_LEVEL 0 (traverse.cs:34-38) and CompressedShadow::traverse with
+    tryLeafmasks false (CompressedShadow.cpp:404-463), every voxel against the depth compare."""
+    D = O.gen_depth(N, SEED0 + 7 * N + T, max(1, N // 16), True)
+    p = O.Pyramid(D)
+    for t in range(T):
+        w, n = O.create(p, t, T, leafmasks=False)
+        grid = O.top_level_grid([w])
+        x, y, z = _all_paths(N, t * N, (t + 1) * N)
+        exp = O.brute_visible(D, T, x, y, z)
+        with O.leafmasks_off():
+            got = np.array([O.traverse_path_cs(w, grid, n, 0, (xi, yi, zi)) for xi, yi, zi in zip(x, y, z - t * N)],
+                           np.uint8)
+        assert np.array_equal(got, exp), (N, T, t)
+        # the same walk with the leafmask lookup reads payload as pointers: never equal here
+        if T == 1:
+            pts = O.gen_queries_uniform(11, 4000, 3)
+            vis = O.evaluate(w, grid, n, 0, pts, leafmasks=False)
+            px = np.array([O.path_from_ndc(q, n, 0) for q in pts])
+            assert np.array_equal(vis, O.brute_visible(D, 1, px[:, 0], px[:, 1], px[:, 2]))
+            res = (1 << (n - 1)) - 1
+            pc = (((pts + np.float32(1)) * np.float32(0.5)) * np.float32(res)).astype(np.int64)
+            cpu = np.array([O.traverse_cpu(w, n, q, False) for q in pts])
+            assert np.array_equal(cpu, O.brute_visible(D, 1, pc[:, 0], pc[:, 1], pc[:, 2]))
+
+
+def test_leafmasks_off_planted_mistake_is_caught():
+    """A level-0 child flipped from lit to shadow: the brute-force comparison sees it."""
+    N = 32
+    D = O.gen_depth(N, SEED0 + 21, 2, True)
+    w, n = O.create(O.Pyramid(D), leafmasks=False)
+    counts = check_dag(w, n, leafmasks=False)
+    last = len(w) - 1
+    while w[last] & 0x5555 == 0:  # a level-0 node (masks only: no pointers there) with a lit child
+        last -= 1
+    bad = w.copy()
+    bit = int(w[last] & 0x5555).bit_length() - 1
+    bad[last] ^= np.uint32(1 << bit)
+    grid = O.top_level_grid([w])
+    x, y, z = _all_paths(N, 0, N)
+    exp = O.brute_visible(D, 1, x, y, z)
+    with O.leafmasks_off():
+        got = np.array([O.traverse_path_cs(bad, grid, n, 0, q) for q in zip(x, y, z)], np.uint8)
+    assert not np.array_equal(got, exp)
+    assert counts[-1] > 0
+
+
+# --- NaN depths: cells partial under a parent that is not -------------------------------------
+
+@pytest.mark.parametrize("leafmasks", [True, False])
+def test_nan_maps_structural_builder_agrees(leafmasks):
+    """Maps with scattered NaN and infinite texels, against the structural restatement with the
+    reference's std::min/std::max pair order; the maps make partial cells that no partial parent
+    reaches (which the top-down build never visits)."""
+    from tests.nanmaps import excess, sparse_nan_map
+    seen = 0
+    for N in (16, 32, 64):
+        for k in range(3):
+            D = sparse_nan_map(N, 900 + 10 * N + k, 0.02, 0.01)
+            p = O.Pyramid(D)
+            for T in (1, 2):
+                for t in range(T):
+                    w, n = O.create(p, t, T, leafmasks=leafmasks)
+                    ws, _ = structural.build(D, t, T, leafmasks=leafmasks)
+                    assert np.array_equal(w, ws), (N, k, t, T)
+                    check_dag(w, n, leafmasks=leafmasks)
+                    seen += excess(D, t, T, leafmasks)
+    assert seen > 0
+
+
 # --- planted mistakes: each check above must catch a plausible bug ------------------------
 
 def test_planted_mistakes_are_caught():
