@@ -53,7 +53,7 @@ def table_slots(n):
 def leaf_waves_bytes(N, nodes_leaf):
     """Algorithmic bytes of k_leaf_waves: the 8x8 depth block of every level-3 column (256 B) and,
     per leaf node, its breadth-first index read and its run-head index and leafmask slot written
-    (12 B).  The first node of each run also writes mask, hash and leafmasks; those bytes are left
+    (12 B).  The first node of each run also writes its mask and leafmasks; those bytes are left
     out, so the figure is a lower bound.  The kernel also clears the leaf level's merge table
     (leaf_table_clear_bytes), which is not part of the leaf computation."""
     return 256 * (N // 8) ** 2 + 12 * nodes_leaf
@@ -785,8 +785,8 @@ def main():
     roofline = roof(rname)
     if rname == "leaf_waves":
         # The HBM roofline is the ceiling; what holds the kernel below it is instruction issue.
-        roofline["limiter"] = ("instruction issue: VALU and SALU (the run heads' key hash), "
-                               "not HBM bandwidth")
+        roofline["limiter"] = ("instruction issue: VALU and SALU (per-column changed-node reductions, "
+                               "run-head leafmask ballots), not HBM bandwidth")
         roofline["table_clear_bytes"] = leaf_table_clear_bytes(int(nodes[2]))
     pyramid_roofline = roof("pyr_tile_root")
     lookup_roofline = roof("evaluate4")
