@@ -153,7 +153,8 @@ int cpvs_container_get(const cpvs_container *c, uint32_t x, uint32_t y, uint32_t
 int cpvs_container_set_filter_size(cpvs_container *c, uint32_t size);
 /* copyToGPU (src/CompressedShadowContainer.cpp:31-46): combineDAGs (:52-69) and
  * createTopLevelGrid (:71-91) on the device.  Grid sentinels are 0xFFFFFFFF (shadow) and
- * 0xFFFFFFFE (visible), the values shader/traverse.cs:85-88 tests. */
+ * 0xFFFFFFFE (visible), the values shader/traverse.cs:85-88 tests.  Every slot holds a DAG of the
+ * same numLevels and leafmask mode (one traverse.cs walks them all). */
 int cpvs_container_copy_to_gpu(cpvs_container *c);
 /* freeOnCPU / moveToGPU (src/CompressedShadowContainer.h:55-68). */
 int cpvs_container_free_on_cpu(cpvs_container *c);
@@ -253,7 +254,7 @@ int cpvs_build_batch_config(cpvs_ctx *ctx, const cpvs_batch_item *items, uint32_
 int cpvs_tune_batch_config(cpvs_ctx *ctx, const cpvs_batch_item *items, uint32_t count, const cpvs_batch_config *cands,
                            uint32_t ncand, uint32_t reps, uint32_t *best, double *ms);
 
-/* A set of DAGs (numLevels >= 5), each looked up as a one-DAG container
+/* A set of DAGs of one leafmask mode (cpvs_dag_leafmasks), each looked up as a one-DAG container
  * (CompressedShadowContainer(unique_ptr<CompressedShadow>), src/CompressedShadowContainer.h:27-32)
  * with an identity light matrix: NDC in, visibility out.  The DAGs are borrowed and must outlive
  * the set. */
