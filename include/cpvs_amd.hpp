@@ -39,7 +39,9 @@ inline void check(int rc, const char* what) {
     if (rc != CPVS_OK) throw Error(std::string(what) + ": " + cpvs_last_error(), rc);
 }
 
-// The process-wide context on device 0 (the reference runs in one GL context).
+// The process-wide context on device 0 (the reference runs in one GL context).  The reference
+// compiles leafmasks in or out (#define LEAFMASKS, src/CompressedShadow.cpp:17); compiling this
+// header with CPVS_AMD_NO_LEAFMASKS defined is the same switch (setLeafmasks at run time).
 class Context {
 public:
     static Context& instance() {
@@ -47,10 +49,22 @@ public:
         return c;
     }
     cpvs_ctx* get() const { return ctx_; }
+    // Builds from now on end in leafmask nodes (true, the default) or go down to level 0.
+    void setLeafmasks(bool on) { check(cpvs_ctx_set_leafmasks(ctx_, on ? 1 : 0), "cpvs_ctx_set_leafmasks"); }
+    bool leafmasks() const {
+        int on = 1;
+        check(cpvs_ctx_leafmasks(ctx_, &on), "cpvs_ctx_leafmasks");
+        return on != 0;
+    }
     ~Context() { cpvs_ctx_destroy(ctx_); }
 
 private:
-    Context() { check(cpvs_ctx_create(0, &ctx_), "cpvs_ctx_create"); }
+    Context() {
+        check(cpvs_ctx_create(0, &ctx_), "cpvs_ctx_create");
+#ifdef CPVS_AMD_NO_LEAFMASKS
+        setLeafmasks(false);
+#endif
+    }
     cpvs_ctx* ctx_ = nullptr;
 };
 

@@ -43,6 +43,23 @@ int main(int argc, char** argv) {
         if (!s) s = CompressedShadow::create(pyr(n));
         return *s;
     };
+    // The same map built without leafmasks (the reference compiled without LEAFMASKS); its
+    // traverse(ndc, false) walks to level 0 and must give the same visibilities.
+    std::map<std::string, std::unique_ptr<CompressedShadow>> shadows_off;
+    auto dag_off = [&](const std::string& n) -> CompressedShadow& {
+        auto& s = shadows_off[n];
+        if (!s) {
+            Context::instance().setLeafmasks(false);
+            try {
+                s = CompressedShadow::create(pyr(n));
+            } catch (...) {
+                Context::instance().setLeafmasks(true);
+                throw;
+            }
+            Context::instance().setLeafmasks(true);
+        }
+        return *s;
+    };
     int fails = 0, cases = 0;
     std::string line;
     while (std::getline(in, line)) {
@@ -96,6 +113,11 @@ int main(int argc, char** argv) {
                         V3 p{(x / 32.0f) * 2.0f + -1.0f, (y / 32.0f) * 2.0f + -1.0f, z * 2.0f + -1.0f};
                         if (dag(name).traverse(p) != exp) {
                             std::printf("FAIL sweep %s %d %d\n", name.c_str(), x, y);
+                            ++fails;
+                        }
+                        ++cases;
+                        if (dag_off(name).traverse(p, false) != exp) {
+                            std::printf("FAIL sweep without leafmasks %s %d %d\n", name.c_str(), x, y);
                             ++fails;
                         }
                     }
