@@ -180,6 +180,7 @@ def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26, batch
     bd, bt = [tiles[(tx, ty)] for tx, ty, _ in keys], [(tz, L) for _, _, tz in keys]
     cands = [cp.batch_config(lanes=batch_lanes, forest_mtexels=0)] + \
         [cp.batch_config(lanes=ln, forest_mtexels=mt) for ln in (1, 2, 4) for mt in (64, 256, 512, 1024)] + \
+        [cp.batch_config(lanes=ln, forest_mtexels=mt) for ln in (6, 8) for mt in (128, 256, 512)] + \
         [cp.batch_config(lanes=2, forest_mtexels=64, leaf_cols=4), cp.batch_config(lanes=2, forest_mtexels=64, graph=True),
          cp.batch_config(lanes=4, forest_mtexels=256, leaf_waves=2), cp.batch_config(lanes=3, forest_mtexels=256)]
     best, tune_ms = cp.tune_batch_config(bd, cands, bt)
