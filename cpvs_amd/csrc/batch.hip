@@ -284,12 +284,13 @@ int run_forests(cpvs_ctx* ctx, const cpvs_batch_item* items, uint32_t count, con
     auto stamp = [&](uint32_t i) {
         if (done_ms) done_ms[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
-    // Builds fin[0 .. m) (items idx[0 .. m)) as one forest, halving it while it is too large.
+    // Builds fin[0 .. m) (items idx[0 .. m)) as one forest, halving it while it is too large for
+    // 32-bit offsets or its scratch cannot be allocated (smaller forests need less).
     std::function<int(Worker*, std::vector<ForestIn>&, std::vector<uint32_t>&, uint32_t, uint32_t)> build_range =
         [&](Worker* w, std::vector<ForestIn>& fin, std::vector<uint32_t>& idx, uint32_t a, uint32_t b) -> int {
         std::vector<cpvs_dag*> dags(b - a, nullptr);
         int lrc = build_forest(ctx, *w, fin.data() + a, b - a, fo, dags.data());
-        if (lrc == kForestTooLarge) {
+        if (lrc == kForestTooLarge || lrc == CPVS_ENOMEM) {
             if (b - a == 1) {  // one DAG alone: the single build
                 cpvs_dag* d = nullptr;
                 lrc = build_dag(ctx, w->arena, w->pinned, fo.merge_mode, fin[a].p, fin[a].t, fin[a].T, w->stream, &d);
