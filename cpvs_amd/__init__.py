@@ -90,6 +90,13 @@ class Context:
         call("cpvs_ctx_leafmasks", self.handle, C.byref(v))
         return bool(v.value)
 
+    def release_scratch(self) -> int:
+        """Frees the build scratch, the batch lanes' arenas and the cache of freed buffers; returns
+        the bytes freed.  Call it between builds."""
+        n = C.c_uint64()
+        call("cpvs_ctx_release_scratch", self.handle, C.byref(n))
+        return int(n.value)
+
     def event(self, stream=None):
         e = C.c_void_p()
         call("cpvs_event_record", self.handle, stream, C.byref(e))

@@ -39,6 +39,7 @@ _SIGS = {
     "cpvs_ctx_set_merge": [vp, i32],
     "cpvs_dag_merge_info": [vp, C.POINTER(i32), C.POINTER(u64)],
     "cpvs_ctx_set_leafmasks": [vp, i32],
+    "cpvs_ctx_release_scratch": [vp, C.POINTER(u64)],
     "cpvs_ctx_leafmasks": [vp, C.POINTER(i32)],
     "cpvs_dag_leafmasks": [vp, C.POINTER(i32)],
     "cpvs_pyramid_build": [vp, vp, i32, u32, u32, vp, PP],

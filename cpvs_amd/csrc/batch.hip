@@ -499,6 +499,26 @@ int cpvs_ctx_set_batch_lanes(cpvs_ctx* ctx, uint32_t lanes) {
     return CPVS_OK;
 }
 
+int cpvs_ctx_release_scratch(cpvs_ctx* ctx, uint64_t* freed) {
+    CHECK_ARG(ctx, "ctx_release_scratch: NULL ctx");
+    GUARD_BEGIN
+    std::lock_guard<std::mutex> lk(ctx->mu);  // builds and batches hold it while they run
+    CPVS_HIP_CHECK(hipSetDevice(ctx->device));
+    CPVS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    uint64_t n = ctx->arena.cap;
+    ctx->arena.release();
+    for (Worker* w : ctx->lanes) {
+        CPVS_HIP_CHECK(hipStreamSynchronize(w->stream));
+        n += w->arena.cap + w->levels.cap;
+        w->arena.release();
+        w->levels.release();
+    }
+    n += ctx->buffers.trim(0);
+    if (freed) *freed = n;
+    return CPVS_OK;
+    GUARD_END
+}
+
 int cpvs_ctx_device_name(cpvs_ctx* ctx, char* buf, uint64_t cap) {
     CHECK_ARG(ctx && buf && cap, "ctx_device_name: NULL argument");
     hipDeviceProp_t prop;

@@ -218,6 +218,12 @@ int cpvs_ctx_set_batch_lanes(cpvs_ctx *ctx, uint32_t lanes); /* 0 = default */
 int cpvs_ctx_batch_lanes(cpvs_ctx *ctx, uint32_t *lanes);    /* the count a batch with lanes = 0 uses */
 /* "<name> <arch> <CUs>CU" of the context's device (keys the lane-count tuning cache). */
 int cpvs_ctx_device_name(cpvs_ctx *ctx, char *buf, uint64_t cap);
+/* Frees the context's build scratch, its batch lanes' scratch and pyramid-level arenas, and its
+ * cache of freed DAG and pyramid buffers (each grows to the largest build or forest it served and
+ * is otherwise kept for reuse); *freed (may be NULL) receives the bytes.  Waits for the context's
+ * streams; call it between builds (it takes the lock builds and batches hold).  No counterpart in
+ * the reference, which keeps no device scratch. */
+int cpvs_ctx_release_scratch(cpvs_ctx *ctx, uint64_t *freed);
 /* Times the batch once per candidate lane count (best of reps runs, DAGs discarded), stores the
  * fastest as the context's batch lane count and returns it in *best; ms[k] (optional) receives
  * candidate k's time. */

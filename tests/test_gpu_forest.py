@@ -197,3 +197,23 @@ def test_batch_config_errors(ctx, mix):
         cp.build_batch(depths[:2], tiles[:2], config=cp.batch_config(leaf_cols=16))
     with pytest.raises(cp.CpvsError):
         cp.build_batch(depths[:2], tiles[:2], config=cp.batch_config(lanes=17))
+
+
+def test_release_scratch_between_batches(mix):
+    """cpvs_ctx_release_scratch frees the lanes' grown arenas and the buffer cache; the next batch
+    grows them again and builds the same words."""
+    maps, _, tiles, plain = mix
+    c2 = cp.Context(0)
+    try:
+        dd = [cp.DeviceArray.from_numpy(D, ctx=c2) for _, D, _, _ in maps]
+        cfg = cp.batch_config(lanes=2, forest_mtexels=256)
+        for rep in range(2):
+            _say("release scratch, batch", rep)
+            shadows, _ = cp.build_batch(dd, tiles, ctx=c2, config=cfg)
+            _check(maps, shadows, plain, ("release", rep))
+            for sh in shadows:
+                sh.close()
+            assert c2.release_scratch() > 0
+        assert c2.release_scratch() == 0  # nothing left to free
+    finally:
+        c2.close()
