@@ -65,12 +65,12 @@ __global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord,
 //   q = floor(h_t/8) - 1 when h_t mod 8 != 7.  Each texel marks at most two nodes, so a wave
 //   OR-reduction of the marks gives a chunk's changed nodes at once, and only those (the run
 //   heads, 1.3 M of 6.7 M leaves at 8192^2) get compares.
-// * Only a run's first node stores its mask, leafmasks and key hash: the merge and the final write
-//   read those of first occurrences only, and a key's first occurrence is always the first node
-//   of a run (breadth-first order grows with z inside a column).  Every node stores the
-//   breadth-first index of its run's first node and the slot of that node's leafmasks.
-// * Leafmasks go to the column's own slots (start + q: a column's nodes are contiguous in z), so
-//   their stores need no breadth-first index.
+// * Only a run's first node stores its mask and leafmasks: the merge and the final write read
+//   those of first occurrences only, and a key's first occurrence is always the first node of a
+//   run (breadth-first order grows with z inside a column).  Every node stores the breadth-first
+//   index of its run's first node.
+// * A run head's leafmasks go to the row at its own breadth-first index (8 slots per node), so the
+//   leaf insert, the key compares and the write find them at the node's index, with no slot map.
 // * Measured on MI355X: 8 columns per wave 0.31 ms, 4 columns 0.32 ms, 16 columns 0.42 ms; one
 //   column per one-wave workgroup (loads issued per column, run heads found by a ballot per node)
 //   0.55 ms; one lane per node over LDS-staged columns 0.51 ms (64 texels of work per node).
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
                                                                 const float2* __restrict__ lv3,
                                                                 u32* __restrict__ mask_out, u64* __restrict__ leafm,
                                                                 u32* __restrict__ runhead,
-                                                                u32* __restrict__ leafslot, u32 unit_chunk,
+                                                                u32 unit_chunk,
                                                                 u64* __restrict__ tab, u32 tab_slots,
                                                                 const u32* __restrict__ col_list,
                                                                 const u32* __restrict__ n_list) {
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
         const int h = f + 8 - 8 * zs;
         const int qa = h >> 3;  // floor(h / 8)
         const bool below = (h & 7) != 7;
-        u32 head_q = 0, head_bfs = 0;  // the run head carried into the next chunk
+        u32 head_bfs = 0;  // the run head carried into the next chunk
         // Nodes [q0, q0 + 64) of the column; bfs = breadth-first index of node q0 + lane.
         auto chunk = [&](u32 q0, u32 bfs) {
             const u32 qn = min(cnt - q0, 64u);
@@ -228,9 +228,9 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
                     const u64 m = __ballot(u >= (int)(cmin + sl));
                     mine = (lane == sl) ? m : mine;
                 }
-                const u32 slot = base + q0 + j;  // < nleaf
-                u64* lslot = leafm + (size_t)slot * 8;
-                if (lane < kp && slot < nleaf) lslot[lane] = mine;
+                // The head's leafmask row sits at its own breadth-first index.
+                const u32 hrow = __builtin_amdgcn_readlane(bfs, j);
+                if (lane < kp && hrow < nleaf) leafm[(size_t)hrow * 8 + lane] = mine;
                 my_mask = (lane == j) ? mask : my_mask;
             }
             // Node q0 + lane's run head: the last changed node at or before it.
@@ -238,13 +238,10 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
             const int src = upto ? 63 - __builtin_clzll(upto) : 0;
             const u32 hb = (u32)__shfl((int)bfs, src, kWave);
             const u32 my_head_bfs = upto ? hb : head_bfs;
-            const u32 my_head_q = upto ? q0 + (u32)src : head_q;
             head_bfs = __builtin_amdgcn_readlane(my_head_bfs, qn - 1);
-            head_q = __builtin_amdgcn_readlane(my_head_q, qn - 1);
             // nleaf bounds every index (only exceeded if the column map were inconsistent).
             if (lane < qn && bfs < nleaf) {
                 runhead[bfs] = my_head_bfs;
-                leafslot[bfs] = base + my_head_q;
                 if ((chg >> lane) & 1ull) mask_out[bfs] = my_mask;
             }
         };
@@ -274,7 +271,6 @@ __global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ tab, u32 
 }
 
 __global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u32* __restrict__ mask, const u64* __restrict__ leafm,
-                                                                    const u32* __restrict__ leafslot,
                                                                     const u32* __restrict__ runhead, u32 n,
                                                                     u64* __restrict__ tab, u32 S, u32* __restrict__ tslot,
                                                                     u32* __restrict__ full, u64* __restrict__ zero,
@@ -297,20 +293,19 @@ __global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u32* _
         rh[k] = (i < n) ? runhead[i] : kNoRep;
     }
     // The key hash of each run's first node (the run's smallest index), from its mask and
-    // leafmasks: computed here, lane-parallel, rather than wave-uniform in the leaf kernel.
-    u32 hm[kLeafPer], hs[kLeafPer];
+    // leafmasks (its row at its own index): computed here, lane-parallel, rather than wave-uniform
+    // in the leaf kernel.
+    u32 hm[kLeafPer];
 #pragma unroll
     for (u32 k = 0; k < kLeafPer; ++k) {
         const u32 i = i0 + k * kLeafInsertThreads;
-        const bool head = rh[k] == i;
-        hm[k] = head ? mask[i] : 0u;
-        hs[k] = head ? min(leafslot[i], n - 1) : 0u;  // bounded as in LeafEq
+        hm[k] = (rh[k] == i) ? mask[i] : 0u;
     }
     u64 h[kLeafPer];
 #pragma unroll
     for (u32 k = 0; k < kLeafPer; ++k) {
         const u32 i = i0 + k * kLeafInsertThreads;
-        h[k] = (rh[k] == i) ? leaf_key_hash(hm[k], leafm + (size_t)hs[k] * 8) : 0ull;
+        h[k] = (rh[k] == i) ? leaf_key_hash(hm[k], leafm + (size_t)i * 8) : 0ull;
         if (h[k] == 0ull && rh[k] == i) h[k] = 1ull;
     }
     __syncthreads();
@@ -361,13 +356,13 @@ __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __rest
 // --- (b) radix sort ----------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_sortkeys_leaf(const u32* __restrict__ mask, const u64* __restrict__ leafm,
-                                                       const u32* __restrict__ leafslot, const u32* __restrict__ runhead,
+                                                       const u32* __restrict__ runhead,
                                                        u32 n, u32* __restrict__ skey, u32* __restrict__ sval) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     u32 h = runhead[i];  // the key is the run's first node's
     h = (h < n) ? h : i;
-    skey[i] = fold32(leaf_key_hash(mask[h], leafm + (size_t)min(leafslot[h], n - 1) * 8));  // bounded as in LeafEq
+    skey[i] = fold32(leaf_key_hash(mask[h], leafm + (size_t)h * 8));
     sval[i] = i;
 }
 
@@ -445,7 +440,6 @@ struct WriteArgs {
     u32 first[kMaxLevels + 1];
     WriteLevel lv[kMaxLevels];
     const u64* leafm;
-    const u32* leafslot;
     u32 nleaf;  // nodes of the leaf level (at least 1)
 };
 
@@ -456,7 +450,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_write_levels(WriteArgs a, u32*
     const u32 r = (blockIdx.x - a.first[k]) * blockDim.x + threadIdx.x;
     if (r >= l.U) return;
     if (l.leaf)
-        write_leaf_node(r, l.uniq, l.mask, a.leafm, a.leafslot, a.nleaf, l.off, l.base, out);
+        write_leaf_node(r, l.uniq, l.mask, a.leafm, a.nleaf, l.off, l.base, out);
     else
         write_inner_node(r, l.uniq, l.mask, l.cstart, l.off, l.base, l.child_rank, l.child_off, l.child_base, l.child_n,
                          out);
@@ -494,7 +488,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_write_levels_dev(WriteArgs a, 
         // is not written.
         if ((u64)s_base[k] + l.off[r] + 17 > cap) continue;
         if (l.leaf)
-            write_leaf_node(r, l.uniq, l.mask, a.leafm, a.leafslot, a.nleaf, l.off, s_base[k], out);
+            write_leaf_node(r, l.uniq, l.mask, a.leafm, a.nleaf, l.off, s_base[k], out);
         else
             write_inner_node(r, l.uniq, l.mask, l.cstart, l.off, s_base[k], l.child_rank, l.child_off, s_cbase[k],
                              l.child_n, out);
@@ -930,7 +924,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         need += arena_bytes(c, 8) + 5 * arena_bytes(c, 4);
     }
     const size_t c2 = leaf ? (cnt[2] ? cnt[2] : 1) : 1;
-    if (leaf) need += arena_bytes(c2 * 8, 8) + 3 * arena_bytes(c2, 4);
+    if (leaf) need += arena_bytes(c2 * 8, 8) + 2 * arena_bytes(c2, 4);  // leafmask rows, run heads, column map
     need += 2 * arena_bytes(cmax, 4);
     const bool col_list = leaf && T > 1;  // compact the z-tile's columns with leaf nodes
     if (col_list) need += arena_bytes(ncol, 4) + arena_bytes(1, 4);
@@ -975,7 +969,6 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     }
     u64* leafm = leaf ? A.take<u64>(c2 * 8) : nullptr;
     u32* runhead = leaf ? A.take<u32>(c2) : nullptr;
-    u32* leafslot = leaf ? A.take<u32>(c2) : nullptr;
     u32* colbfs = leaf ? A.take<u32>(c2) : nullptr;
     u32* clist = col_list ? A.take<u32>(ncol) : nullptr;
     u32* nclist = col_list ? A.take<u32>(1) : nullptr;
@@ -1050,11 +1043,11 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             // The leaf level is the first merged: its table is cleared by this kernel.
             if (col_list)
                 CPVS_LAUNCH("leaf_waves", k_leaf_waves<true>, dim3(unit_chunk * kXcds), dim3(64 * kLeafWaves), 0, s, lc,
-                            bound, c, p->depth, N, T, p->lv[3], lv[2].mask, leafm, runhead, leafslot, unit_chunk, tab,
+                            bound, c, p->depth, N, T, p->lv[3], lv[2].mask, leafm, runhead, unit_chunk, tab,
                             slots, clist, nclist);
             else
                 CPVS_LAUNCH("leaf_waves", k_leaf_waves<false>, dim3(unit_chunk * kXcds), dim3(64 * kLeafWaves), 0, s, lc,
-                            bound, c, p->depth, N, T, p->lv[3], lv[2].mask, leafm, runhead, leafslot, unit_chunk, tab,
+                            bound, c, p->depth, N, T, p->lv[3], lv[2].mask, leafm, runhead, unit_chunk, tab,
                             slots, nullptr, nullptr);
             break;
         }
@@ -1109,9 +1102,9 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             if (L == deepest && !is_leaf) CPVS_LAUNCH("table_clear", k_table_clear, grid1d(Sl), dim3(256), 0, s, tab, Sl);
             if (is_leaf) {
                 CPVS_LAUNCH("insert_leaf", k_insert_leaf, dim3(ceil_div<u32>(c, kLeafWindow)), dim3(kLeafInsertThreads), 0, s,
-                            lv[2].mask, leafm, leafslot, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
+                            lv[2].mask, leafm, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
                 CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, runhead, c,
-                            LeafEq{lv[2].mask, leafm, leafslot, nullptr, c}, rep, meta + kMetaColl, lv[2].mask, 1u, rpart, rrounds);
+                            LeafEq{lv[2].mask, leafm, nullptr, c}, rep, meta + kMetaColl, lv[2].mask, 1u, rpart, rrounds);
             } else {
                 CPVS_LAUNCH("insert_inner", k_insert_inner, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, lv[L].mask,
                             lv[L].cstart, child_rank, c,
@@ -1121,7 +1114,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             }
         } else {
             if (is_leaf)
-                CPVS_LAUNCH("sortkeys", k_sortkeys_leaf, grid1d(c), dim3(256), 0, s, lv[2].mask, leafm, leafslot, runhead, c, skey,
+                CPVS_LAUNCH("sortkeys", k_sortkeys_leaf, grid1d(c), dim3(256), 0, s, lv[2].mask, leafm, runhead, c, skey,
                             sval);
             else
                 CPVS_LAUNCH("sortkeys", k_sortkeys_inner, grid1d(c), dim3(256), 0, s, lv[L].mask, lv[L].cstart, child_rank, c,
@@ -1130,7 +1123,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             device_scan(SegHeadIn{skey}, SegStartOut{seg}, c, OpMax{}, nullptr, scan_scratch, s);
             if (is_leaf)
                 CPVS_LAUNCH("representative", k_representative<LeafEq>, grid1d(c), dim3(256), 0, s, sval, seg, c,
-                            LeafEq{lv[2].mask, leafm, leafslot, runhead, c}, rep, meta + kMetaSortColl);
+                            LeafEq{lv[2].mask, leafm, runhead, c}, rep, meta + kMetaSortColl);
             else
                 CPVS_LAUNCH("representative", k_representative<InnerEq>, grid1d(c), dim3(256), 0, s, sval, seg, c,
                             InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaSortColl);
@@ -1187,7 +1180,6 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
         wmax += ceil_div<u32>(lv[L].count ? lv[L].count : 1, kNodeThreads);
     }
     wa.leafm = leafm;
-    wa.leafslot = leafslot;
     wa.nleaf = (u32)c2;
     if (staged)
         CPVS_LAUNCH("write_levels", k_write_levels_dev, dim3(std::min<u32>(wmax, 2048)), dim3(kNodeThreads), 0, s, wa, meta,

@@ -223,7 +223,7 @@ __device__ inline int leaf_f(float D) {
 }
 
 // Key hash of a leaf node from its mask and its partial slices' leafmasks (slot8: the node's
-// leafmask slot, partial slices in ascending order).  All loads are issued before the hash.
+// leafmask row, partial slices in ascending order).  All loads are issued before the hash.
 __device__ inline u64 leaf_key_hash(u32 m, const u64* __restrict__ slot8) {
     const u32 k = num_children(m);
     u64 v[8];
@@ -262,14 +262,13 @@ __device__ inline u64 inner_key_hash(const u32* mask, const u32* cstart, const u
     return h;
 }
 
-// Run heads and leafmask slots are bounded by the level's n nodes where they are used as indices:
+// Run heads are bounded by the level's n nodes where they are used as indices:
 // past a level's emitted children (counts that exceed them, which the host then reports) the
 // entries were never written.
 struct LeafEq {
     const u32* mask;
-    const u64* leafm;     // indexed by leafmask slot
-    const u32* leafslot;  // node -> leafmask slot of its run's first node
-    const u32* runhead;   // when set, a node's mask is read from its run's first node
+    const u64* leafm;     // a run's first node's partial leafmasks: the row at that node's index
+    const u32* runhead;   // when set, a node's key is read from its run's first node
     u32 n;                // nodes of the leaf level
     __device__ bool operator()(u32 a, u32 b) const {
         if (runhead) {
@@ -278,13 +277,11 @@ struct LeafEq {
             b = (hb < n) ? hb : b;
             if (a == b) return true;
         }
-        // Masks and leafmask slots are loaded together (one round trip before the leafmasks).
         const u32 m = mask[a], mb = mask[b];
-        const u32 sa = min(leafslot[a], n - 1), sb = min(leafslot[b], n - 1);
         if (m != mb) return false;
         const u32 k = num_children(m);
-        const u64* pa = leafm + (size_t)sa * 8;
-        const u64* pb = leafm + (size_t)sb * 8;
+        const u64* pa = leafm + (size_t)a * 8;
+        const u64* pb = leafm + (size_t)b * 8;
         bool eq = true;
 #pragma unroll
         for (u32 j = 0; j < 8; ++j)
@@ -607,14 +604,14 @@ __device__ inline void write_inner_node(u32 r, const u32* __restrict__ uniq, con
 
 // Unique leaf node r: [mask, (lo, hi) of each partial slice's leafmask].
 __device__ inline void write_leaf_node(u32 r, const u32* __restrict__ uniq, const u32* __restrict__ mask,
-                                       const u64* __restrict__ leafm, const u32* __restrict__ leafslot, u32 nleaf,
-                                       const u32* __restrict__ off, u32 base, u32* __restrict__ out) {
-    const u32 i = uniq[r];
+                                       const u64* __restrict__ leafm, u32 nleaf, const u32* __restrict__ off, u32 base,
+                                       u32* __restrict__ out) {
+    const u32 i = min(uniq[r], nleaf - 1);  // a first occurrence, which heads its run: its own row
     const u32 o = base + off[r];
     const u32 m = mask[i];
     out[o] = m;
     const u32 k = num_children(m);
-    const u64* p = leafm + (size_t)min(leafslot[i], nleaf - 1) * 8;  // bounded as in LeafEq
+    const u64* p = leafm + (size_t)i * 8;
     u64 v[8];
 #pragma unroll
     for (u32 j = 0; j < 8; ++j) v[j] = (j < k) ? p[j] : 0ull;
@@ -640,7 +637,6 @@ inline u32 leaf_table_slots(u32 c, bool small) { return table_slots((small && c 
 // Non-template kernels defined in build.hip.
 __global__ __launch_bounds__(256) void k_table_clear(u64* __restrict__ tab, u32 S);
 __global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u32* __restrict__ mask, const u64* __restrict__ leafm,
-                                                                    const u32* __restrict__ leafslot,
                                                                     const u32* __restrict__ runhead, u32 n,
                                                                     u64* __restrict__ tab, u32 S, u32* __restrict__ tslot,
                                                                     u32* __restrict__ full, u64* __restrict__ zero,
@@ -650,7 +646,7 @@ __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __rest
                                                                u32 S, u32* __restrict__ tslot, u32* __restrict__ full,
                                                                u64* __restrict__ zero, u32 nzero);
 __global__ __launch_bounds__(256) void k_sortkeys_leaf(const u32* __restrict__ mask, const u64* __restrict__ leafm,
-                                                       const u32* __restrict__ leafslot, const u32* __restrict__ runhead,
+                                                       const u32* __restrict__ runhead,
                                                        u32 n, u32* __restrict__ skey, u32* __restrict__ sval);
 __global__ __launch_bounds__(256) void k_sortkeys_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
                                                         const u32* __restrict__ child_rank, u32 n, u32* __restrict__ skey,

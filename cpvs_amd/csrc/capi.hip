@@ -515,7 +515,7 @@ int cpvs_estimate_build(cpvs_ctx* ctx, const cpvs_pyramid* p, uint32_t t, uint32
     }
     const u64 c2 = (bottom == 2) ? (cnt[2] ? cnt[2] : 1) : 0;
     const u64 ncol = (bottom == 2) ? (u64)(p->N >> 3) * (p->N >> 3) : 0;
-    const u64 leaf = (bottom == 2) ? arena_bytes(c2 * 8, 8) + 3 * arena_bytes(c2, 4) +
+    const u64 leaf = (bottom == 2) ? arena_bytes(c2 * 8, 8) + 2 * arena_bytes(c2, 4) +
                                          arena_bytes(ncol, 16)
                                    : 0;
     u64 slots = 1024;

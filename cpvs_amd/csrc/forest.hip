@@ -233,7 +233,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
                                                              const FItem* __restrict__ items, u32 nleaf,
                                                              u32* __restrict__ mask_out, u64* __restrict__ leafm,
                                                              u32* __restrict__ runhead,
-                                                             u32* __restrict__ leafslot, u64* __restrict__ tab, u32 tab_slots,
+                                                             u64* __restrict__ tab, u32 tab_slots,
                                                              const u32* __restrict__ col_list, const u32* __restrict__ n_list,
                                                              u32 list_cap) {
     __shared__ int s_f[kWaves][kCols][64];
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
         const int h = f + 8 - 8 * zs;
         const int qa = h >> 3;
         const bool below = (h & 7) != 7;
-        u32 head_q = 0, head_bfs = 0;
+        u32 head_bfs = 0;
         auto chunk = [&](u32 q0, u32 bfs) {
             const u32 qn = min(cnt - q0, 64u);
             const int ra = qa - (int)q0;
@@ -331,21 +331,17 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
                     const u64 m = __ballot(u >= (int)(cmin + sl));
                     mine = (lane == sl) ? m : mine;
                 }
-                const u32 slot = base + q0 + j;
-                u64* lslot = leafm + (size_t)slot * 8;
-                if (lane < kp && slot < nleaf) lslot[lane] = mine;
+                const u32 hrow = __builtin_amdgcn_readlane(bfs, j);  // the head's row: its own index
+                if (lane < kp && hrow < nleaf) leafm[(size_t)hrow * 8 + lane] = mine;
                 my_mask = (lane == j) ? mask : my_mask;
             }
             const u64 upto = chg & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
             const int src = upto ? 63 - __builtin_clzll(upto) : 0;
             const u32 hb = (u32)__shfl((int)bfs, src, kWave);
             const u32 my_head_bfs = upto ? hb : head_bfs;
-            const u32 my_head_q = upto ? q0 + (u32)src : head_q;
             head_bfs = __builtin_amdgcn_readlane(my_head_bfs, qn - 1);
-            head_q = __builtin_amdgcn_readlane(my_head_q, qn - 1);
             if (lane < qn && bfs < nleaf) {
                 runhead[bfs] = my_head_bfs;
-                leafslot[bfs] = base + my_head_q;
                 if ((chg >> lane) & 1ull) mask_out[bfs] = my_mask;
             }
         };
@@ -417,7 +413,6 @@ struct FWriteArgs {
 // Unique node r of level L (global rank order) into its DAG's buffer: [mask, absolute offsets of
 // the partial children] or [mask, lo, hi, ...] at the leaf level (src/CompressedShadow.cpp:326-392).
 __global__ __launch_bounds__(kNodeThreads) void k_forest_write(FWriteArgs a, FLevels lv, const u64* __restrict__ leafm,
-                                                                const u32* __restrict__ leafslot,
                                                                 const u32* __restrict__ base_off, const u32* __restrict__ B,
                                                                 u32* const* __restrict__ outp) {
     int k = 0;
@@ -435,7 +430,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_forest_write(FWriteArgs a, FLe
     const u32 kc = num_children(m);
     if (!kc) return;
     if (L == 2) {
-        const u64* p = leafm + (size_t)leafslot[i] * 8;
+        const u64* p = leafm + (size_t)i * 8;  // a first occurrence heads its run: its own row
         u64 v[8];
 #pragma unroll
         for (u32 j = 0; j < 8; ++j) v[j] = (j < kc) ? p[j] : 0ull;
@@ -471,22 +466,22 @@ constexpr int kRetryExact = -1012;  // the counts exceed the emitted nodes (a ma
 
 // Launches the leaf kernel in the shape the options name.
 void launch_leaf(const ForestOpts& o, u32 units_cols, hipStream_t s, const uint4* colinfo, const u32* colbfs,
-                 const FItem* items, u32 nleaf, u32* mask, u64* leafm, u32* runhead, u32* leafslot, u64* tab,
+                 const FItem* items, u32 nleaf, u32* mask, u64* leafm, u32* runhead, u64* tab,
                  u32 slots, const u32* clist, const u32* nclist, u32 cap) {
     const u32 per = o.leaf_cols * o.leaf_waves;
     const dim3 grid((units_cols + per - 1) / per);
     if (o.leaf_cols == 4 && o.leaf_waves == 1)
         CPVS_LAUNCH("forest_leaf", (k_forest_leaf<4, 1>), grid, dim3(64), 0, s, colinfo, colbfs, items, nleaf, mask, leafm,
-                    runhead, leafslot, tab, slots, clist, nclist, cap);
+                    runhead, tab, slots, clist, nclist, cap);
     else if (o.leaf_cols == 4)
         CPVS_LAUNCH("forest_leaf", (k_forest_leaf<4, 2>), grid, dim3(128), 0, s, colinfo, colbfs, items, nleaf, mask, leafm,
-                    runhead, leafslot, tab, slots, clist, nclist, cap);
+                    runhead, tab, slots, clist, nclist, cap);
     else if (o.leaf_waves == 1)
         CPVS_LAUNCH("forest_leaf", (k_forest_leaf<8, 1>), grid, dim3(64), 0, s, colinfo, colbfs, items, nleaf, mask, leafm,
-                    runhead, leafslot, tab, slots, clist, nclist, cap);
+                    runhead, tab, slots, clist, nclist, cap);
     else
         CPVS_LAUNCH("forest_leaf", (k_forest_leaf<8, 2>), grid, dim3(128), 0, s, colinfo, colbfs, items, nleaf, mask, leafm,
-                    runhead, leafslot, tab, slots, clist, nclist, cap);
+                    runhead, tab, slots, clist, nclist, cap);
 }
 
 // Pinned staging of one forest build.
@@ -662,7 +657,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
         const size_t c = C[L] ? C[L] : 1;
         need += (L > 2 ? arena_bytes(c, 8) + arena_bytes(c, 4) : 0) + 4 * arena_bytes(c, 4);  // no coords, starts at leaves
     }
-    need += arena_bytes(c2 * 8, 8) + 3 * arena_bytes(c2, 4);
+    need += arena_bytes(c2 * 8, 8) + 2 * arena_bytes(c2, 4);  // leafmask rows, run heads, column map
     need += arena_bytes(ncol, 4) + arena_bytes(1, 4);
     need += 2 * arena_bytes(cmax, 4);
     if (use_table) need += arena_bytes(S, 8);
@@ -689,7 +684,6 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
     }
     u64* leafm = A.take<u64>(c2 * 8);
     u32* runhead = A.take<u32>(c2);
-    u32* leafslot = A.take<u32>(c2);
     u32* colbfs = A.take<u32>(c2);
     u32* clist = A.take<u32>(ncol ? ncol : 1);
     u32* nclist = A.take<u32>(1);
@@ -744,7 +738,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
         const u32 slots2 = use_table ? leaf_table_slots((u32)C[2], small_leaf) : 0u;
         if (C[2]) {
             const u32 bound = (u32)std::min<u64>(ncol, C[2]);
-            launch_leaf(o, bound, s, colinfo, colbfs, items, (u32)C[2], lv.mask[2], leafm, runhead, leafslot, tab, slots2,
+            launch_leaf(o, bound, s, colinfo, colbfs, items, (u32)C[2], lv.mask[2], leafm, runhead, tab, slots2,
                         clist, nclist, bound);
         }
         // Bottom-up merge over every level with nodes (roots included: a root is alone in its DAG).
@@ -766,9 +760,9 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
                 if (!is_leaf && prev < 0) CPVS_LAUNCH("table_clear", k_table_clear, grid_of(Sl, 256), dim3(256), 0, s, tab, Sl);
                 if (is_leaf) {
                     CPVS_LAUNCH("insert_leaf", k_insert_leaf, dim3(ceil_div<u32>(c, kLeafWindow)), dim3(kLeafInsertThreads), 0, s,
-                                lv.mask[2], leafm, leafslot, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
+                                lv.mask[2], leafm, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
                     CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid_of(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab,
-                                runhead, c, LeafEq{lv.mask[2], leafm, leafslot, nullptr, c}, rep, meta + kMetaColl, lv.mask[2], 1u,
+                                runhead, c, LeafEq{lv.mask[2], leafm, nullptr, c}, rep, meta + kMetaColl, lv.mask[2], 1u,
                                 rpart, rrounds);
                 } else {
                     CPVS_LAUNCH("insert_inner", k_insert_inner, grid_of(c, kInsertBlock), dim3(kInsertBlock), 0, s, lv.mask[L],
@@ -783,7 +777,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
                                           c, OpSum64{}, nullptr, rpart, s);
             } else {
                 if (is_leaf)
-                    CPVS_LAUNCH("sortkeys", k_sortkeys_leaf, grid_of(c, 256), dim3(256), 0, s, lv.mask[2], leafm, leafslot, runhead, c,
+                    CPVS_LAUNCH("sortkeys", k_sortkeys_leaf, grid_of(c, 256), dim3(256), 0, s, lv.mask[2], leafm, runhead, c,
                                 skey, sval);
                 else
                     CPVS_LAUNCH("sortkeys", k_sortkeys_inner, grid_of(c, 256), dim3(256), 0, s, lv.mask[L], lv.cstart[L],
@@ -792,7 +786,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
                 device_scan(SegHeadIn{skey}, SegStartOut{segs}, c, OpMax{}, nullptr, scan_scratch, s);
                 if (is_leaf)
                     CPVS_LAUNCH("representative", k_representative<LeafEq>, grid_of(c, 256), dim3(256), 0, s, sval, segs, c,
-                                LeafEq{lv.mask[2], leafm, leafslot, runhead, c}, rep, meta + kMetaSortColl);
+                                LeafEq{lv.mask[2], leafm, runhead, c}, rep, meta + kMetaSortColl);
                 else
                     CPVS_LAUNCH("representative", k_representative<InnerEq>, grid_of(c, 256), dim3(256), 0, s, sval, segs, c,
                                 InnerEq{lv.mask[L], lv.cstart[L], child_rank}, rep, meta + kMetaSortColl);
@@ -878,7 +872,7 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
     }
     wa.first[wa.nlev] = wblocks;
     if (e == hipSuccess && wblocks) {
-        CPVS_LAUNCH("forest_write", k_forest_write, dim3(wblocks), dim3(kNodeThreads), 0, s, wa, lv, leafm, leafslot, boff, B,
+        CPVS_LAUNCH("forest_write", k_forest_write, dim3(wblocks), dim3(kNodeThreads), 0, s, wa, lv, leafm, boff, B,
                     dptrs);
         e = hipGetLastError();
     }
