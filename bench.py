@@ -199,10 +199,13 @@ def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26, batch
     same = all(np.array_equal(batch[keys.index(k)].getDAG(), w) for k, w in sample.items())
     for sh in batch:
         sh.close()
-    c.moveToGPU()
-    dl, gl, words = c.info()
     for d in tiles.values():
         d.free()
+    # the lookup top table tuned on the queries (untimed), then the host side freed (moveToGPU)
+    top_set, top_ms = c.tuneTopLevels(q, candidates=(-1, 0, 3, 4, 5), visibilities=vis)
+    c.freeOnCPU()
+    dl, gl, words = c.info()
+    top_levels = c.top_levels()
     c.evaluate(q, None, vis)
     ctx.synchronize()
     lk = []
@@ -225,7 +228,9 @@ def run_tiled(cp, np, ctx, full=65536, tile=8192, boxes=16384, nq=1 << 26, batch
             "tune_ms": {json.dumps(c.as_dict(), sort_keys=True): round(m, 2) for c, m in zip(cands, tune_ms)},
             "batched_words_identical": bool(same),
             "dag_build_ms_p50": round(pct(per_dag, 50), 3), "dag_build_ms_p99": round(pct(per_dag, 99), 3),
-            "lookup_G_per_s": round(nq / (min(lk) * 1e-3) / 1e9, 3)}
+            "lookup_G_per_s": round(nq / (min(lk) * 1e-3) / 1e9, 3),
+            "lookup_top_table": {"setting": top_set, "levels": top_levels,
+                                 "tune_ms": {str(k): round(v, 4) for k, v in top_ms.items()}}}
 
 
 def run_c0(cp, np, ctx, N=1024, boxes=32, nq=1 << 20, reps=20):
@@ -234,6 +239,11 @@ def run_c0(cp, np, ctx, N=1024, boxes=32, nq=1 << 20, reps=20):
     d = cp.gen_depth(N, SEED0, boxes, False)
     q = cp.gen_queries_uniform(SEED0, nq, 4)
     vis = cp.DeviceArray((nq,), np.uint8, ctx)
+    mm = cp.MinMaxHierarchy(d, borrow=True)
+    c = cp.CompressedShadowContainer(cp.CompressedShadow.create(mm))
+    top_set, top_ms = c.tuneTopLevels(q, candidates=(-1, 0, 4, 5, 6, 7), visibilities=vis)  # untimed
+    c.close()
+    mm.close()
     bms, lms = [], []
     for it in range(reps + 3):
         e0 = ctx.event(stream)
@@ -242,6 +252,7 @@ def run_c0(cp, np, ctx, N=1024, boxes=32, nq=1 << 20, reps=20):
         e1 = ctx.event(stream)
         words = cs.num_words()
         c = cp.CompressedShadowContainer(cs)
+        c.setTopLevels(top_set)
         c.copyToGPU()
         e2 = ctx.event(stream)
         c.evaluate(q, None, vis)
@@ -259,7 +270,8 @@ def run_c0(cp, np, ctx, N=1024, boxes=32, nq=1 << 20, reps=20):
     return {"workload": f"{N}x{N} depth (plane + {boxes} boxes), {nq} uniform lookups",
             "build_ms_p50": round(pct(bms, 50), 4), "build_ms_p99": round(pct(bms, 99), 4),
             "Mtexels_per_s": round(N * N / (pct(bms, 50) * 1e-3) / 1e6, 1), "dag_words": words,
-            "lookup_G_per_s": round(nq / (pct(lms, 50) * 1e-3) / 1e9, 3)}
+            "lookup_G_per_s": round(nq / (pct(lms, 50) * 1e-3) / 1e9, 3),
+            "lookup_top_table": {"setting": top_set, "tune_ms": {str(k): round(v, 4) for k, v in top_ms.items()}}}
 
 
 def splitmix64(seed):
@@ -686,10 +698,20 @@ def main():
     dst.free()
     log(f"[bench] device copy of {copy_bytes >> 20} MiB: {min(copy_ms):.3f} ms = {copy_gbs:.0f} GB/s (read + write)")
 
-    build_ms, lookup_ms, step_ms = [], [], []
+    # The container's lookup top table, tuned on these queries before the steps (untimed): every
+    # step's container is built with the setting it picks.
+    mm = cp.MinMaxHierarchy(depth, borrow=True)
+    c = cp.CompressedShadowContainer(cp.CompressedShadow.create(mm))
+    top_set, top_ms = c.tuneTopLevels(queries, candidates=(-1, 0, 4, 5, 6, 7), visibilities=vis)
+    c.close()
+    mm.close()
+    log(f"[bench] top table: tuner picked {top_set} ({ {k: round(v, 4) for k, v in top_ms.items()} } ms)")
+
+    build_ms, lookup_ms, step_ms, container_ms = [], [], [], []
     words = None
     dag_words_host = None
     levels = None
+    top_levels = None
     for it in range(args.warmup + args.steps):
         t0 = time.perf_counter()
         e0 = ctx.event(stream)
@@ -701,12 +723,15 @@ def main():
             levels = cs.level_stats()
             dag_words_host = cs.getDAG() if not args.no_cpu_baseline else None
         c = cp.CompressedShadowContainer(cs)
+        c.setTopLevels(top_set)
         c.copyToGPU()
         e2 = ctx.event(stream)
         c.evaluate(queries, None, vis)
         e3 = ctx.event(stream)
         b = ctx.elapsed_ms(e0, e1)
         lk = ctx.elapsed_ms(e2, e3)
+        cm = ctx.elapsed_ms(e1, e2)
+        top_levels = c.top_levels()
         t1 = time.perf_counter()
         for e in (e0, e1, e2, e3):
             ctx.destroy_event(e)
@@ -715,6 +740,7 @@ def main():
         if it >= args.warmup:
             build_ms.append(b)
             lookup_ms.append(lk)
+            container_ms.append(cm)
             step_ms.append((t1 - t0) * 1e3)
         log(f"[bench] step {it}: build {b:.3f} ms, lookup {lk:.3f} ms, step {(t1 - t0) * 1e3:.2f} ms")
 
@@ -750,6 +776,7 @@ def main():
         mm = cp.MinMaxHierarchy(depth, borrow=True)
         cs = cp.CompressedShadow.create(mm)
         c = cp.CompressedShadowContainer(cs)
+        c.setTopLevels(top_set)
         c.copyToGPU()
         c.evaluate(queries, None, vis)
         ctx.synchronize()
@@ -906,6 +933,11 @@ def main():
         "lookup_G_per_s": round(lookup_rate, 3),
         "lookup_ms_p50": round(pct(lookup_ms, 50), 3),
         "lookup_ms_p99": round(pct(lookup_ms, 99), 3),
+        "lookup_top_table": {"setting": top_set, "levels": top_levels,
+                             "tune_ms": {str(k): round(v, 4) for k, v in top_ms.items()},
+                             "container_ms_p50": round(pct(container_ms, 50), 3),
+                             "note": "copyToGPU (combineDAGs, grid, top table) between build and lookup, GPU "
+                                     "events; not in build_ms or lookup_ms"},
         "roofline": roofline,
         "pyramid_roofline": pyramid_roofline,
         "lookup_roofline": lookup_roofline,

@@ -409,6 +409,34 @@ class CompressedShadowContainer:
     def setFilterSize(self, size: int):
         call("cpvs_container_set_filter_size", self.handle, size)
 
+    def setTopLevels(self, levels: int):
+        """Levels of the lookup top table built by copyToGPU: -1 automatic, 0 none, 1..7."""
+        call("cpvs_container_set_top_levels", self.handle, levels)
+
+    def top_levels(self) -> int:
+        """Levels of the top table the last copyToGPU built (0: none)."""
+        k = C.c_int()
+        call("cpvs_container_top_levels", self.handle, C.byref(k))
+        return k.value
+
+    def tuneTopLevels(self, positions: DeviceArray, candidates=(0, 3, 4, 5, 6, 7), lightViewProj=None,
+                      visibilities: Optional[DeviceArray] = None, reps: int = 3):
+        """Times evaluate() on these positions with each top-table setting (cpvs_container_tune_top_levels),
+        keeps the fastest.  Returns (best setting, {setting: ms})."""
+        count, stride = positions.shape
+        if visibilities is None:
+            visibilities = DeviceArray((count,), np.uint8, self.ctx)
+        m = None
+        if lightViewProj is not None:
+            m = np.ascontiguousarray(np.asarray(lightViewProj, np.float32).T.reshape(16))
+        cand = (C.c_int * len(candidates))(*candidates)
+        ms = np.zeros(len(candidates), np.float64)
+        best = C.c_int()
+        call("cpvs_container_tune_top_levels", self.handle, positions.ptr, stride, count,
+             None if m is None else m.ctypes.data_as(C.c_void_p), visibilities.ptr, cand, len(candidates), reps,
+             C.byref(best), ms.ctypes.data_as(C.c_void_p))
+        return int(candidates[best.value]), {int(k): float(t) for k, t in zip(candidates, ms)}
+
     def copyToGPU(self):
         call("cpvs_container_copy_to_gpu", self.handle)
 

@@ -683,6 +683,23 @@ int cpvs_container_set_filter_size(cpvs_container* c, uint32_t size) {
     return CPVS_OK;
 }
 
+int cpvs_container_set_top_levels(cpvs_container* c, int levels) {
+    CHECK_ARG(c, "container_set_top_levels: NULL container");
+    CHECK_ARG(levels >= -1 && levels <= kMaxTopLevels, "container_set_top_levels: %d levels (-1..%d)", levels,
+              kMaxTopLevels);
+    CHECK_ARG(levels < 0 || ((u64)c->dags.size() << (3 * levels)) * sizeof(u32) <= kTopTableMaxBytes,
+              "container_set_top_levels: %d levels over %zu cells exceed %llu bytes", levels, c->dags.size(),
+              (unsigned long long)kTopTableMaxBytes);
+    c->top_request = levels;
+    return CPVS_OK;
+}
+
+int cpvs_container_top_levels(const cpvs_container* c, int* levels) {
+    CHECK_ARG(c && levels, "container_top_levels: NULL argument");
+    *levels = (int)c->top_levels;
+    return CPVS_OK;
+}
+
 int cpvs_container_copy_to_gpu(cpvs_container* c) {
     CHECK_ARG(c, "container_copy_to_gpu: NULL container");
     if (c->freed_cpu) {
@@ -722,8 +739,11 @@ int cpvs_container_copy_to_gpu(cpvs_container* c) {
     hipStream_t s = ctx->stream;
     if (c->d_dag) hipFree(c->d_dag);
     if (c->d_grid) hipFree(c->d_grid);
+    if (c->d_top) hipFree(c->d_top);
     c->d_dag = nullptr;
     c->d_grid = nullptr;
+    c->d_top = nullptr;
+    c->top_levels = 0;
     if (hipMalloc(&c->d_dag, sizeof(u32) * (total ? total : 1)) != hipSuccess ||
         hipMalloc(&c->d_grid, sizeof(u32) * cells) != hipSuccess) {
         (void)hipGetLastError();
@@ -738,6 +758,27 @@ int cpvs_container_copy_to_gpu(cpvs_container* c) {
         off += d->words;
     }
     CPVS_HIP_CHECK(hipMemcpyAsync(c->d_grid, grid.data(), sizeof(u32) * cells, hipMemcpyHostToDevice, s));
+    // The lookups' top table (launch_build_top): K levels below the grid cell, no more than the
+    // DAGs' inner levels above the last (level 3 with leafmasks, 1 without) nor kMaxTopLevels, and
+    // within kTopTableBytes; skipped below 3 levels, where it saves little.
+    const int lead = dl - 1 - (c->dags[0]->leafmasks ? 3 : 1);
+    int K = (c->top_request < 0) ? std::min(lead, kMaxTopLevels) : std::min(c->top_request, lead);
+    if (c->top_request < 0) {
+        while (K > 0 && ((u64)cells << (3 * K)) * sizeof(u32) > kTopTableBytes) --K;
+        if (K < 3) K = 0;
+    }
+    if (K > 0 && total < 0xFFFFFFFDull) {
+        const u64 entries = (u64)cells << (3 * K);
+        if (hipMalloc(&c->d_top, sizeof(u32) * entries) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("container_copy_to_gpu: hipMalloc of the %llu-entry top table failed",
+                      (unsigned long long)entries);
+            return CPVS_ENOMEM;
+        }
+        int rc = launch_build_top(c->d_dag, (u32)total, c->d_grid, (u32)cells, dl, (u32)K, c->d_top, s);
+        if (rc) return rc;
+        c->top_levels = (u32)K;
+    }
     CPVS_HIP_CHECK(hipStreamSynchronize(s));
     c->dag_words = total;
     c->dag_levels = dl;
@@ -809,7 +850,8 @@ int cpvs_container_evaluate(cpvs_container* c, const float* pos, uint32_t stride
     }
     CPVS_HIP_CHECK(hipSetDevice(c->ctx->device));
     ProfInstall prof(c->ctx->prof_on ? &c->ctx->prof : nullptr);
-    return launch_evaluate(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, c->leafmasks, pos, stride, count,
+    return launch_evaluate(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, c->leafmasks, c->d_top,
+                           c->top_levels, pos, stride, count,
                            to_mat(m), vis, pick_stream(c->ctx, stream));
 }
 
@@ -825,7 +867,8 @@ int cpvs_container_evaluate_pcf(cpvs_container* c, const float* pos, uint32_t st
     }
     CPVS_HIP_CHECK(hipSetDevice(c->ctx->device));
     ProfInstall prof(c->ctx->prof_on ? &c->ctx->prof : nullptr);
-    return launch_evaluate_pcf(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, c->leafmasks, pos, stride, count,
+    return launch_evaluate_pcf(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, c->leafmasks, c->d_top,
+                               c->top_levels, pos, stride, count,
                                to_mat(m), c->filter_size, vis, pick_stream(c->ctx, stream));
 }
 
@@ -838,8 +881,72 @@ int cpvs_container_evaluate_image(cpvs_container* c, const float* pos, uint32_t 
     }
     CPVS_HIP_CHECK(hipSetDevice(c->ctx->device));
     ProfInstall prof(c->ctx->prof_on ? &c->ctx->prof : nullptr);
-    return launch_evaluate_image(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, c->leafmasks, pos, width, height,
+    return launch_evaluate_image(c->d_dag, (u32)c->dag_words, c->d_grid, c->dag_levels, c->grid_levels, c->leafmasks,
+                                 c->d_top, c->top_levels, pos, width, height,
                                  to_mat(m), vis, pick_stream(c->ctx, stream));
+}
+
+int cpvs_container_tune_top_levels(cpvs_container* c, const float* pos, uint32_t stride, uint64_t count,
+                                   const float* m, uint8_t* vis, const int* cands, uint32_t ncand, uint32_t reps,
+                                   int* best, double* ms) {
+    CHECK_ARG(c && pos && vis && count && cands && ncand && best, "container_tune_top_levels: NULL or empty argument");
+    CHECK_ARG(reps >= 1 && reps <= 100, "container_tune_top_levels: reps %u must be in [1, 100]", reps);
+    if (c->freed_cpu) {
+        set_error("container_tune_top_levels: the DAGs were freed (free_on_cpu): nothing to rebuild from");
+        return CPVS_ESTATE;
+    }
+    const int keep = c->top_request;
+    for (uint32_t k = 0; k < ncand; ++k) {
+        const int rc = cpvs_container_set_top_levels(c, cands[k]);
+        c->top_request = keep;
+        if (rc) return rc;
+    }
+    GUARD_BEGIN
+    CPVS_HIP_CHECK(hipSetDevice(c->ctx->device));
+    hipStream_t s = c->ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CPVS_HIP_CHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        set_error("container_tune_top_levels: hipEventCreate failed");
+        return CPVS_EHIP;
+    }
+    int rc = CPVS_OK;
+    float best_ms = 0.0f;
+    uint32_t pick = 0;
+    for (uint32_t k = 0; k < ncand && rc == CPVS_OK; ++k) {
+        c->top_request = cands[k];
+        rc = cpvs_container_copy_to_gpu(c);
+        // one warm-up, then the best of reps timed lookups
+        if (rc == CPVS_OK) rc = cpvs_container_evaluate(c, pos, stride, count, m, vis, s);
+        float t = 0.0f;
+        for (uint32_t r = 0; r < reps && rc == CPVS_OK; ++r) {
+            float one = 0.0f;
+            if (hipEventRecord(e0, s) != hipSuccess) rc = CPVS_EHIP;
+            if (rc == CPVS_OK) rc = cpvs_container_evaluate(c, pos, stride, count, m, vis, s);
+            if (rc == CPVS_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                  hipEventElapsedTime(&one, e0, e1) != hipSuccess))
+                rc = CPVS_EHIP;
+            if (rc == CPVS_OK && (r == 0 || one < t)) t = one;
+        }
+        if (rc == CPVS_OK) {
+            if (ms) ms[k] = t;
+            if (k == 0 || t < best_ms) {
+                best_ms = t;
+                pick = k;
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc == CPVS_EHIP) set_error("container_tune_top_levels: event timing failed");
+    c->top_request = (rc == CPVS_OK) ? cands[pick] : keep;
+    if (rc == CPVS_OK) {
+        rc = cpvs_container_copy_to_gpu(c);
+        *best = (int)pick;
+    }
+    return rc;
+    GUARD_END
 }
 
 int cpvs_container_destroy(cpvs_container* c) {
@@ -848,6 +955,7 @@ int cpvs_container_destroy(cpvs_container* c) {
     for (auto* d : c->dags) cpvs_dag_destroy(d);
     if (c->d_dag) hipFree(c->d_dag);
     if (c->d_grid) hipFree(c->d_grid);
+    if (c->d_top) hipFree(c->d_top);
     delete c;
     return CPVS_OK;
 }

@@ -16,6 +16,12 @@ namespace cpvs {
 
 constexpr int kMaxLevels = 24;  // N <= 2^21 texels per side
 constexpr uint32_t kMaxFilterSize = 64;  // PCF kernel width limit (64 x 64 lookups per point)
+// A container's lookup top table: at most this many levels, and (chosen automatically) at most
+// kTopTableBytes, one XCD's L2, so that lookups in any order find it cached; an explicit request
+// (coherent queries gain from more levels) may take up to kTopTableMaxBytes.
+constexpr int kMaxTopLevels = 7;
+constexpr uint64_t kTopTableBytes = 4ull << 20;
+constexpr uint64_t kTopTableMaxBytes = 1ull << 30;
 
 void set_error(const char* fmt, ...);
 
@@ -174,6 +180,9 @@ struct cpvs_container {
     uint32_t* d_grid = nullptr;  // length^3 entries
     int dag_levels = 0, grid_levels = 0;
     bool leafmasks = true;       // the DAGs' mode (all alike)
+    uint32_t* d_top = nullptr;   // the lookups' top table (launch_build_top), or null
+    uint32_t top_levels = 0;     // its levels (0: none)
+    int top_request = -1;        // cpvs_container_set_top_levels: -1 automatic
     bool on_gpu = false;
     bool freed_cpu = false;
 };
@@ -246,13 +255,16 @@ int noalloc_setup(int device);
 struct Mat4 {
     float m[16];
 };
-// leaf: the DAGs end in leafmask nodes at level 2; 0: they go down to level 0.
-int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos, u32 stride,
-                    u64 count, const Mat4& m, uint8_t* vis, hipStream_t s);
-int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos, u32 stride,
-                        u64 count, const Mat4& m, u32 f, float* vis, hipStream_t s);
-int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos,
-                          u32 width, u32 height, const Mat4& m, uint8_t* vis, hipStream_t s);
+// leaf: the DAGs end in leafmask nodes at level 2; 0: they go down to level 0.  top (may be null):
+// the container's top table of topk levels (launch_build_top).
+int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+                    const float* pos, u32 stride, u64 count, const Mat4& m, uint8_t* vis, hipStream_t s);
+int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+                        const float* pos, u32 stride, u64 count, const Mat4& m, u32 f, float* vis, hipStream_t s);
+int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+                          const float* pos, u32 width, u32 height, const Mat4& m, uint8_t* vis, hipStream_t s);
+// The top table of a container: cells x 8^K entries (the walk's state after K levels per cell).
+int launch_build_top(const u32* dag, u32 words, const u32* grid, u32 cells, int dl, u32 K, u32* top, hipStream_t s);
 // One DAG of a set, walked as a one-cell container.  base is 0, or the grid sentinel
 // 0xFFFFFFFF / 0xFFFFFFFE when the root is all shadow / all lit.
 struct DagDesc {

@@ -18,6 +18,8 @@ struct GridDag {
     int dl, gl;
     u32 words;  // DAG word count: every read is bounds-checked against it
     u32 leaf;   // 1: leafmask nodes at level 2; 0: the DAG goes down to level 0
+    const u32* __restrict__ top;  // the walk's state after the top topk levels (null: none)
+    u32 topk;
 };
 
 // Workgroups are dealt to the XCDs in turn (block b to XCD b mod kXcds), each XCD with its own L2.
@@ -50,11 +52,11 @@ __device__ inline u32 quantise(float v, u32 R) {
 // without, the inner levels go down to level 0, whose children are single voxels, lit or shadow
 // (absoluteVisible, src/CompressedShadowUtil.cpp:34-40), never partial; a walk that passed level 0
 // would end visible (traverse.cs:130-132).
-__device__ inline u32 walk_dag(const u32* __restrict__ dag, u32 words, int dl, u32 leaf, u32 base, u32 px, u32 py,
-                               u32 pz) {
-    u32 off = base;
+// walk_from continues a walk at node `off` of level level0 (walk_dag starts at the root).
+__device__ inline u32 walk_from(const u32* __restrict__ dag, u32 words, u32 leaf, u32 base, u32 off, int level0,
+                                u32 px, u32 py, u32 pz) {
     const int last = leaf ? 3 : 0;
-    for (int level = dl - 2; level >= last; --level) {
+    for (int level = level0; level >= last; --level) {
         const u32 ci = (((px >> level) & 1u) << 1) | (((py >> level) & 1u) << 2) | (((pz >> level) & 1u) << 3);
         if (off >= words) return 0u;
         const u32 m = dag[off];
@@ -78,14 +80,80 @@ __device__ inline u32 walk_dag(const u32* __restrict__ dag, u32 words, int dl, u
     const u32 w = dag[idx];
     return (w >> (bit & 31u)) & 1u;
 }
+__device__ inline u32 walk_dag(const u32* __restrict__ dag, u32 words, int dl, u32 leaf, u32 base, u32 px, u32 py,
+                               u32 pz) {
+    return walk_from(dag, words, leaf, base, base, dl - 2, px, py, pz);
+}
 
-// traverse.cs:75-88: the grid cell, its sentinels, then the walk.
+// A container's top table: for every grid cell and every cell of its DAG's level dl-2-K (K = topk),
+// the result of walking the K levels above it: the node offset there, or the value a walk that
+// stopped earlier returns (kTopShadow 0, kTopVisible 1, kTopThree 3 for a child state 3, which no
+// valid DAG holds).  Entry (cell, t) with t = z * 4^K + y * 2^K + x over the K path bits below
+// the cell's.  A lookup reads one entry instead of walking K levels: the same result by
+// construction.  Valid offsets lie below kTopThree (the container checks its word count).
+constexpr u32 kTopShadow = 0xFFFFFFFFu, kTopVisible = 0xFFFFFFFEu, kTopThree = 0xFFFFFFFDu;
+
+__global__ __launch_bounds__(256) void k_build_top(const u32* __restrict__ dag, u32 words, const u32* __restrict__ grid,
+                                                   u64 entries, int dl, u32 K, u32* __restrict__ top) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= entries) return;
+    const u32 msk = (1u << K) - 1u;
+    const u32 t = (u32)(i & ((1ull << (3 * K)) - 1ull));
+    const u32 tx = t & msk, ty = (t >> K) & msk, tz = t >> (2 * K);
+    const u32 base = grid[i >> (3 * K)];
+    u32 e = base;  // a sentinel cell keeps its sentinel
+    if (base < kTopVisible) {
+        u32 off = base;
+        e = kTopShadow;  // a read past the words ends the walk as shadow, as walk_from does
+        bool done = false;
+        for (int b = (int)K - 1; b >= 0 && !done; --b) {
+            const u32 ci = (((tx >> b) & 1u) << 1) | (((ty >> b) & 1u) << 2) | (((tz >> b) & 1u) << 3);
+            if (off >= words) {
+                done = true;
+                break;
+            }
+            const u32 v = (dag[off] >> ci) & 3u;
+            if (v != 2u) {
+                e = (v == 0u) ? kTopShadow : (v == 1u) ? kTopVisible : kTopThree;
+                done = true;
+                break;
+            }
+            const u32 co = (u32)__builtin_popcount(dag[off] & 0xAAAAu & ((1u << ci) - 1u));
+            if (off + 1 + co >= words) {
+                done = true;
+                break;
+            }
+            off = base + dag[off + 1 + co];
+        }
+        if (!done) e = (off < words) ? off : kTopShadow;  // walk_from's first check on it
+    }
+    top[i] = e;
+}
+
+int launch_build_top(const u32* dag, u32 words, const u32* grid, u32 cells, int dl, u32 K, u32* top, hipStream_t s) {
+    const u64 entries = (u64)cells << (3 * K);
+    CPVS_LAUNCH("build_top", k_build_top, dim3((u32)((entries + 255) / 256)), dim3(256), 0, s, dag, words, grid, entries,
+                dl, K, top);
+    CPVS_HIP_CHECK(hipGetLastError());
+    return CPVS_OK;
+}
+
+// traverse.cs:75-88: the grid cell, its sentinels, then the walk (from the top table's entry when
+// the container has one).
 __device__ inline u32 walk(const GridDag& g, u32 px, u32 py, u32 pz) {
     const u32 G = 1u << g.gl;
     const u32 sh = (u32)(g.dl - 1);
-    const u32 base = g.grid[((pz >> sh) * G + (py >> sh)) * G + (px >> sh)];
+    const u32 cell = ((pz >> sh) * G + (py >> sh)) * G + (px >> sh);
+    const u32 base = g.grid[cell];
     if (base == 0xFFFFFFFFu) return 0u;
     if (base == 0xFFFFFFFEu) return 1u;
+    if (g.topk) {
+        const u32 K = g.topk, s = sh - K, msk = (1u << K) - 1u;
+        const u32 t = (((pz >> s) & msk) << (2 * K)) | (((py >> s) & msk) << K) | ((px >> s) & msk);
+        const u32 e = g.top[((size_t)cell << (3 * K)) + t];
+        if (e >= kTopThree) return (e == kTopShadow) ? 0u : (e == kTopVisible) ? 1u : 3u;
+        return walk_from(g.dag, g.words, g.leaf, base, e, g.dl - 2 - (int)K, px, py, pz);
+    }
     return walk_dag(g.dag, g.words, g.dl, g.leaf, base, px, py, pz);
 }
 
@@ -136,10 +204,10 @@ __global__ __launch_bounds__(kLookupThreads) void k_evaluate_pcf(GridDag g, Mat4
     vis[i] = (float)lit / (float)(f * f);
 }
 
-int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos, u32 stride,
-                        u64 count, const Mat4& m, u32 f, float* vis, hipStream_t s) {
+int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+                        const float* pos, u32 stride, u64 count, const Mat4& m, u32 f, float* vis, hipStream_t s) {
     if (count == 0) return CPVS_OK;
-    GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u};
+    GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u, top, top ? topk : 0u};
     const u64 blocks = (count + kLookupThreads - 1) / kLookupThreads;
     if (blocks > 0x7FFFFFF0ull) {
         set_error("evaluate_pcf: %llu points exceed one launch", (unsigned long long)count);
@@ -186,10 +254,10 @@ __global__ __launch_bounds__(256) void k_evaluate_image(GridDag g, Mat4 M, const
     vis[i] = (uint8_t)lookup_point(g, M, q.x, q.y, q.z);
 }
 
-int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos, u32 stride,
-                    u64 count, const Mat4& m, uint8_t* vis, hipStream_t s) {
+int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+                    const float* pos, u32 stride, u64 count, const Mat4& m, uint8_t* vis, hipStream_t s) {
     if (count == 0) return CPVS_OK;
-    GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u};
+    GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u, top, top ? topk : 0u};
     const u64 blocks = (count + kLookupThreads - 1) / kLookupThreads;
     if (blocks > 0x7FFFFFF0ull) {
         set_error("evaluate: %llu points exceed one launch", (unsigned long long)count);
@@ -205,10 +273,10 @@ int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, 
     return CPVS_OK;
 }
 
-int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const float* pos,
-                          u32 width, u32 height, const Mat4& m, uint8_t* vis, hipStream_t s) {
+int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+                          const float* pos, u32 width, u32 height, const Mat4& m, uint8_t* vis, hipStream_t s) {
     if (width == 0 || height == 0) return CPVS_OK;
-    GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u};
+    GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u, top, top ? topk : 0u};
     const u64 gx = ceil_div<u32>(width, 32), nblocks = gx * ceil_div<u32>(height, 8);
     if (nblocks > 0x7FFFFFF0ull) {
         set_error("evaluate_image: %ux%u pixels exceed one launch", width, height);

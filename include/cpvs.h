@@ -156,6 +156,22 @@ int cpvs_container_set_filter_size(cpvs_container *c, uint32_t size);
  * 0xFFFFFFFE (visible), the values shader/traverse.cs:85-88 tests.  Every slot holds a DAG of the
  * same numLevels and leafmask mode (one traverse.cs walks them all). */
 int cpvs_container_copy_to_gpu(cpvs_container *c);
+/* No counterpart in the reference (its lookup walks every level, shader/traverse.cs:89-133).
+ * copy_to_gpu also builds a top table: per grid cell, the walk's state after its first K levels,
+ * which the lookups read in place of walking them (same results).  levels -1 (default): K
+ * automatic (up to 7 levels within 4 MiB, one XCD's L2); 0: no table; 1..7: K = min(levels, the DAGs' inner
+ * levels), a table of at most 1 GiB.  Takes effect at the next copy_to_gpu.  top_levels reports
+ * the K of the table built (0: none). */
+int cpvs_container_set_top_levels(cpvs_container *c, int levels);
+int cpvs_container_top_levels(const cpvs_container *c, int *levels);
+/* Tunes the top table on the caller's lookups (no counterpart in the reference): for each
+ * candidate levels value (as cpvs_container_set_top_levels takes), copy_to_gpu and the best of
+ * reps timed cpvs_container_evaluate calls after one warm-up on the context stream.  Keeps the
+ * fastest (set and built), returns its index in *best; ms[k] (optional) gets candidate k's time.
+ * Needs the DAGs on the host side (before free_on_cpu). */
+int cpvs_container_tune_top_levels(cpvs_container *c, const float *positions, uint32_t stride_floats, uint64_t count,
+                                   const float *light_view_proj, uint8_t *vis, const int *candidates,
+                                   uint32_t ncand, uint32_t reps, int *best, double *ms);
 /* freeOnCPU / moveToGPU (src/CompressedShadowContainer.h:55-68). */
 int cpvs_container_free_on_cpu(cpvs_container *c);
 int cpvs_container_move_to_gpu(cpvs_container *c);
