@@ -52,11 +52,11 @@ def table_slots(n):
 
 def leaf_waves_bytes(N, nodes_leaf):
     """Algorithmic bytes of k_leaf_waves: the 8x8 depth block of every level-3 column (256 B) and,
-    per leaf node, its breadth-first index read and its run-head index and leafmask slot written
-    (12 B).  The first node of each run also writes its mask and leafmasks; those bytes are left
-    out, so the figure is a lower bound.  The kernel also clears the leaf level's merge table
-    (leaf_table_clear_bytes), which is not part of the leaf computation."""
-    return 256 * (N // 8) ** 2 + 12 * nodes_leaf
+    per leaf node, its breadth-first index read and its run head's index written (8 B).  The first
+    node of each run also writes its mask and leafmasks; those bytes are left out, so the figure is
+    a lower bound.  The kernel also clears the leaf level's merge table (leaf_table_clear_bytes),
+    which is not part of the leaf computation."""
+    return 256 * (N // 8) ** 2 + 8 * nodes_leaf
 
 
 def leaf_table_clear_bytes(nodes_leaf):
@@ -638,6 +638,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--N", type=int, default=8192)
     ap.add_argument("--boxes", type=int, default=512)
+    ap.add_argument("--top-levels", type=int, default=None,
+                    help="lookup top-table setting of the headline container (default: tuned on the queries)")
     ap.add_argument("--width", type=int, default=3840)
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -700,12 +702,15 @@ def main():
 
     # The container's lookup top table, tuned on these queries before the steps (untimed): every
     # step's container is built with the setting it picks.
-    mm = cp.MinMaxHierarchy(depth, borrow=True)
-    c = cp.CompressedShadowContainer(cp.CompressedShadow.create(mm))
-    top_set, top_ms = c.tuneTopLevels(queries, candidates=(-1, 0, 4, 5, 6, 7), visibilities=vis)
-    c.close()
-    mm.close()
-    log(f"[bench] top table: tuner picked {top_set} ({ {k: round(v, 4) for k, v in top_ms.items()} } ms)")
+    if args.top_levels is None:
+        mm = cp.MinMaxHierarchy(depth, borrow=True)
+        c = cp.CompressedShadowContainer(cp.CompressedShadow.create(mm))
+        top_set, top_ms = c.tuneTopLevels(queries, candidates=(-1, 0, 4, 5, 6, 7), visibilities=vis)
+        c.close()
+        mm.close()
+        log(f"[bench] top table: tuner picked {top_set} ({ {k: round(v, 4) for k, v in top_ms.items()} } ms)")
+    else:
+        top_set, top_ms = args.top_levels, {}
 
     build_ms, lookup_ms, step_ms, container_ms = [], [], [], []
     words = None
@@ -825,12 +830,14 @@ def main():
     tiled = None
     if not args.no_tiled:
         tiled = run_tiled(cp, np, ctx)
+        log(f"[bench] released {ctx.release_scratch() >> 20} MiB of scratch")
         log(f"[bench] tiled: {tiled}")
 
     batched = None
     if not args.no_batched:
         batched = run_batched(cp, np, ctx)
         log(f"[bench] batched: {batched}")
+        log(f"[bench] released {ctx.release_scratch() >> 20} MiB of scratch")
 
     serving = None
     if not args.no_serving:

@@ -371,6 +371,15 @@ int cpvs_server_create(cpvs_ctx* ctx, uint32_t lanes, uint64_t hbm_budget_bytes,
     s->ctx = ctx;
     s->serial = serial != 0;
     s->budget = hbm_budget_bytes;
+    {  // memory the process already holds (other contexts' scratch, caller buffers) is not there to give
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const uint64_t avail = free_b > (1ull << 30) ? free_b - (1ull << 30) : 0;
+            if (avail < s->budget) s->budget = avail ? avail : 1;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
     const uint32_t nl = s->serial ? 1 : lanes;
     int rc = CPVS_OK;
     for (uint32_t i = 0; i < nl && rc == CPVS_OK; ++i) {

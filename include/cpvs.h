@@ -297,7 +297,8 @@ int cpvs_dag_evaluate(const cpvs_dag *dag, const float *ndc, uint32_t stride, ui
  * request completes.  Builds run on `lanes` concurrent streams; every lookup pending when the
  * lookup stream frees up goes into one launch; DAGs stay resident, named by their build's ticket,
  * within hbm_budget_bytes (resident DAGs + builds in flight, each reserving its scratch growth
- * and an upper bound of its DAG from the exact node counts).  Least recently used DAGs without
+ * and an upper bound of its DAG from the exact node counts).  The budget is capped at creation to
+ * the device memory then free, less 1 GiB (cpvs_server_stats reports the budget in force).  Least recently used DAGs without
  * queued lookups are evicted to admit a build; a build that cannot fit fails with CPVS_EBUDGET.
  * Resident bytes are the DAGs' buffer sizes; the lanes' pyramid-level and scratch arenas count as
  * lane scratch, and their growth is admitted before it happens.  An evicted or released DAG's

@@ -165,6 +165,22 @@ def test_build_that_cannot_fit_fails(ctx, work):
         srv.close()
 
 
+def test_budget_capped_at_free_memory(ctx):
+    """A budget beyond the device's memory is lowered to what is free at creation, less 1 GiB."""
+    _say("server budget cap")
+    srv = cp.Server(lanes=1, budget_bytes=1 << 50)
+    try:
+        b = srv.stats()["budget"]
+        assert 0 < b < (1 << 40)  # no device holds 1 TiB
+    finally:
+        srv.close()
+    small = cp.Server(lanes=1, budget_bytes=1 << 30)
+    try:
+        assert small.stats()["budget"] == 1 << 30
+    finally:
+        small.close()
+
+
 def test_bad_requests_are_rejected(ctx, work):
     maps, depths, qh, q, plain = work
     srv = cp.Server(lanes=1)
