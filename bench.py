@@ -378,6 +378,25 @@ def run_batched(cp, np, ctx, maps=256, kmin=8, kmax=13, qper=1 << 20, reps=3):
         cp._lib.call("cpvs_gen_queries_uniform_range", ctx.handle, ctypes.c_void_p(q_mm.ptr.value + 12 * qper * i),
                      seed + 17, i, maps, qper, 3, None)
     vis_mm = cp.DeviceArray((nq,), np.uint8, ctx)
+    # Each DAG's lookup top table, one setting for all, picked by timing the batched lookups
+    # (untimed); the plain and the batched lookups below both read the tables.
+    top_ms = {}
+    for k in (0, 2, 3, 4, 5):
+        for sh in batch:
+            sh.setTopLevels(k)
+        ds = cp.DagSet(batch)
+        ds.evaluate(q_rr, None, vis_rr, stream)
+        e0 = ctx.event(stream)
+        ds.evaluate(q_rr, None, vis_rr, stream)
+        e1 = ctx.event(stream)
+        top_ms[k] = ctx.elapsed_ms(e0, e1)
+        for e in (e0, e1):
+            ctx.destroy_event(e)
+        ds.close()
+    top_set = min(top_ms, key=top_ms.get)
+    for sh in batch:
+        sh.setTopLevels(top_set)
+    log(f"[bench] batched: DAG top tables {top_set} levels ({ {k: round(v, 2) for k, v in top_ms.items()} } ms)")
     dset = cp.DagSet(batch)
     lk = {"plain": [], "plain_grouped": [], "batched": []}
 
@@ -425,6 +444,7 @@ def run_batched(cp, np, ctx, maps=256, kmin=8, kmax=13, qper=1 << 20, reps=3):
             "lookup_plain_pregrouped_G_per_s": round(rate["plain_grouped"], 3),
             "lookup_batched_G_per_s": round(rate["batched"], 3),
             "lookup_speedup": round(rate["batched"] / rate["plain"], 3),
+            "lookup_top_levels": top_set, "lookup_top_tune_ms": {str(k): round(v, 3) for k, v in top_ms.items()},
             "lookups_identical": same_lookup}
 
 

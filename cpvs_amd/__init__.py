@@ -324,6 +324,15 @@ class CompressedShadow:
         """CompressedShadow::traverse(vec3 ndc, bool tryLeafmasks) (src/CompressedShadow.cpp:404-463)."""
         return int(self.traverse_many(np.asarray(position, np.float32).reshape(1, 3), tryLeafmasks)[0])
 
+    def setTopLevels(self, levels: int):
+        """The DAG's own lookup top table (cpvs_dag_set_top_levels): -1 automatic, 0 none, 1..7."""
+        call("cpvs_dag_set_top_levels", self._h(), levels)
+
+    def top_levels(self) -> int:
+        k = C.c_int()
+        call("cpvs_dag_top_levels", self._h(), C.byref(k))
+        return k.value
+
     def evaluate(self, ndc: DeviceArray, visibilities: Optional[DeviceArray] = None, count: Optional[int] = None,
                  offset: int = 0, stride: Optional[int] = None, stream=None) -> DeviceArray:
         """Device lookup of NDC points as a one-DAG container with an identity light matrix

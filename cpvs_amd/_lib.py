@@ -62,6 +62,8 @@ _SIGS = {
     "cpvs_container_get": [vp, u32, u32, u32, PP],
     "cpvs_container_set_filter_size": [vp, u32],
     "cpvs_container_set_top_levels": [vp, C.c_int],
+    "cpvs_dag_set_top_levels": [vp, C.c_int],
+    "cpvs_dag_top_levels": [vp, C.POINTER(C.c_int)],
     "cpvs_container_top_levels": [vp, C.POINTER(C.c_int)],
     "cpvs_container_tune_top_levels": [vp, vp, u32, u64, vp, vp, vp, u32, u32, C.POINTER(C.c_int), vp],
     "cpvs_container_copy_to_gpu": [vp],

@@ -630,10 +630,58 @@ void dag_destroy_idle(cpvs_dag* d) {
 
 extern "C" {
 
+int cpvs_dag_set_top_levels(cpvs_dag* d, int levels) {
+    CHECK_ARG(d, "dag_set_top_levels: NULL DAG");
+    CHECK_ARG(levels >= -1 && levels <= kMaxTopLevels, "dag_set_top_levels: %d levels (-1..%d)", levels, kMaxTopLevels);
+    // As a container's table over one cell (cpvs_container_copy_to_gpu): at most the DAG's inner
+    // levels above the last; automatic: the largest within kTopTableBytes, none below 3 levels.
+    const int lead = d->n - 1 - (d->leafmasks ? 3 : 1);
+    int K = (levels < 0) ? std::min(lead, kMaxTopLevels) : std::min(levels, lead);
+    if (levels < 0) {
+        while (K > 0 && (sizeof(u32) << (3 * K)) > kTopTableBytes) --K;
+        if (K < 3) K = 0;
+    }
+    // A root that is all lit or all shadow is answered before any walk; offsets must stay below
+    // the table's sentinels.
+    if (d->root_mask == 0u || d->root_mask == 0x5555u || d->words >= 0xFFFFFFFDull) K = 0;
+    cpvs_ctx* ctx = d->ctx;
+    GUARD_BEGIN
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CPVS_HIP_CHECK(hipSetDevice(ctx->device));
+    u32* nt = nullptr;
+    if (K > 0) {
+        nt = static_cast<u32*>(ctx->buffers.take(sizeof(u32) << (3 * K)));
+        if (!nt) {
+            set_error("dag_set_top_levels: allocation of a %d-level table failed", K);
+            return CPVS_ENOMEM;
+        }
+        int rc = launch_build_top(d->d_words, (u32)d->words, nullptr, 1u, d->n, (u32)K, nt, ctx->stream);
+        if (rc == CPVS_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = CPVS_EHIP;
+        if (rc) {
+            ctx->buffers.give(nt);
+            if (rc == CPVS_EHIP) set_error("dag_set_top_levels: building the table failed");
+            return rc;
+        }
+    }
+    if (d->d_top) d->retired_tops.push_back(d->d_top);
+    d->d_top = nt;
+    d->top_levels = (u32)K;
+    return CPVS_OK;
+    GUARD_END
+}
+
+int cpvs_dag_top_levels(const cpvs_dag* d, int* levels) {
+    CHECK_ARG(d && levels, "dag_top_levels: NULL argument");
+    *levels = d->d_top ? (int)d->top_levels : 0;
+    return CPVS_OK;
+}
+
 int cpvs_dag_destroy(cpvs_dag* d) {
     if (!d) return CPVS_OK;
     hipSetDevice(d->ctx->device);
     if (d->d_words) d->ctx->buffers.give(d->d_words);
+    if (d->d_top) d->ctx->buffers.give(d->d_top);
+    for (u32* t : d->retired_tops) d->ctx->buffers.give(t);
     delete d;
     return CPVS_OK;
 }

@@ -168,6 +168,9 @@ struct cpvs_dag {
     int merge_mode = CPVS_MERGE_TABLE;        // merge path that produced the words
     uint32_t key_collisions = 0;              // 32-bit sort-key collisions resolved by scanning
     bool leafmasks = false;                   // level 2 holds leafmask nodes (lookups stop there)
+    uint32_t* d_top = nullptr;                // lookup top table (cpvs_dag_set_top_levels), or null
+    uint32_t top_levels = 0;
+    std::vector<uint32_t*> retired_tops;      // replaced tables, which DAG sets made earlier still name
 };
 
 struct cpvs_container {
@@ -263,7 +266,8 @@ int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int 
                         const float* pos, u32 stride, u64 count, const Mat4& m, u32 f, float* vis, hipStream_t s);
 int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
                           const float* pos, u32 width, u32 height, const Mat4& m, uint8_t* vis, hipStream_t s);
-// The top table of a container: cells x 8^K entries (the walk's state after K levels per cell).
+// The top table of a container: cells x 8^K entries (the walk's state after K levels per cell);
+// grid null: one DAG of its own (cells 1, base 0).
 int launch_build_top(const u32* dag, u32 words, const u32* grid, u32 cells, int dl, u32 K, u32* top, hipStream_t s);
 // One DAG of a set, walked as a one-cell container.  base is 0, or the grid sentinel
 // 0xFFFFFFFF / 0xFFFFFFFE when the root is all shadow / all lit.
@@ -273,6 +277,8 @@ struct DagDesc {
     u32 base;
     u32 dl;    // numLevels
     u32 leaf;  // 1: leafmask nodes at level 2; 0: the DAG goes down to level 0
+    const u32* top;  // the DAG's lookup top table (launch_build_top, one cell), or null
+    u32 topk;        // its levels
 };
 // The descriptor of a built DAG (createTopLevelGrid sentinels for a one-cell grid,
 // src/CompressedShadowContainer.cpp:71-91).
@@ -283,6 +289,8 @@ inline DagDesc dag_desc(const cpvs_dag* d) {
     r.base = (d->root_mask == 0u) ? 0xFFFFFFFFu : (d->root_mask == 0x5555u) ? 0xFFFFFFFEu : 0u;
     r.dl = (u32)d->n;
     r.leaf = d->leafmasks ? 1u : 0u;
+    r.top = d->d_top;
+    r.topk = d->d_top ? d->top_levels : 0u;
     return r;
 }
 constexpr uint8_t kBadId = 0xFF;  // visibility written for a query whose DAG id is outside the set

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_build_top(const u32* __restrict__ dag, 
     const u32 msk = (1u << K) - 1u;
     const u32 t = (u32)(i & ((1ull << (3 * K)) - 1ull));
     const u32 tx = t & msk, ty = (t >> K) & msk, tz = t >> (2 * K);
-    const u32 base = grid[i >> (3 * K)];
+    const u32 base = grid ? grid[i >> (3 * K)] : 0u;
     u32 e = base;  // a sentinel cell keeps its sentinel
     if (base < kTopVisible) {
         u32 off = base;
@@ -360,6 +360,13 @@ __device__ inline uint8_t dagset_lookup(const DagDesc& d, const float* __restric
     const u32 px = quantise(q[0], R), py = quantise(q[1], R), pz = quantise(q[2], R);
     if (d.base == 0xFFFFFFFFu) return 0;
     if (d.base == 0xFFFFFFFEu) return 1;
+    if (d.topk) {  // the DAG's top table: the first topk levels in one read (as a one-cell container's)
+        const u32 K = d.topk, s = d.dl - 1 - K, msk = (1u << K) - 1u;
+        const u32 t = (((pz >> s) & msk) << (2 * K)) | (((py >> s) & msk) << K) | ((px >> s) & msk);
+        const u32 e = d.top[t];
+        if (e >= kTopThree) return (e == kTopShadow) ? 0 : (e == kTopVisible) ? 1 : 3;
+        return (uint8_t)walk_from(d.words, d.nwords, d.leaf, 0u, e, (int)d.dl - 2 - (int)K, px, py, pz);
+    }
     return (uint8_t)walk_dag(d.words, d.nwords, (int)d.dl, d.leaf, 0u, px, py, pz);
 }
 

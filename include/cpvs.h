@@ -290,6 +290,14 @@ int cpvs_dagset_destroy(cpvs_dagset *s);
 /* The same for one DAG. */
 int cpvs_dag_evaluate(const cpvs_dag *dag, const float *ndc, uint32_t stride, uint64_t count, uint8_t *vis,
                       void *stream);
+/* No counterpart in the reference: a lookup top table of the DAG's own, as a one-cell container
+ * builds (cpvs_container_set_top_levels; levels -1 automatic, 0 none, 1..7), read by
+ * cpvs_dag_evaluate, by DAG sets created after the call and by server lookups; results are the
+ * same with any setting.  Built on the context stream, synchronously; call it while no lookup on
+ * the DAG is queued.  A replaced table is kept until the DAG is destroyed (DAG sets created earlier
+ * name it).  Its bytes are not counted in a server's budget.  top_levels reports the levels built. */
+int cpvs_dag_set_top_levels(cpvs_dag *d, int levels);
+int cpvs_dag_top_levels(const cpvs_dag *d, int *levels);
 
 /* ---- Serving: a request loop over builds and lookups -----------------------------------------
  * No counterpart in the reference (its renderer builds at start-up and evaluates once per frame,

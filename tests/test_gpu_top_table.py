@@ -31,10 +31,13 @@ def _expected_k(req, dl, cells, leaf):
     return max(k, 0)
 
 
-def _queries(seed, n):
+def _queries(seed, n, finite=False):
+    """Positions in [-1.1, 1.1)^3 (w = 1), 5% of the coordinates replaced by special values (only
+    finite ones with finite=True: the DAG-set lookups take NDC as given, while the oracle's identity
+    product turns an infinity into NaN)."""
     rng = np.random.default_rng(seed)
     q = (rng.random((n, 4), dtype=np.float32) * 2.2 - 1.1).astype(np.float32)
-    special = np.array([np.nan, np.inf, -np.inf, -1.0, 1.0, 0.0], np.float32)
+    special = np.array([-1.0, 1.0, 0.0] if finite else [np.nan, np.inf, -np.inf, -1.0, 1.0, 0.0], np.float32)
     pick = rng.random(q.shape) < 0.05
     q[pick] = special[rng.integers(0, special.size, q.shape)][pick]
     q[:, 3] = 1
@@ -166,3 +169,67 @@ def test_tuner_keeps_the_fastest_and_results(ctx):
         c.tuneTopLevels(dq)  # nothing left to rebuild the table from
     assert np.array_equal(c.evaluate(dq).numpy(), O.evaluate(w, O.top_level_grid([w]), n, 0, q, nthreads=8))
     c.close()
+
+
+def test_dag_tables_in_dag_lookups_and_sets(ctx):
+    """A DAG's own table (cpvs_dag_set_top_levels): one-DAG lookups and DAG sets, round-robin and
+    by id, for every setting; a set made before a table is replaced keeps its answers."""
+    specs = [(32, 1), (64, 2), (256, 6), (1024, 24)]
+    shadows, words, levels = [], [], []
+    for k, (N, boxes) in enumerate(specs):
+        D = O.gen_depth(N, SEED0 + 750 + k, boxes, True)
+        cs = cp.CompressedShadow.create(cp.MinMaxHierarchy(D))
+        shadows.append(cs)
+        words.append(cs.getDAG())
+        levels.append(cs.getNumLevels())
+    flat = cp.CompressedShadow.create(cp.MinMaxHierarchy(np.full((64, 64), 1.0, np.float32)))  # all lit
+    shadows.append(flat)
+    words.append(flat.getDAG())
+    levels.append(flat.getNumLevels())
+    m = len(shadows)
+    q = _queries(91, 1 << 16, finite=True)[:, :3].copy()
+    dq = cp.DeviceArray.from_numpy(q)
+    want = [O.evaluate(w, O.top_level_grid([w]), n, 0, q, nthreads=8) for w, n in zip(words, levels)]
+    ids = np.random.default_rng(4).integers(0, m, len(q)).astype(np.uint32)
+    dids = cp.DeviceArray.from_numpy(ids)
+    want_rr = np.array([want[i % m][i] for i in range(len(q))], np.uint8)
+    want_id = np.array([want[ids[i]][i] for i in range(len(q))], np.uint8)
+    before = None
+    for req in [0, 2, 3, 5, 7, -1]:
+        for cs, n in zip(shadows, levels):
+            cs.setTopLevels(req)
+            k = 0 if cs is flat else _expected_k(req, n, 1, True)
+            _say("dag top table", req, "->", cs.top_levels(), "levels", n)
+            assert cs.top_levels() == k, (req, n)
+        for cs, w in zip(shadows, want):
+            assert np.array_equal(cs.evaluate(dq).numpy(), w), req
+        s = cp.DagSet(shadows)
+        assert np.array_equal(s.evaluate(dq).numpy(), want_rr), req
+        assert np.array_equal(s.evaluate(dq, ids=dids).numpy(), want_id), req
+        if before is None:
+            before = s  # made with no tables; every later setting replaces them
+        else:
+            s.close()
+        assert np.array_equal(before.evaluate(dq).numpy(), want_rr), req
+    before.close()
+    for cs in shadows:
+        cs.close()
+
+
+def test_dag_table_without_leafmasks():
+    nctx = cp.Context(0)
+    try:
+        nctx.set_leafmasks(False)
+        D = O.gen_depth(512, SEED0 + 760, 16, True)
+        cs = cp.CompressedShadow.create(cp.MinMaxHierarchy(D, ctx=nctx))
+        w, n = cs.getDAG(), cs.getNumLevels()
+        q = _queries(92, 1 << 16, finite=True)[:, :3].copy()
+        want = O.evaluate(w, O.top_level_grid([w]), n, 0, q, nthreads=8, leafmasks=False)
+        dq = cp.DeviceArray.from_numpy(q, ctx=nctx)
+        for req in [0, 4, 7, -1]:
+            cs.setTopLevels(req)
+            assert cs.top_levels() == _expected_k(req, n, 1, False)
+            assert np.array_equal(cs.evaluate(dq).numpy(), want), req
+        cs.close()
+    finally:
+        nctx.close()
