@@ -97,6 +97,9 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
     // only ever reads back its own entries, so the arrays need no barrier).
     __shared__ int s_f[kLeafWaves][kLeafCols][64];
     __shared__ u32 s_b[kLeafWaves][kLeafCols][64];
+    // A chunk's changed-node flags: byte j for node q0 + j, bytes 64..127 a sink for marks outside
+    // the chunk (each lane writes its own sink byte).
+    __shared__ uint8_t s_chg[kLeafWaves][128];
     // The wave index is uniform across the wave; saying so keeps the column bases in scalar registers.
     const u32 lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // Workgroups are dealt to the XCDs in turn; XCD x takes the contiguous units [x*unit_chunk, ...).
@@ -203,12 +206,21 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
         auto chunk = [&](u32 q0, u32 bfs) {
             const u32 qn = min(cnt - q0, 64u);
             const int ra = qa - (int)q0;
-            u64 marks = 0;
-            if ((u32)ra < 64u) marks |= 1ull << ra;
-            if (below && (u32)(ra - 1) < 64u) marks |= 1ull << (ra - 1);
-            const u32 chg_hi = __builtin_amdgcn_readfirstlane(__ockl_wfred_or_u32((u32)(marks >> 32)));
-            const u32 chg_lo = __builtin_amdgcn_readfirstlane(__ockl_wfred_or_u32((u32)marks));
-            u64 chg = ((u64)chg_hi << 32) | chg_lo;  // wave-uniform
+            // Each lane flags its (at most two) nodes in LDS, then reads back the flag of node
+            // q0 + lane: one ballot gives the chunk's changed nodes.  LDS instructions of one wave run
+            // in program order, so the clear, the flags and the read need no wait between them;
+            // the fences keep the compiler from moving them across one another.
+            uint8_t* fl = s_chg[wave];
+            reinterpret_cast<u32*>(fl)[lane & 15u] = 0u;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            fl[((u32)ra < 64u) ? (u32)ra : 64u + lane] = 1;
+            fl[(below && (u32)(ra - 1) < 64u) ? (u32)(ra - 1) : 64u + lane] = 1;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            u64 chg = __ballot(fl[lane] != 0);  // wave-uniform
             if (q0 == 0) chg |= 1ull;                // a column's first node starts a run
             if (qn < 64) chg &= (1ull << qn) - 1ull;
             // The run heads of this chunk, one after the other (wave-uniform): mask and leafmasks (their
