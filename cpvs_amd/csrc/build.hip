@@ -226,6 +226,9 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
             // The run heads of this chunk, one after the other (wave-uniform): mask and leafmasks (their
             // key hash is computed lane-parallel by the leaf insert, which reads them anyway).
             u32 my_mask = 0;
+            // Node q0 + lane's run head: the last changed node at or before it, set head by head
+            // (lanes before the chunk's first head continue the previous chunk's run).
+            u32 my_head_bfs = head_bfs;
             for (u64 pend = chg; pend; pend &= pend - 1ull) {
                 const u32 j = (u32)__builtin_ctzll(pend);
                 const int z8 = 8 * (zs + (int)(q0 + j));
@@ -234,22 +237,19 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
                 const u32 cmax = (u32)min(max(fmax - z8 + 1, 0), 8);
                 const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
                 const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min);
-                u64 mine = 0;  // lane s gets the leafmask of partial slice cmin + s
+                u32 mine_lo = 0, mine_hi = 0;  // lane s gets the leafmask of partial slice cmin + s
                 const u32 kp = cmax - cmin;
                 for (u32 sl = 0; sl < kp; ++sl) {
                     const u64 m = __ballot(u >= (int)(cmin + sl));
-                    mine = (lane == sl) ? m : mine;
+                    mine_lo = write_lane(mine_lo, (u32)m, sl);
+                    mine_hi = write_lane(mine_hi, (u32)(m >> 32), sl);
                 }
                 // The head's leafmask row sits at its own breadth-first index.
                 const u32 hrow = __builtin_amdgcn_readlane(bfs, j);
-                if (lane < kp && hrow < nleaf) leafm[(size_t)hrow * 8 + lane] = mine;
-                my_mask = (lane == j) ? mask : my_mask;
+                if (lane < kp && hrow < nleaf) leafm[(size_t)hrow * 8 + lane] = ((u64)mine_hi << 32) | mine_lo;
+                my_mask = write_lane(my_mask, mask, j);
+                my_head_bfs = (lane >= j) ? hrow : my_head_bfs;
             }
-            // Node q0 + lane's run head: the last changed node at or before it.
-            const u64 upto = chg & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
-            const int src = upto ? 63 - __builtin_clzll(upto) : 0;
-            const u32 hb = (u32)__shfl((int)bfs, src, kWave);
-            const u32 my_head_bfs = upto ? hb : head_bfs;
             head_bfs = __builtin_amdgcn_readlane(my_head_bfs, qn - 1);
             // nleaf bounds every index (only exceeded if the column map were inconsistent).
             if (lane < qn && bfs < nleaf) {

@@ -236,6 +236,14 @@ __device__ inline u64 leaf_key_hash(u32 m, const u64* __restrict__ slot8) {
     return hs.value();
 }
 
+// Lane l of a vector value takes the wave-uniform v (v_writelane_b32, an ALU instruction this
+// compiler has no builtin for); every other lane keeps `old`.  Works whatever the exec mask.  The
+// lane select goes through M0: one instruction may read only one SGPR besides M0.
+__device__ inline u32 write_lane(u32 old, u32 v, u32 l) {
+    asm("s_mov_b32 m0, %2\n\tv_writelane_b32 %0, %1, m0" : "+v"(old) : "s"(v), "s"(l) : "m0");
+    return old;
+}
+
 constexpr u32 kXcds = 8;
 constexpr u32 kLeafCols = 8;
 constexpr u32 kLeafWaves = 1;
