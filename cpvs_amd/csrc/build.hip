@@ -183,10 +183,10 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
         s_b[wave][k][lane] = b[k];
     }
 
-    for (u32 k = 0; k < kLeafCols; ++k) {
-        if (c0 + k >= nvalid) break;
+    // The wave's columns with leaf nodes (ci is zero for lanes past the launch's columns).
+    for (u32 todo = (u32)__ballot(ci.x != 0u); todo; todo &= todo - 1u) {
+        const u32 k = (u32)__builtin_ctz(todo);
         const u32 cnt = __builtin_amdgcn_readlane(ci.x, k);
-        if (cnt == 0) continue;
         const u32 base = __builtin_amdgcn_readlane(ci.y, k);
         const int zs = (int)__builtin_amdgcn_readlane(ci.z, k);
         const int f = s_f[wave][k][lane];
@@ -210,19 +210,19 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
             // q0 + lane: one ballot gives the chunk's changed nodes.  LDS instructions of one wave run
             // in program order, so the clear, the flags and the read need no wait between them;
             // the fences keep the compiler from moving them across one another.
+            // Only the chunk's qn nodes are flagged.
             uint8_t* fl = s_chg[wave];
             reinterpret_cast<u32*>(fl)[lane & 15u] = 0u;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            fl[((u32)ra < 64u) ? (u32)ra : 64u + lane] = 1;
-            fl[(below && (u32)(ra - 1) < 64u) ? (u32)(ra - 1) : 64u + lane] = 1;
+            fl[((u32)ra < qn) ? (u32)ra : 64u + lane] = 1;
+            fl[(below && (u32)(ra - 1) < qn) ? (u32)(ra - 1) : 64u + lane] = 1;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             u64 chg = __ballot(fl[lane] != 0);  // wave-uniform
             if (q0 == 0) chg |= 1ull;                // a column's first node starts a run
-            if (qn < 64) chg &= (1ull << qn) - 1ull;
             // The run heads of this chunk, one after the other (wave-uniform): mask and leafmasks (their
             // key hash is computed lane-parallel by the leaf insert, which reads them anyway).
             u32 my_mask = 0;

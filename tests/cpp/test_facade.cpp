@@ -15,6 +15,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "cpvs_amd.hpp"
 
@@ -140,6 +141,37 @@ int main(int argc, char** argv) {
         if (!in || in->getDAG() != d.getDAG() || in->getNumLevels() != d.getNumLevels() ||
             in->getTotalVisibility() != d.getTotalVisibility()) {
             std::printf("FAIL container get\n");
+            ++fails;
+        }
+        // The lookup top table (no counterpart in the reference): same visibilities with and
+        // without it, over a grid of positions in and around the unit cube.
+        const int G = 24, n = G * G * G;
+        std::vector<float> pos(4 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            pos[4 * i + 0] = -1.1f + 2.2f * (float)(i % G) / (G - 1);
+            pos[4 * i + 1] = -1.1f + 2.2f * (float)((i / G) % G) / (G - 1);
+            pos[4 * i + 2] = -1.1f + 2.2f * (float)(i / (G * G)) / (G - 1);
+            pos[4 * i + 3] = 1.0f;
+        }
+        cpvs_ctx* cx = Context::instance().get();
+        void *dpos = nullptr, *dvis = nullptr;
+        check(cpvs_device_alloc(cx, 16ull * n, &dpos), "cpvs_device_alloc");
+        check(cpvs_device_alloc(cx, (uint64_t)n, &dvis), "cpvs_device_alloc");
+        check(cpvs_memcpy_h2d(cx, dpos, pos.data(), 16ull * n), "cpvs_memcpy_h2d");
+        std::vector<uint8_t> v0(n), v2(n);
+        c.setTopLevels(0);
+        c.copyToGPU();
+        c.evaluate(static_cast<const float*>(dpos), 4, n, nullptr, static_cast<uint8_t*>(dvis));
+        check(cpvs_memcpy_d2h(cx, v0.data(), dvis, n), "cpvs_memcpy_d2h");
+        c.setTopLevels(2);  // 32^2: two inner levels above the last
+        c.copyToGPU();
+        c.evaluate(static_cast<const float*>(dpos), 4, n, nullptr, static_cast<uint8_t*>(dvis));
+        check(cpvs_memcpy_d2h(cx, v2.data(), dvis, n), "cpvs_memcpy_d2h");
+        cpvs_device_free(cx, dpos);
+        cpvs_device_free(cx, dvis);
+        ++cases;
+        if (c.topLevels() != 2 || v0 != v2) {
+            std::printf("FAIL container top table (levels %d)\n", c.topLevels());
             ++fails;
         }
         c.moveToGPU();
