@@ -229,12 +229,17 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
             // Node q0 + lane's run head: the last changed node at or before it, set head by head
             // (lanes before the chunk's first head continue the previous chunk's run).
             u32 my_head_bfs = head_bfs;
-            for (u64 pend = chg; pend; pend &= pend - 1ull) {
+            // Node q0 + j lies at z8 = zb + 8j; its lit and partial slice counts come from the
+            // column extremes relative to it.
+            const int zb = 8 * (zs + (int)q0);
+            const int fmin1 = fmin + 1 - zb, fmax1 = fmax + 1 - zb, fb = f - zb;
+            for (u64 pend = chg; pend;) {
                 const u32 j = (u32)__builtin_ctzll(pend);
-                const int z8 = 8 * (zs + (int)(q0 + j));
-                const int u = f - z8;
-                const u32 cmin = (u32)min(max(fmin - z8 + 1, 0), 8);
-                const u32 cmax = (u32)min(max(fmax - z8 + 1, 0), 8);
+                pend &= ~(1ull << j);
+                const int j8 = 8 * (int)j;
+                const int u = fb - j8;
+                const u32 cmin = (u32)min(max(fmin1 - j8, 0), 8);
+                const u32 cmax = (u32)min(max(fmax1 - j8, 0), 8);
                 const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
                 const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min);
                 u32 mine_lo = 0, mine_hi = 0;  // lane s gets the leafmask of partial slice cmin + s

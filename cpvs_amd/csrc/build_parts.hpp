@@ -238,9 +238,16 @@ __device__ inline u64 leaf_key_hash(u32 m, const u64* __restrict__ slot8) {
 
 // Lane l of a vector value takes the wave-uniform v (v_writelane_b32, an ALU instruction this
 // compiler has no builtin for); every other lane keeps `old`.  Works whatever the exec mask.  The
-// lane select goes through M0: one instruction may read only one SGPR besides M0.
+// lane select goes through M0 and the value through a plain SGPR (the instruction takes neither a
+// second SGPR nor VCC beside M0).
+// v and l must be wave-uniform; readfirstlane says so to the compiler (and costs nothing when
+// they already sit in SGPRs).
 __device__ inline u32 write_lane(u32 old, u32 v, u32 l) {
-    asm("s_mov_b32 m0, %2\n\tv_writelane_b32 %0, %1, m0" : "+v"(old) : "s"(v), "s"(l) : "m0");
+    u32 t;
+    asm("s_mov_b32 %1, %2\n\ts_mov_b32 m0, %3\n\tv_writelane_b32 %0, %1, m0"
+        : "+v"(old), "=&s"(t)
+        : "s"(__builtin_amdgcn_readfirstlane(v)), "s"(__builtin_amdgcn_readfirstlane(l))
+        : "m0");
     return old;
 }
 
