@@ -62,16 +62,22 @@ __global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord,
 //   its later 64-node chunks in the loop.
 // * Which nodes differ from their predecessor follows from the texels alone.  With
 //   h_t = f_t + 8 - 8*zs, node q changes iff some h_t - 8q lies in [0, 14]: q = floor(h_t/8), or
-//   q = floor(h_t/8) - 1 when h_t mod 8 != 7.  Each texel marks at most two nodes, so a wave
-//   OR-reduction of the marks gives a chunk's changed nodes at once, and only those (the run
+//   q = floor(h_t/8) - 1 when h_t mod 8 != 7.  Each texel flags at most two nodes in LDS and a
+//   ballot over the flags read back gives a chunk's changed nodes at once; only those (the run
 //   heads, 1.3 M of 6.7 M leaves at 8192^2) get compares.
+// * The kernel is bound by instruction issue, the scalar unit most (one per CU, shared by its
+//   four SIMDs): the per-run-head work is wave-uniform.  What cut it: the changed nodes from LDS
+//   flags and one ballot instead of two wave OR-reductions (0.262 -> 0.244 ms), lane inserts by
+//   v_writelane and run heads propagated head by head (0.239 ms), the wave's columns walked by a
+//   bitmask of those with nodes (0.224 ms), slice counts from per-chunk offsets (0.217 ms).
 // * Only a run's first node stores its mask and leafmasks: the merge and the final write read
 //   those of first occurrences only, and a key's first occurrence is always the first node of a
 //   run (breadth-first order grows with z inside a column).  Every node stores the breadth-first
 //   index of its run's first node.
 // * A run head's leafmasks go to the row at its own breadth-first index (8 slots per node), so the
 //   leaf insert, the key compares and the write find them at the node's index, with no slot map.
-// * Measured on MI355X: 8 columns per wave 0.31 ms, 4 columns 0.32 ms, 16 columns 0.42 ms; one
+// * Measured on MI355X (earlier forms of the kernel): 8 columns per wave 0.31 ms, 4 columns 0.32
+//   ms, 16 columns 0.42 ms; one
 //   column per one-wave workgroup (loads issued per column, run heads found by a ballot per node)
 //   0.55 ms; one lane per node over LDS-staged columns 0.51 ms (64 texels of work per node).
 //   Workgroups of 1, 2, 4 and 8 waves: 0.308, 0.315, 0.320 and 0.335 ms.
@@ -190,13 +196,10 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
         const u32 base = __builtin_amdgcn_readlane(ci.y, k);
         const int zs = (int)__builtin_amdgcn_readlane(ci.z, k);
         const int f = s_f[wave][k][lane];
-        int fmin, fmax;
+        int fmin = __builtin_amdgcn_readlane(fext, k), fmax = __builtin_amdgcn_readlane(fext, k + kLeafCols);
         if ((nan_cols >> k) & 1u) {
             fmin = __ockl_wfred_min_i32(f);
             fmax = __ockl_wfred_max_i32(f);
-        } else {
-            fmin = __builtin_amdgcn_readlane(fext, k);
-            fmax = __builtin_amdgcn_readlane(fext, k + kLeafCols);
         }
         const int h = f + 8 - 8 * zs;
         const int qa = h >> 3;  // floor(h / 8)
