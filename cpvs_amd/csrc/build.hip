@@ -254,7 +254,8 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
                 }
                 // The head's leafmask row sits at its own breadth-first index.
                 const u32 hrow = __builtin_amdgcn_readlane(bfs, j);
-                if (lane < kp && hrow < nleaf) leafm[(size_t)hrow * 8 + lane] = ((u64)mine_hi << 32) | mine_lo;
+                if (lane < kp && hrow < nleaf)  // the two halves stored as they are (no 64-bit assembly)
+                    *reinterpret_cast<uint2*>(leafm + (size_t)hrow * 8 + lane) = make_uint2(mine_lo, mine_hi);
                 my_mask = write_lane(my_mask, mask, j);
                 my_head_bfs = (lane >= j) ? hrow : my_head_bfs;
             }
