@@ -201,74 +201,8 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
             fmin = __ockl_wfred_min_i32(f);
             fmax = __ockl_wfred_max_i32(f);
         }
-        const int h = f + 8 - 8 * zs;
-        const int qa = h >> 3;  // floor(h / 8)
-        const bool below = (h & 7) != 7;
-        u32 head_bfs = 0;  // the run head carried into the next chunk
-        // Nodes [q0, q0 + 64) of the column; bfs = breadth-first index of node q0 + lane.
-        auto chunk = [&](u32 q0, u32 bfs) {
-            const u32 qn = min(cnt - q0, 64u);
-            const int ra = qa - (int)q0;
-            // Each lane flags its (at most two) nodes in LDS, then reads back the flag of node
-            // q0 + lane: one ballot gives the chunk's changed nodes.  LDS instructions of one wave run
-            // in program order, so the clear, the flags and the read need no wait between them;
-            // the fences keep the compiler from moving them across one another.
-            // Only the chunk's qn nodes are flagged.
-            uint8_t* fl = s_chg[wave];
-            reinterpret_cast<u32*>(fl)[lane & 15u] = 0u;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            fl[((u32)ra < qn) ? (u32)ra : 64u + lane] = 1;
-            fl[(below && (u32)(ra - 1) < qn) ? (u32)(ra - 1) : 64u + lane] = 1;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            u64 chg = __ballot(fl[lane] != 0);  // wave-uniform
-            if (q0 == 0) chg |= 1ull;                // a column's first node starts a run
-            // The run heads of this chunk, one after the other (wave-uniform): mask and leafmasks (their
-            // key hash is computed lane-parallel by the leaf insert, which reads them anyway).
-            u32 my_mask = 0;
-            // Node q0 + lane's run head: the last changed node at or before it, set head by head
-            // (lanes before the chunk's first head continue the previous chunk's run).
-            u32 my_head_bfs = head_bfs;
-            // Node q0 + j lies at z8 = zb + 8j; its lit and partial slice counts come from the
-            // column extremes relative to it.
-            const int zb = 8 * (zs + (int)q0);
-            const int fmin1 = fmin + 1 - zb, fmax1 = fmax + 1 - zb, fb = f - zb;
-            for (u64 pend = chg; pend;) {
-                const u32 j = (u32)__builtin_ctzll(pend);
-                pend &= ~(1ull << j);
-                const int j8 = 8 * (int)j;
-                const int u = fb - j8;
-                const u32 cmin = (u32)min(max(fmin1 - j8, 0), 8);
-                const u32 cmax = (u32)min(max(fmax1 - j8, 0), 8);
-                const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
-                const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min);
-                u32 mine_lo = 0, mine_hi = 0;  // lane s gets the leafmask of partial slice cmin + s
-                const u32 kp = cmax - cmin;
-                for (u32 sl = 0; sl < kp; ++sl) {
-                    const u64 m = __ballot(u >= (int)(cmin + sl));
-                    mine_lo = write_lane(mine_lo, (u32)m, sl);
-                    mine_hi = write_lane(mine_hi, (u32)(m >> 32), sl);
-                }
-                // The head's leafmask row sits at its own breadth-first index.
-                const u32 hrow = __builtin_amdgcn_readlane(bfs, j);
-                if (lane < kp && hrow < nleaf)  // the two halves stored as they are (no 64-bit assembly)
-                    *reinterpret_cast<uint2*>(leafm + (size_t)hrow * 8 + lane) = make_uint2(mine_lo, mine_hi);
-                my_mask = write_lane(my_mask, mask, j);
-                my_head_bfs = (lane >= j) ? hrow : my_head_bfs;
-            }
-            head_bfs = __builtin_amdgcn_readlane(my_head_bfs, qn - 1);
-            // nleaf bounds every index (only exceeded if the column map were inconsistent).
-            if (lane < qn && bfs < nleaf) {
-                runhead[bfs] = my_head_bfs;
-                if ((chg >> lane) & 1ull) mask_out[bfs] = my_mask;
-            }
-        };
-        // The first chunk's indices are staged; a longer column (box walls) loads the rest here.
-        chunk(0u, s_b[wave][k][lane]);
-        for (u32 q0 = 64; q0 < cnt; q0 += 64) chunk(q0, lc.bfs[base + min(q0 + lane, cnt - 1)]);
+        leaf_column(f, fmin, fmax, zs, cnt, s_b[wave][k][lane], lc.bfs + base, 0u, nleaf, s_chg[wave], lane, mask_out,
+                    leafm, runhead);
     }
     // The leaf level's merge table is cleared here, after all the work (stores issued earlier
     // would be waited for with the loads): this kernel leaves HBM bandwidth to spare.

@@ -251,6 +251,87 @@ __device__ inline u32 write_lane(u32 old, u32 v, u32 l) {
     return old;
 }
 
+// One column's leaf nodes, the walk both leaf kernels (one map's and a forest's) run per column:
+// nodes q = 0 .. cnt-1 of the column at level-2 z cells zs + q, lane = texel of the column's 8x8
+// block with f = floor(depth * H - 0.5) (fmin, fmax: the column's extremes), first_bfs = the
+// breadth-first index of node lane (the first 64 nodes), bfs_run[q] that of node q.  Writes each
+// node's run head (the breadth-first index of its run's first node), and for every run head its
+// mask (with gtag, a forest's DAG index, in the upper half) and the leafmasks of its partial slices
+// in the row at its own index (8 slots per node).  fl: the wave's 128 bytes of LDS flags.
+//
+// Node q differs from node q-1 iff some texel's h = f + 8 - 8*zs has h - 8q in [0, 14]: q =
+// floor(h/8), or floor(h/8) - 1 when h mod 8 != 7.  Each lane flags its (at most two) nodes in LDS,
+// then reads back the flag of node q0 + lane: one ballot gives a 64-node chunk's changed nodes.  LDS
+// instructions of one wave run in program order, so the clear, the flags and the read need no wait
+// between them; the fences keep the compiler from moving them across one another.
+__device__ inline void leaf_column(int f, int fmin, int fmax, int zs, u32 cnt, u32 first_bfs, const u32* bfs_run,
+                                   u32 gtag, u32 nleaf, uint8_t* fl, u32 lane, u32* __restrict__ mask_out,
+                                   u64* __restrict__ leafm, u32* __restrict__ runhead) {
+    const int h = f + 8 - 8 * zs;
+    const int qa = h >> 3;  // floor(h / 8)
+    const bool below = (h & 7) != 7;
+    u32 head_bfs = 0;  // the run head carried into the next chunk
+    // Nodes [q0, q0 + 64) of the column; bfs = breadth-first index of node q0 + lane.
+    auto chunk = [&](u32 q0, u32 bfs) {
+        const u32 qn = min(cnt - q0, 64u);
+        const int ra = qa - (int)q0;
+        // Only the chunk's qn nodes are flagged; bytes 64 + lane are each lane's sink.
+        reinterpret_cast<u32*>(fl)[lane & 15u] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        fl[((u32)ra < qn) ? (u32)ra : 64u + lane] = 1;
+        fl[(below && (u32)(ra - 1) < qn) ? (u32)(ra - 1) : 64u + lane] = 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        u64 chg = __ballot(fl[lane] != 0);  // wave-uniform
+        if (q0 == 0) chg |= 1ull;            // a column's first node starts a run
+        // The run heads of this chunk, one after the other (wave-uniform): mask and leafmasks (their
+        // key hash is computed lane-parallel by the leaf insert, which reads them anyway).
+        u32 my_mask = 0;
+        // Node q0 + lane's run head: the last changed node at or before it, set head by head
+        // (lanes before the chunk's first head continue the previous chunk's run).
+        u32 my_head_bfs = head_bfs;
+        // Node q0 + j lies at z8 = zb + 8j; its lit and partial slice counts come from the column
+        // extremes relative to it.
+        const int zb = 8 * (zs + (int)q0);
+        const int fmin1 = fmin + 1 - zb, fmax1 = fmax + 1 - zb, fb = f - zb;
+        for (u64 pend = chg; pend;) {
+            const u32 j = (u32)__builtin_ctzll(pend);
+            pend &= ~(1ull << j);
+            const int j8 = 8 * (int)j;
+            const int u = fb - j8;
+            const u32 cmin = (u32)min(max(fmin1 - j8, 0), 8);
+            const u32 cmax = (u32)min(max(fmax1 - j8, 0), 8);
+            const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
+            const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min) | gtag;
+            u32 mine_lo = 0, mine_hi = 0;  // lane s gets the leafmask of partial slice cmin + s
+            const u32 kp = cmax - cmin;
+            for (u32 sl = 0; sl < kp; ++sl) {
+                const u64 m = __ballot(u >= (int)(cmin + sl));
+                mine_lo = write_lane(mine_lo, (u32)m, sl);
+                mine_hi = write_lane(mine_hi, (u32)(m >> 32), sl);
+            }
+            // The head's leafmask row sits at its own breadth-first index.
+            const u32 hrow = __builtin_amdgcn_readlane(bfs, j);
+            if (lane < kp && hrow < nleaf)  // the two halves stored as they are (no 64-bit assembly)
+                *reinterpret_cast<uint2*>(leafm + (size_t)hrow * 8 + lane) = make_uint2(mine_lo, mine_hi);
+            my_mask = write_lane(my_mask, mask, j);
+            my_head_bfs = (lane >= j) ? hrow : my_head_bfs;
+        }
+        head_bfs = __builtin_amdgcn_readlane(my_head_bfs, qn - 1);
+        // nleaf bounds every index (only exceeded if the column map were inconsistent).
+        if (lane < qn && bfs < nleaf) {
+            runhead[bfs] = my_head_bfs;
+            if ((chg >> lane) & 1ull) mask_out[bfs] = my_mask;
+        }
+    };
+    // The first chunk's indices are staged; a longer column (box walls) loads the rest here.
+    chunk(0u, first_bfs);
+    for (u32 q0 = 64; q0 < cnt; q0 += 64) chunk(q0, bfs_run[min(q0 + lane, cnt - 1)]);
+}
+
 constexpr u32 kXcds = 8;
 constexpr u32 kLeafCols = 8;
 constexpr u32 kLeafWaves = 1;

@@ -238,6 +238,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
                                                              u32 list_cap) {
     __shared__ int s_f[kWaves][kCols][64];
     __shared__ u32 s_b[kWaves][kCols][64];
+    __shared__ uint8_t s_chg[kWaves][128];  // each wave's changed-node flags (leaf_column)
     const u32 lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 c0 = (blockIdx.x * kWaves + wave) * kCols;
     const u32 nvalid = min(*n_list, list_cap);
@@ -285,68 +286,21 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
         s_b[wave][k][lane] = b[k];
     }
 
-    for (u32 k = 0; k < kCols; ++k) {
-        if (c0 + k >= nvalid) break;
+    // The wave's columns with leaf nodes (ci is zero for lanes past the list).
+    for (u32 todo = (u32)__ballot(ci.x != 0u); todo; todo &= todo - 1u) {
+        const u32 k = (u32)__builtin_ctz(todo);
         const u32 cnt = __builtin_amdgcn_readlane(ci.x, k);
-        if (cnt == 0) continue;
         const u32 base = __builtin_amdgcn_readlane(ci.y, k);
         const int zs = (int)__builtin_amdgcn_readlane(ci.z, k);
         const u32 gtag = __builtin_amdgcn_readlane(ci.w, k) << 16;
         const int f = s_f[wave][k][lane];
-        int fmin, fmax;
+        int fmin = __builtin_amdgcn_readlane(fext, k), fmax = __builtin_amdgcn_readlane(fext, k + kCols);
         if ((nan_cols >> k) & 1u) {
             fmin = __ockl_wfred_min_i32(f);
             fmax = __ockl_wfred_max_i32(f);
-        } else {
-            fmin = __builtin_amdgcn_readlane(fext, k);
-            fmax = __builtin_amdgcn_readlane(fext, k + kCols);
         }
-        const int h = f + 8 - 8 * zs;
-        const int qa = h >> 3;
-        const bool below = (h & 7) != 7;
-        u32 head_bfs = 0;
-        auto chunk = [&](u32 q0, u32 bfs) {
-            const u32 qn = min(cnt - q0, 64u);
-            const int ra = qa - (int)q0;
-            u64 marks = 0;
-            if ((u32)ra < 64u) marks |= 1ull << ra;
-            if (below && (u32)(ra - 1) < 64u) marks |= 1ull << (ra - 1);
-            const u32 chg_hi = __builtin_amdgcn_readfirstlane(__ockl_wfred_or_u32((u32)(marks >> 32)));
-            const u32 chg_lo = __builtin_amdgcn_readfirstlane(__ockl_wfred_or_u32((u32)marks));
-            u64 chg = ((u64)chg_hi << 32) | chg_lo;
-            if (q0 == 0) chg |= 1ull;
-            if (qn < 64) chg &= (1ull << qn) - 1ull;
-            u32 my_mask = 0;
-            for (u64 pend = chg; pend; pend &= pend - 1ull) {
-                const u32 j = (u32)__builtin_ctzll(pend);
-                const int z8 = 8 * (zs + (int)(q0 + j));
-                const int u = f - z8;
-                const u32 cmin = (u32)min(max(fmin - z8 + 1, 0), 8);
-                const u32 cmax = (u32)min(max(fmax - z8 + 1, 0), 8);
-                const u32 below_min = (1u << (2 * cmin)) - 1u, below_max = (1u << (2 * cmax)) - 1u;
-                const u32 mask = (0x5555u & below_min) | (0xAAAAu & below_max & ~below_min) | gtag;
-                u64 mine = 0;
-                const u32 kp = cmax - cmin;
-                for (u32 sl = 0; sl < kp; ++sl) {
-                    const u64 m = __ballot(u >= (int)(cmin + sl));
-                    mine = (lane == sl) ? m : mine;
-                }
-                const u32 hrow = __builtin_amdgcn_readlane(bfs, j);  // the head's row: its own index
-                if (lane < kp && hrow < nleaf) leafm[(size_t)hrow * 8 + lane] = mine;
-                my_mask = (lane == j) ? mask : my_mask;
-            }
-            const u64 upto = chg & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
-            const int src = upto ? 63 - __builtin_clzll(upto) : 0;
-            const u32 hb = (u32)__shfl((int)bfs, src, kWave);
-            const u32 my_head_bfs = upto ? hb : head_bfs;
-            head_bfs = __builtin_amdgcn_readlane(my_head_bfs, qn - 1);
-            if (lane < qn && bfs < nleaf) {
-                runhead[bfs] = my_head_bfs;
-                if ((chg >> lane) & 1ull) mask_out[bfs] = my_mask;
-            }
-        };
-        chunk(0u, s_b[wave][k][lane]);
-        for (u32 q0 = 64; q0 < cnt; q0 += 64) chunk(q0, colbfs[base + min(q0 + lane, cnt - 1)]);
+        leaf_column(f, fmin, fmax, zs, cnt, s_b[wave][k][lane], colbfs + base, gtag, nleaf, s_chg[wave], lane, mask_out,
+                    leafm, runhead);
     }
     clear_table(tab, tab_slots);
 }
