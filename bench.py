@@ -963,8 +963,13 @@ def main():
         "lookup_top_table": {"setting": top_set, "levels": top_levels,
                              "tune_ms": {str(k): round(v, 4) for k, v in top_ms.items()},
                              "container_ms_p50": round(pct(container_ms, 50), 3),
+                             "build_top_ms": round(prof["build_top"][0] / prof_steps, 4) if "build_top" in prof else 0.0,
+                             "lookup_G_per_s_incl_table": round(
+                                 nq / ((pct(lookup_ms, 50) + (prof["build_top"][0] / prof_steps if "build_top" in prof
+                                                                else 0.0)) * 1e-3) / 1e9, 3),
                              "note": "copyToGPU (combineDAGs, grid, top table) between build and lookup, GPU "
-                                     "events; not in build_ms or lookup_ms"},
+                                     "events; not in build_ms or lookup_ms.  The inclusive rate counts the table's "
+                                     "build (k_build_top, once per container) with one evaluate"},
         "roofline": roofline,
         "pyramid_roofline": pyramid_roofline,
         "lookup_roofline": lookup_roofline,
