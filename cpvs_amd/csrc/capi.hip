@@ -655,6 +655,7 @@ int cpvs_dag_set_top_levels(cpvs_dag* d, int levels) {
             set_error("dag_set_top_levels: allocation of a %d-level table failed", K);
             return CPVS_ENOMEM;
         }
+        ProfInstall prof(ctx->prof_on ? &ctx->prof : nullptr);
         int rc = launch_build_top(d->d_words, (u32)d->words, nullptr, 1u, d->n, (u32)K, nt, ctx->stream);
         if (rc == CPVS_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = CPVS_EHIP;
         if (rc) {
@@ -823,6 +824,7 @@ int cpvs_container_copy_to_gpu(cpvs_container* c) {
                       (unsigned long long)entries);
             return CPVS_ENOMEM;
         }
+        ProfInstall prof(ctx->prof_on ? &ctx->prof : nullptr);
         int rc = launch_build_top(c->d_dag, (u32)total, c->d_grid, (u32)cells, dl, (u32)K, c->d_top, s);
         if (rc) return rc;
         c->top_levels = (u32)K;

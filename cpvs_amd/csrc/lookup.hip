@@ -93,47 +93,78 @@ __device__ inline u32 walk_dag(const u32* __restrict__ dag, u32 words, int dl, u
 // construction.  Valid offsets lie below kTopThree (the container checks its word count).
 constexpr u32 kTopShadow = 0xFFFFFFFFu, kTopVisible = 0xFFFFFFFEu, kTopThree = 0xFFFFFFFDu;
 
+// One thread per entry of level K-1 (the parent): the walk over the parent's K-1 path bits, then
+// the last level for its 8 children from the one node they share (one mask load, a pointer per
+// partial child).  Each child's value is the one a per-entry walk of all K bits gives, checks
+// included.
 __global__ __launch_bounds__(256) void k_build_top(const u32* __restrict__ dag, u32 words, const u32* __restrict__ grid,
-                                                   u64 entries, int dl, u32 K, u32* __restrict__ top) {
+                                                   u64 parents, u32 K, u32* __restrict__ top) {
     const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= entries) return;
-    const u32 msk = (1u << K) - 1u;
-    const u32 t = (u32)(i & ((1ull << (3 * K)) - 1ull));
-    const u32 tx = t & msk, ty = (t >> K) & msk, tz = t >> (2 * K);
-    const u32 base = grid ? grid[i >> (3 * K)] : 0u;
-    u32 e = base;  // a sentinel cell keeps its sentinel
+    if (i >= parents) return;
+    const u32 P = K - 1, pm = (1u << P) - 1u;
+    const u32 t = (u32)(i & ((1ull << (3 * P)) - 1ull));
+    const u32 px = t & pm, py = (t >> P) & pm, pz = t >> (2 * P);
+    const u64 cell = i >> (3 * P);
+    const u32 base = grid ? grid[cell] : 0u;
+    u32 e = base;      // a sentinel cell keeps its sentinel
+    bool node = false; // e is the parent's node offset (possibly past the words)
     if (base < kTopVisible) {
         u32 off = base;
         e = kTopShadow;  // a read past the words ends the walk as shadow, as walk_from does
-        bool done = false;
-        for (int b = (int)K - 1; b >= 0 && !done; --b) {
-            const u32 ci = (((tx >> b) & 1u) << 1) | (((ty >> b) & 1u) << 2) | (((tz >> b) & 1u) << 3);
+        node = true;
+        for (int b = (int)P - 1; b >= 0; --b) {
+            const u32 ci = (((px >> b) & 1u) << 1) | (((py >> b) & 1u) << 2) | (((pz >> b) & 1u) << 3);
             if (off >= words) {
-                done = true;
+                node = false;
                 break;
             }
-            const u32 v = (dag[off] >> ci) & 3u;
+            const u32 m = dag[off];
+            const u32 v = (m >> ci) & 3u;
             if (v != 2u) {
                 e = (v == 0u) ? kTopShadow : (v == 1u) ? kTopVisible : kTopThree;
-                done = true;
+                node = false;
                 break;
             }
-            const u32 co = (u32)__builtin_popcount(dag[off] & 0xAAAAu & ((1u << ci) - 1u));
+            const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
             if (off + 1 + co >= words) {
-                done = true;
+                node = false;
                 break;
             }
             off = base + dag[off + 1 + co];
         }
-        if (!done) e = (off < words) ? off : kTopShadow;  // walk_from's first check on it
+        if (node) e = off;
     }
-    top[i] = e;
+    const bool inside = node && e < words;
+    const u32 m = inside ? dag[e] : 0u;
+    u32* out = top + ((size_t)cell << (3 * K));
+#pragma unroll
+    for (u32 c = 0; c < 8; ++c) {
+        const u32 cx = c & 1u, cy = (c >> 1) & 1u, cz = c >> 2;
+        const u32 idx = (((2 * pz + cz) << (2 * K)) | ((2 * py + cy) << K)) | (2 * px + cx);
+        u32 r = node ? kTopShadow : e;  // a parent past the words: shadow, as walk_from's first check
+        if (inside) {
+            const u32 ci = (cx << 1) | (cy << 2) | (cz << 3);
+            const u32 v = (m >> ci) & 3u;
+            if (v != 2u) {
+                r = (v == 0u) ? kTopShadow : (v == 1u) ? kTopVisible : kTopThree;
+            } else {
+                const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
+                if (e + 1 + co < words) {
+                    const u32 nx = base + dag[e + 1 + co];
+                    r = (nx < words) ? nx : kTopShadow;  // walk_from's first check on it
+                }
+            }
+        }
+        out[idx] = r;
+    }
 }
 
 int launch_build_top(const u32* dag, u32 words, const u32* grid, u32 cells, int dl, u32 K, u32* top, hipStream_t s) {
-    const u64 entries = (u64)cells << (3 * K);
-    CPVS_LAUNCH("build_top", k_build_top, dim3((u32)((entries + 255) / 256)), dim3(256), 0, s, dag, words, grid, entries,
-                dl, K, top);
+    (void)dl;
+    if (K == 0) return CPVS_OK;
+    const u64 parents = (u64)cells << (3 * (K - 1));
+    CPVS_LAUNCH("build_top", k_build_top, dim3((u32)((parents + 255) / 256)), dim3(256), 0, s, dag, words, grid, parents,
+                K, top);
     CPVS_HIP_CHECK(hipGetLastError());
     return CPVS_OK;
 }
