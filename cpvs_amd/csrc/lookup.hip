@@ -200,9 +200,17 @@ __device__ inline void project(const GridDag& g, const Mat4& M, float x, float y
         r[c] = acc;
     }
     const u32 R = 1u << (g.dl + g.gl - 1);
-    px = quantise(r[0] / r[3], R);
-    py = quantise(r[1] / r[3], R);
-    pz = quantise(r[2] / r[3], R);
+    // x / 1 is x exactly (infinities and NaN included), so an affine matrix's w = 1 skips the
+    // three IEEE divides, about ten instructions each.
+    float nx = r[0], ny = r[1], nz = r[2];
+    if (r[3] != 1.0f) {
+        nx = nx / r[3];
+        ny = ny / r[3];
+        nz = nz / r[3];
+    }
+    px = quantise(nx, R);
+    py = quantise(ny, R);
+    pz = quantise(nz, R);
 }
 
 __device__ inline u32 lookup_point(const GridDag& g, const Mat4& M, float x, float y, float z) {
