@@ -577,6 +577,30 @@ def run_serving(cp, np, ctx, nreq=10000, p_build=0.3, kmin=9, kmax=14, warm_per_
                          "requests_failed": {m: v["failed"] for m, v in press.items()}}}
 
 
+def run_tiled_batch_only(cp, ctx, full=65536, tile=8192, boxes=16384, lanes=6, forest_mtexels=128, reps=2):
+    """For a kernel trace: the tiled configuration's 512 z-tile DAGs built as one batch `reps`
+    times (forests, the given lanes and forest size), after the tiles are generated.  Prints the
+    host time of each build."""
+    L = full // tile
+    tiles = {(tx, ty): cp.gen_depth(full, SEED0 + 2, boxes, True, tx * tile, ty * tile, tile)
+             for ty in range(L) for tx in range(L)}
+    keys = [(tx, ty, tz) for ty in range(L) for tx in range(L) for tz in range(L)]
+    bd, bt = [tiles[(tx, ty)] for tx, ty, _ in keys], [(tz, L) for _, _, tz in keys]
+    cfg = cp.batch_config(lanes=lanes, forest_mtexels=forest_mtexels)
+    ctx.synchronize()
+    out = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        batch, _ = cp.build_batch(bd, bt, config=cfg)
+        ctx.synchronize()
+        out.append(round((time.perf_counter() - t0) * 1e3, 2))
+        for sh in batch:
+            sh.close()
+    for d in tiles.values():
+        d.free()
+    return {"tiled_batch_ms": out, "config": cfg.as_dict()}
+
+
 def run_busy_phases(cp, np, ctx, maps=256, kmin=8, kmax=13, config=None):
     """For a kernel trace (rocprofv3 --kernel-trace): the batched configuration's 256 maps built
     plainly (one map after another on one stream), then as one batch (forests), each phase after
@@ -672,6 +696,8 @@ def main():
                     help="seconds the faithful (linear-search) create() may run at the headline map")
     ap.add_argument("--busy-phases", action="store_true",
                     help="only build the 256-map batch plainly and batched, paused apart, for a kernel trace")
+    ap.add_argument("--tiled-batch-only", action="store_true",
+                    help="only build the tiled map's 512 DAGs as one batch twice, for a kernel trace")
     ap.add_argument("--busy-from", metavar="RESULTS_DB",
                     help="print the GPU-busy fraction of each phase of a rocprofv3 kernel trace, and exit")
     args = ap.parse_args()
@@ -693,6 +719,9 @@ def main():
     ctx = cp.default_context()
     if args.busy_phases:
         print(json.dumps(run_busy_phases(cp, np, ctx)), flush=True)
+        return
+    if args.tiled_batch_only:
+        print(json.dumps(run_tiled_batch_only(cp, ctx)), flush=True)
         return
     N = args.N
     log(f"[bench] generating {N}^2 depth map and {2 * args.width * args.height} queries on the device")
