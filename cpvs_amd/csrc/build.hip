@@ -103,8 +103,7 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
     // only ever reads back its own entries, so the arrays need no barrier).
     __shared__ int s_f[kLeafWaves][kLeafCols][64];
     __shared__ u32 s_b[kLeafWaves][kLeafCols][64];
-    // A chunk's changed-node flags: byte j for node q0 + j, bytes 64..127 a sink for marks outside
-    // the chunk (each lane writes its own sink byte).
+    // A chunk's changed-node flags (leaf_column): byte j + 1 for node q0 + j, byte 65 a sink.
     __shared__ uint8_t s_chg[kLeafWaves][128];
     // The wave index is uniform across the wave; saying so keeps the column bases in scalar registers.
     const u32 lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -201,8 +200,9 @@ __global__ __launch_bounds__(64 * kLeafWaves) void k_leaf_waves(LeafColumns lc, 
             fmin = __ockl_wfred_min_i32(f);
             fmax = __ockl_wfred_max_i32(f);
         }
-        leaf_column(f, fmin, fmax, zs, cnt, s_b[wave][k][lane], lc.bfs + base, 0u, nleaf, s_chg[wave], lane, mask_out,
-                    leafm, runhead);
+        // (with one wave per workgroup its flags sit at a constant LDS address)
+        leaf_column(f, fmin, fmax, zs, cnt, s_b[wave][k][lane], lc.bfs + base, 0u, nleaf,
+                    kLeafWaves == 1 ? s_chg[0] : s_chg[wave], lane, mask_out, leafm, runhead);
     }
     // The leaf level's merge table is cleared here, after all the work (stores issued earlier
     // would be waited for with the loads): this kernel leaves HBM bandwidth to spare.

@@ -275,17 +275,19 @@ __device__ inline void leaf_column(int f, int fmin, int fmax, int zs, u32 cnt, u
     auto chunk = [&](u32 q0, u32 bfs) {
         const u32 qn = min(cnt - q0, 64u);
         const int ra = qa - (int)q0;
-        // Only the chunk's qn nodes are flagged; bytes 64 + lane are each lane's sink.
-        reinterpret_cast<u32*>(fl)[lane & 15u] = 0u;
+        // Node n's flag is byte n + 1 (n = -1 .. 63); every node outside the chunk lands on byte 65.
+        // Bytes 0 .. 67 are cleared first (lanes past 16 repeat the last word).
+        reinterpret_cast<u32*>(fl)[min(lane, 16u)] = 0u;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        fl[((u32)ra < qn) ? (u32)ra : 64u + lane] = 1;
-        fl[(below && (u32)(ra - 1) < qn) ? (u32)(ra - 1) : 64u + lane] = 1;
+        fl[min((u32)(ra + 1), 65u)] = 1;           // node ra
+        fl[below ? min((u32)ra, 65u) : 65u] = 1;   // node ra - 1
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        u64 chg = __ballot(fl[lane] != 0);  // wave-uniform
+        const bool hit = fl[lane + 1] != 0;           // read by every lane (no branch around it)
+        u64 chg = __ballot(hit) & __ballot(lane < qn);  // wave-uniform
         if (q0 == 0) chg |= 1ull;            // a column's first node starts a run
         // The run heads of this chunk, one after the other (wave-uniform): mask and leafmasks (their
         // key hash is computed lane-parallel by the leaf insert, which reads them anyway).
