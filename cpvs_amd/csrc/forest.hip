@@ -299,8 +299,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_forest_leaf(const uint4* __rest
             fmin = __ockl_wfred_min_i32(f);
             fmax = __ockl_wfred_max_i32(f);
         }
-        leaf_column(f, fmin, fmax, zs, cnt, s_b[wave][k][lane], colbfs + base, gtag, nleaf, s_chg[wave], lane, mask_out,
-                    leafm, runhead);
+        leaf_column(f, fmin, fmax, zs, cnt, s_b[wave][k][lane], colbfs + base, gtag, nleaf,
+                    kWaves == 1 ? s_chg[0] : s_chg[wave], lane, mask_out, leafm, runhead);
     }
     clear_table(tab, tab_slots);
 }
