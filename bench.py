@@ -868,8 +868,8 @@ def main():
     roofline = roof(rname)
     if rname == "leaf_waves":
         # The HBM roofline is the ceiling; what holds the kernel below it is instruction issue.
-        roofline["limiter"] = ("instruction issue, the scalar unit most (wave-uniform per-run-head slice counts, "
-                               "masks and leafmask ballots), not HBM bandwidth")
+        roofline["limiter"] = ("instruction issue (SQ counters: vector units 73% and scalar units 57% busy; the "
+                               "per-column flags and wave-uniform per-run-head work), not HBM bandwidth")
         roofline["table_clear_bytes"] = leaf_table_clear_bytes(int(nodes[2]))
     pyramid_roofline = roof("pyr_tile_root")
     lookup_roofline = roof("evaluate4")

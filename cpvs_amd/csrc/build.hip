@@ -65,11 +65,12 @@ __global__ __launch_bounds__(256) void k_classify(const u64* __restrict__ coord,
 //   q = floor(h_t/8) - 1 when h_t mod 8 != 7.  Each texel flags at most two nodes in LDS and a
 //   ballot over the flags read back gives a chunk's changed nodes at once; only those (the run
 //   heads, 1.3 M of 6.7 M leaves at 8192^2) get compares.
-// * The kernel is bound by instruction issue, the scalar unit most (one per CU, shared by its
-//   four SIMDs): the per-run-head work is wave-uniform.  What cut it: the changed nodes from LDS
+// * The kernel is bound by instruction issue: the per-run-head work is wave-uniform, and the
+//   scalar unit is one per CU, shared by its four SIMDs.  What cut it: the changed nodes from LDS
 //   flags and one ballot instead of two wave OR-reductions (0.262 -> 0.244 ms), lane inserts by
 //   v_writelane and run heads propagated head by head (0.239 ms), the wave's columns walked by a
-//   bitmask of those with nodes (0.224 ms), slice counts from per-chunk offsets (0.217 ms).
+//   bitmask of those with nodes (0.224 ms), slice counts from per-chunk offsets (0.217 ms), and
+//   simpler flag addressing (0.209 ms).  The vector units are now the busier (73%, scalar 57%).
 // * Only a run's first node stores its mask and leafmasks: the merge and the final write read
 //   those of first occurrences only, and a key's first occurrence is always the first node of a
 //   run (breadth-first order grows with z inside a column).  Every node stores the breadth-first
