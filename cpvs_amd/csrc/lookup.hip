@@ -427,25 +427,47 @@ __global__ __launch_bounds__(256) void k_dagset_eval(const DagDesc* __restrict__
     vis[i] = dagset_lookup(d, ndc + (size_t)i * stride);
 }
 
-// Round-robin queries (query i -> DAG i mod nset), walked DAG by DAG.  Waves are ordered
-// (group of kRRGroup DAGs, block of 64 queries per DAG, DAG in the group): all lanes of a wave
-// share one DAG's upper levels, the waves resident at any time walk only the kRRGroup DAGs of one
-// group, and since blocks are dealt to the XCDs in turn, each XCD's L2 serves kRRGroup/4 of
-// them.  The waves of one group read neighbouring queries, so input lines are shared in L2.
+// Round-robin queries (query i -> DAG i mod nset), walked DAG by DAG.  A workgroup of kRRGroup
+// waves takes kRRGroup consecutive DAGs (a group) and 64 consecutive rows of queries (row r holds
+// the queries r*nset .. r*nset + nset-1): it loads the tile of positions row by row (each row's
+// group of points is contiguous, so every line is fetched once), one wave per DAG walks its 64
+// queries from LDS (all lanes on one DAG's upper levels), and the results go back row by row
+// through LDS.  Groups are spread over the XCDs in contiguous ranges, so each XCD's L2 holds the
+// upper levels of its own DAGs only.
 constexpr u32 kRRGroup = 16;
 
-__global__ __launch_bounds__(256) void k_dagset_eval_rr(const DagDesc* __restrict__ set, u32 nset, u32 group,
-                                                        u32 kblocks, const float* __restrict__ ndc, u32 stride,
-                                                        u32 count, uint8_t* __restrict__ vis) {
-    const u32 g = blockIdx.x * blockDim.x + threadIdx.x;
-    const u32 w = g >> 6, lane = g & 63;
-    const u32 din = w % group, rest = w / group;
-    const u32 kb = rest % kblocks, grp = rest / kblocks;
-    const u32 dag = grp * group + din;
-    if (dag >= nset) return;
-    const u64 i = ((u64)kb * 64 + lane) * nset + dag;
-    if (i >= count) return;
-    vis[i] = dagset_lookup(set[dag], ndc + (size_t)i * stride);
+__global__ __launch_bounds__(64 * kRRGroup) void k_dagset_eval_rr(const DagDesc* __restrict__ set, u32 nset, u32 ngroups,
+                                                               u32 gper, u32 kblocks, const float* __restrict__ ndc,
+                                                               u32 stride, u64 count, uint8_t* __restrict__ vis) {
+    __shared__ float s_q[kRRGroup][3][64];
+    __shared__ uint8_t s_v[64][kRRGroup];
+    // XCD x (blocks x, x + kXcds, ...) takes groups [x*gper, (x+1)*gper), every row block of each.
+    const u32 x = blockIdx.x % kXcds, c = blockIdx.x / kXcds;
+    const u32 grp = x * gper + c / kblocks, kb = c % kblocks;
+    if (grp >= ngroups || c >= gper * kblocks) return;  // whole workgroup: no barrier is left waiting
+    const u32 t = threadIdx.x;
+    const u32 r = t / kRRGroup, dd = t % kRRGroup;  // tile element of this thread for the load and the store
+    const u32 d = grp * kRRGroup + dd;
+    const u64 i = ((u64)kb * 64 + r) * nset + d;
+    const bool live = d < nset && i < count;
+    if (live) {
+        const float* q = ndc + i * stride;
+        s_q[dd][0][r] = q[0];
+        s_q[dd][1][r] = q[1];
+        s_q[dd][2][r] = q[2];
+    }
+    __syncthreads();
+    {
+        const u32 w = t / 64, lane = t % 64;  // wave w walks DAG grp*kRRGroup + w, lane = row
+        const u32 dw = grp * kRRGroup + w;
+        const u64 iw = ((u64)kb * 64 + lane) * nset + dw;
+        if (dw < nset && iw < count) {
+            const float q[3] = {s_q[w][0][lane], s_q[w][1][lane], s_q[w][2][lane]};
+            s_v[lane][w] = dagset_lookup(set[dw], q);
+        }
+    }
+    __syncthreads();
+    if (live) vis[i] = s_v[r][dd];
 }
 
 // One block of 256 queries of one request: the block finds its request by a binary search over
@@ -479,18 +501,17 @@ int launch_dagset_eval(const DagDesc* set, u32 nset, const DagDesc& one, const u
         return CPVS_EINVAL;
     }
     if (set && !ids) {
-        const u64 per = (count + nset - 1) / nset;  // queries of DAG 0 (the most of any DAG)
+        const u64 per = (count + nset - 1) / nset;  // rows of queries (DAG 0 has the most)
         const u64 kblocks = (per + 63) / 64;
-        const u32 group = nset < kRRGroup ? nset : kRRGroup;
-        const u64 groups = (nset + group - 1) / group;
-        const u64 threads = groups * group * kblocks * 64;  // one wave per DAG per 64 queries
-        const u64 blocks = (threads + 255) / 256;
-        if (blocks * 256 > 0xFFFFFFFFull) {  // thread indices are 32-bit
+        const u64 ngroups = (nset + kRRGroup - 1) / kRRGroup;
+        const u64 gper = (ngroups + kXcds - 1) / kXcds;  // groups per XCD
+        const u64 blocks = gper * kblocks * kXcds;
+        if (blocks > 0x7FFFFFF0ull) {
             set_error("dagset evaluate: %llu queries over %u DAGs exceed one launch", (unsigned long long)count, nset);
             return CPVS_EINVAL;
         }
-        CPVS_LAUNCH("dagset_eval_rr", k_dagset_eval_rr, dim3((u32)blocks), dim3(256), 0, s, set, nset, group, (u32)kblocks,
-                    ndc, stride, (u32)count, vis);
+        CPVS_LAUNCH("dagset_eval_rr", k_dagset_eval_rr, dim3((u32)blocks), dim3(64 * kRRGroup), 0, s, set, nset,
+                    (u32)ngroups, (u32)gper, (u32)kblocks, ndc, stride, count, vis);
     } else {
         CPVS_LAUNCH("dagset_eval", k_dagset_eval, dim3((u32)((count + 255) / 256)), dim3(256), 0, s, set, nset, one, ids,
                     ndc, stride, (u32)count, vis);
