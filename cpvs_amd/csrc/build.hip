@@ -363,18 +363,39 @@ __global__ void k_start(u64* root_coord, u64 root, u32* meta, u32 meta_words) {
 // ---------------------------------------------------------------------------------------
 // 4. Compress.
 // ---------------------------------------------------------------------------------------
-__global__ void k_level_bases(u32* __restrict__ meta, int R, int bottom, const u32* __restrict__ root_mask,
-                              u32* __restrict__ host_meta) {
-    // B[R] = 0, B[L-1] = B[L] + S[L]; total = B[bottom] + S[bottom] (levels absent have S = 0).
-    meta[kMetaRoot] = *root_mask;
+// B[R] = 0, B[L-1] = B[L] + S[L]; total = B[bottom] + S[bottom] (levels absent have S = 0).  One
+// thread; every S is loaded before any base is stored (stored one by one behind each load, the
+// loads would wait for one another).
+__device__ inline void level_bases(u32* meta, int R, int bottom) {
+    u32 S[kMaxLevels];
+#pragma unroll
+    for (int L = 0; L < kMaxLevels; ++L) S[L] = meta[kMetaS + L];
     unsigned long long acc = 0;
-    for (int L = R; L >= bottom; --L) {
-        meta[kMetaB + L] = (u32)acc;
-        acc += meta[kMetaS + L];
+#pragma unroll
+    for (int L = kMaxLevels - 1; L >= 0; --L) {
+        if (L <= R && L >= bottom) {
+            meta[kMetaB + L] = (u32)acc;
+            acc += S[L];
+        }
     }
     meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
-    for (u32 i = 0; i < kMetaWords; ++i) host_meta[i] = meta[i];  // the build's readback
+}
+
+// The build's readback: the metadata block copied to pinned host memory by the whole workgroup,
+// one word per thread (after a barrier, so that thread 0's bases are in).
+__device__ inline void meta_to_host(const u32* meta, u32* host_meta) {
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < kMetaWords; i += blockDim.x) host_meta[i] = meta[i];
     __threadfence_system();
+}
+
+__global__ __launch_bounds__(128) void k_level_bases(u32* __restrict__ meta, int R, int bottom,
+                                                     const u32* __restrict__ root_mask, u32* __restrict__ host_meta) {
+    if (threadIdx.x == 0) {
+        meta[kMetaRoot] = *root_mask;
+        level_bases(meta, R, bottom);
+    }
+    meta_to_host(meta, host_meta);
 }
 
 // Every level's words in one launch: once the offsets and bases are known the levels' words are
@@ -706,16 +727,10 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_merge(SmallArgs a) {
             a.off[a.R][0] = 0;
             a.meta[kMetaS + a.R] = 1u + num_children(root);
             a.meta[kMetaRoot] = root;
-            unsigned long long acc = 0;
-            for (int L = a.R; L >= a.deepest; --L) {
-                a.meta[kMetaB + L] = (u32)acc;
-                acc += a.meta[kMetaS + L];
-            }
-            a.meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
-            for (u32 i = 0; i < kMetaWords; ++i) a.host_meta[i] = a.meta[i];
-            __threadfence_system();
+            level_bases(a.meta, a.R, a.deepest);
         }
     }
+    if (a.sizes_in_merge) meta_to_host(a.meta, a.host_meta);  // uniform: every thread reaches the barrier
 }
 
 // Size scans of the small levels (1 + k words per unique node), then the level bases of all
@@ -744,17 +759,12 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_compress(SmallArgs a) {
         }
         if (threadIdx.x == 0) a.meta[kMetaS + L] = total;
     }
+    __syncthreads();  // every level's size (thread 0 wrote them) before the bases
     if (threadIdx.x == 0) {
         a.meta[kMetaRoot] = a.mask[a.R][0];
-        unsigned long long acc = 0;
-        for (int L = a.R; L >= a.deepest; --L) {
-            a.meta[kMetaB + L] = (u32)acc;
-            acc += a.meta[kMetaS + L];
-        }
-        a.meta[kMetaW] = (acc > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (u32)acc;
-        for (u32 i = 0; i < kMetaWords; ++i) a.host_meta[i] = a.meta[i];  // the build's readback
-        __threadfence_system();
+        level_bases(a.meta, a.R, a.deepest);
     }
+    meta_to_host(a.meta, a.host_meta);
 }
 
 // The DAG's word buffer: from the admission's allocator when it has one (a server keeps its budget
@@ -1115,7 +1125,7 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
     if (Ls <= R) {
         if (!sa.sizes_in_merge) CPVS_LAUNCH("small_compress", k_small_compress, dim3(1), dim3(kSmallThreads), 0, s, sa);
     } else {
-        CPVS_LAUNCH("level_bases", k_level_bases, dim3(1), dim3(1), 0, s, meta, R, deepest, lv[R].mask,
+        CPVS_LAUNCH("level_bases", k_level_bases, dim3(1), dim3(128), 0, s, meta, R, deepest, lv[R].mask,
                     reinterpret_cast<u32*>(pinned));
     }
     // The word writer's per-level arrays (the counts and bases come from meta or the readback).
