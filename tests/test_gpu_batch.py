@@ -249,3 +249,24 @@ def test_uniform_range_generator(ctx):
     assert np.array_equal(got, full[5::7][:1000])
     got4 = cp.gen_queries_uniform_range(SEED0 + 3, 0, 1, 500, 4).numpy()
     assert np.array_equal(got4, O.gen_queries_uniform(SEED0 + 3, 500, 4))
+
+
+def test_dagset_round_robin_many_dags(ctx):
+    """Round-robin lookups over 37 DAGs (three tiles of 16, the last partial; groups on several XCDs)
+    and a ragged query count, with 3- and 4-float points: each DAG's queries as the oracle's walk."""
+    k = 37
+    Ds = [O.gen_depth(16 << (j % 4), SEED0 + 900 + j, 1 + j % 5, j % 2 == 0) for j in range(k)]
+    shadows = [cp.CompressedShadow.create(cp.MinMaxHierarchy(D)) for D in Ds]
+    s = cp.DagSet(shadows)
+    nq = k * 1500 + 17
+    for stride in (3, 4):
+        _say("dagset round robin", k, nq, stride)
+        q = O.gen_queries_uniform(SEED0 + 910 + stride, nq, stride)
+        rr = s.evaluate(cp.DeviceArray.from_numpy(q)).numpy()
+        for j, sh in enumerate(shadows):
+            words, n = sh.getDAG(), sh.getNumLevels()
+            sel = (np.arange(nq) % k) == j
+            assert np.array_equal(rr[sel], O.evaluate(words, O.top_level_grid([words]), n, 0, q[sel], nthreads=8)), j
+    s.close()
+    for sh in shadows:
+        sh.close()
