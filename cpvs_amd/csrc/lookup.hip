@@ -32,8 +32,9 @@ __device__ inline u64 xcd_query(u32 chunk) {
 }
 
 // Threads per workgroup of the flat and PCF lookups (measured on 16.6 M G-buffer queries: 64, 128,
-// 256, 512 and 1024 threads took 0.301, 0.297, 0.289, 0.277 and 0.276 ms).
-constexpr u32 kLookupThreads = 512;
+// 256, 512 and 1024 threads took 0.301, 0.297, 0.289, 0.277 and 0.276 ms walking every level; with
+// the 7-level top table, 512 and 1024 threads take 0.159 and 0.150 ms).
+constexpr u32 kLookupThreads = 1024;
 
 // traverse.cs:43-48: path = int(((ndc + 1) * 0.5) * float(R - 1)), clamped to [0, R-1]
 // (out-of-range positions index outside the grid in the reference).
