@@ -79,6 +79,12 @@ class Context:
         """'table' (default) or 'sort': how builds merge common subtrees (same words)."""
         call("cpvs_ctx_set_merge", self.handle, {"table": _lib.CPVS_MERGE_TABLE, "sort": _lib.CPVS_MERGE_SORT}[mode])
 
+    def set_hash_bits(self, leaf_bits: int, inner_bits: int = 0):
+        """Test hook: cut the table merge's key hashes to their low bits (0 = whole), at the leaf
+        level and above it, so that distinct keys collide; the key compares must catch each
+        collision (the build then sorts: same words)."""
+        call("cpvs_ctx_set_hash_bits", self.handle, leaf_bits, inner_bits)
+
     def set_leafmasks(self, on: bool):
         """The reference's LEAFMASKS compile switch (src/CompressedShadow.cpp:17, shader/traverse.cs:8-9)
         as a mode of this context, on by default.  Off, builds of maps of N >= 16 go down to level 0

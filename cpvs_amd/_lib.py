@@ -37,6 +37,7 @@ _SIGS = {
     "cpvs_ctx_profile": [vp, i32],
     "cpvs_ctx_profile_read": [vp, C.c_char_p, u64],
     "cpvs_ctx_set_merge": [vp, i32],
+    "cpvs_ctx_set_hash_bits": [vp, u32, u32],
     "cpvs_dag_merge_info": [vp, C.POINTER(i32), C.POINTER(u64)],
     "cpvs_ctx_set_leafmasks": [vp, i32],
     "cpvs_ctx_release_scratch": [vp, C.POINTER(u64)],

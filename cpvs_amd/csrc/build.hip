@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u32* _
                                                                     const u32* __restrict__ runhead, u32 n,
                                                                     u64* __restrict__ tab, u32 S, u32* __restrict__ tslot,
                                                                     u32* __restrict__ full, u64* __restrict__ zero,
-                                                                    u32 nzero) {
+                                                                    u32 nzero, u32 hbits) {
     __shared__ u64 gh[kLeafSlots];
     __shared__ u32 gi[kLeafSlots];  // smallest node index of the slot's hash, then its global slot
     __shared__ unsigned short glist[kLeafWindow];  // LDS slots in use
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u32* _
 #pragma unroll
     for (u32 k = 0; k < kLeafPer; ++k) {
         const u32 i = i0 + k * kLeafInsertThreads;
-        h[k] = (rh[k] == i) ? leaf_key_hash(hm[k], leafm + (size_t)i * 8) : 0ull;
+        h[k] = (rh[k] == i) ? weaken_hash(leaf_key_hash(hm[k], leafm + (size_t)i * 8), hbits) : 0ull;
         if (h[k] == 0ull && rh[k] == i) h[k] = 1ull;
     }
     __syncthreads();
@@ -301,11 +301,12 @@ __global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u32* _
 __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
                                                                const u32* __restrict__ child_rank, u32 n, u64* __restrict__ tab,
                                                                u32 S, u32* __restrict__ tslot, u32* __restrict__ full,
-                                                               u64* __restrict__ zero, u32 nzero) {
+                                                               u64* __restrict__ zero, u32 nzero, u32 hbits) {
     clear_partials(zero, nzero);
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < n;
-    const u32 slot = table_insert_group(tab, S, act ? inner_key_hash(mask, cstart, child_rank, i) : 0ull, i, act, full);
+    const u64 h = act ? weaken_hash(inner_key_hash(mask, cstart, child_rank, i), hbits) : 0ull;
+    const u32 slot = table_insert_group(tab, S, h, i, act, full);
     if (act) tslot[i] = slot;
 }
 
@@ -1068,13 +1069,14 @@ int build_once(cpvs_ctx* ctx, Arena& A, u64* pinned, const cpvs_pyramid* p, u32 
             if (L == deepest && !is_leaf) CPVS_LAUNCH("table_clear", k_table_clear, grid1d(Sl), dim3(256), 0, s, tab, Sl);
             if (is_leaf) {
                 CPVS_LAUNCH("insert_leaf", k_insert_leaf, dim3(ceil_div<u32>(c, kLeafWindow)), dim3(kLeafInsertThreads), 0, s,
-                            lv[2].mask, leafm, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
+                            lv[2].mask, leafm, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero,
+                            ctx->leaf_hash_bits.load());
                 CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, runhead, c,
                             LeafEq{lv[2].mask, leafm, nullptr, c}, rep, meta + kMetaColl, lv[2].mask, 1u, rpart, rrounds);
             } else {
                 CPVS_LAUNCH("insert_inner", k_insert_inner, grid1d(c, kInsertBlock), dim3(kInsertBlock), 0, s, lv[L].mask,
                             lv[L].cstart, child_rank, c,
-                            tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
+                            tab, Sl, tslot, meta + kMetaFull, rpart, nzero, ctx->inner_hash_bits.load());
                 CPVS_LAUNCH("resolve", k_resolve<InnerEq>, grid1d(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab, nullptr, c,
                             InnerEq{lv[L].mask, lv[L].cstart, child_rank}, rep, meta + kMetaColl, lv[L].mask, 0u, rpart, rrounds);
             }

@@ -360,6 +360,14 @@ __device__ inline u64 inner_key_hash(const u32* mask, const u32* cstart, const u
     return h;
 }
 
+// A merge hash cut to its low `bits` bits, in both the tag and the slot position, so that distinct
+// keys collide (bits = 0: the full hash).  Builds use it only to exercise collision detection.
+__device__ inline u64 weaken_hash(u64 h, u32 bits) {
+    if (!bits) return h;
+    const u32 t = (u32)h & ((1u << bits) - 1u);
+    return ((u64)t << 32) | t;
+}
+
 // Run heads are bounded by the level's n nodes where they are used as indices:
 // past a level's emitted children (counts that exceed them, which the host then reports) the
 // entries were never written.
@@ -738,11 +746,11 @@ __global__ __launch_bounds__(kLeafInsertThreads) void k_insert_leaf(const u32* _
                                                                     const u32* __restrict__ runhead, u32 n,
                                                                     u64* __restrict__ tab, u32 S, u32* __restrict__ tslot,
                                                                     u32* __restrict__ full, u64* __restrict__ zero,
-                                                                    u32 nzero);
+                                                                    u32 nzero, u32 hbits);
 __global__ __launch_bounds__(kInsertBlock) void k_insert_inner(const u32* __restrict__ mask, const u32* __restrict__ cstart,
                                                                const u32* __restrict__ child_rank, u32 n, u64* __restrict__ tab,
                                                                u32 S, u32* __restrict__ tslot, u32* __restrict__ full,
-                                                               u64* __restrict__ zero, u32 nzero);
+                                                               u64* __restrict__ zero, u32 nzero, u32 hbits);
 __global__ __launch_bounds__(256) void k_sortkeys_leaf(const u32* __restrict__ mask, const u64* __restrict__ leafm,
                                                        const u32* __restrict__ runhead,
                                                        u32 n, u32* __restrict__ skey, u32* __restrict__ sval);

@@ -328,6 +328,14 @@ int cpvs_ctx_set_leafmasks(cpvs_ctx* ctx, int on) {
     return CPVS_OK;
 }
 
+int cpvs_ctx_set_hash_bits(cpvs_ctx* ctx, uint32_t leaf_bits, uint32_t inner_bits) {
+    CHECK_ARG(ctx, "ctx_set_hash_bits: NULL ctx");
+    CHECK_ARG(leaf_bits < 32 && inner_bits < 32, "ctx_set_hash_bits: %u, %u (0 .. 31)", leaf_bits, inner_bits);
+    ctx->leaf_hash_bits.store(leaf_bits);
+    ctx->inner_hash_bits.store(inner_bits);
+    return CPVS_OK;
+}
+
 int cpvs_ctx_leafmasks(const cpvs_ctx* ctx, int* on) {
     CHECK_ARG(ctx && on, "ctx_leafmasks: NULL argument");
     *on = ctx->leafmasks.load();

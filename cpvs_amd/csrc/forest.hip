@@ -714,13 +714,15 @@ int forest_once(cpvs_ctx* ctx, Worker& w, const ForestIn* in, u32 M, const Fores
                 if (!is_leaf && prev < 0) CPVS_LAUNCH("table_clear", k_table_clear, grid_of(Sl, 256), dim3(256), 0, s, tab, Sl);
                 if (is_leaf) {
                     CPVS_LAUNCH("insert_leaf", k_insert_leaf, dim3(ceil_div<u32>(c, kLeafWindow)), dim3(kLeafInsertThreads), 0, s,
-                                lv.mask[2], leafm, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
+                                lv.mask[2], leafm, runhead, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero,
+                                ctx->leaf_hash_bits.load());
                     CPVS_LAUNCH("resolve", k_resolve<LeafEq>, grid_of(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab,
                                 runhead, c, LeafEq{lv.mask[2], leafm, nullptr, c}, rep, meta + kMetaColl, lv.mask[2], 1u,
                                 rpart, rrounds);
                 } else {
                     CPVS_LAUNCH("insert_inner", k_insert_inner, grid_of(c, kInsertBlock), dim3(kInsertBlock), 0, s, lv.mask[L],
-                                lv.cstart[L], child_rank, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero);
+                                lv.cstart[L], child_rank, c, tab, Sl, tslot, meta + kMetaFull, rpart, nzero,
+                                ctx->inner_hash_bits.load());
                     CPVS_LAUNCH("resolve", k_resolve<InnerEq>, grid_of(c, kNodeThreads), dim3(kNodeThreads), 0, s, tslot, tab,
                                 nullptr, c, InnerEq{lv.mask[L], lv.cstart[L], child_rank}, rep, meta + kMetaColl, lv.mask[L],
                                 0u, rpart, rrounds);

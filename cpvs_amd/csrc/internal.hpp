@@ -141,6 +141,10 @@ struct cpvs_ctx {
     // default: builds of maps of numLevels >= 5 end in 17-word leafmask nodes at level 2; off, they
     // go down to level 0 (cpvs_ctx_set_leafmasks).
     std::atomic<int> leafmasks{1};
+    // Merge hashes of the table path cut to this many low bits (0: full 64-bit hashes), at the leaf
+    // level and at the levels above it, so that distinct keys collide and the full-key compares are
+    // exercised (cpvs_ctx_set_hash_bits).
+    std::atomic<uint32_t> leaf_hash_bits{0}, inner_hash_bits{0};
     std::vector<cpvs::Worker*> lanes;  // batch build lanes, created on first use and kept
 };
 

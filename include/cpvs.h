@@ -65,6 +65,12 @@ int cpvs_ctx_profile(cpvs_ctx *ctx, int mode);
  * different keys that the table path cannot tell apart (equal hash tags in one probe chain) are
  * detected by the full-key compare, and the build is redone with the sort path. */
 int cpvs_ctx_set_merge(cpvs_ctx *ctx, int mode);
+/* Test hook with no counterpart in the reference (whose linear search compares whole keys): the
+ * table path's 64-bit key hashes cut to their low bits (1 .. 31; 0, the default, keeps them
+ * whole), leaf_bits at the leafmask level and inner_bits above it, so that distinct keys share
+ * hashes.  The full-key compares must then detect every such collision and send the build to the
+ * sort path, with the same words. */
+int cpvs_ctx_set_hash_bits(cpvs_ctx *ctx, uint32_t leaf_bits, uint32_t inner_bits);
 /* The reference's compile-time LEAFMASKS switch (#define LEAFMASKS, src/CompressedShadow.cpp:17;
  * shader/traverse.cs:8-9, MIN_LEVEL 34-38), as a run-time mode of the context, on (1) by default.
  * Off (0), builds of maps of numLevels >= 5 go down to level 0 with 1+k-word nodes (useLeafmasks
