@@ -241,7 +241,7 @@ def run_c0(cp, np, ctx, N=1024, boxes=32, nq=1 << 20, reps=20):
     vis = cp.DeviceArray((nq,), np.uint8, ctx)
     mm = cp.MinMaxHierarchy(d, borrow=True)
     c = cp.CompressedShadowContainer(cp.CompressedShadow.create(mm))
-    top_set, top_ms = c.tuneTopLevels(q, candidates=(-1, 0, 4, 5, 6, 7), visibilities=vis)  # untimed
+    top_set, top_ms = c.tuneTopLevels(q, candidates=(-1, 0, 4, 5, 6, 7, 8), visibilities=vis)  # untimed
     c.close()
     mm.close()
     bms, lms = [], []
@@ -754,7 +754,7 @@ def main():
     if args.top_levels is None:
         mm = cp.MinMaxHierarchy(depth, borrow=True)
         c = cp.CompressedShadowContainer(cp.CompressedShadow.create(mm))
-        top_set, top_ms = c.tuneTopLevels(queries, candidates=(-1, 0, 4, 5, 6, 7), visibilities=vis)
+        top_set, top_ms = c.tuneTopLevels(queries, candidates=(-1, 0, 4, 5, 6, 7, 8), visibilities=vis)
         c.close()
         mm.close()
         log(f"[bench] top table: tuner picked {top_set} ({ {k: round(v, 4) for k, v in top_ms.items()} } ms)")
