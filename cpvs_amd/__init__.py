@@ -331,7 +331,7 @@ class CompressedShadow:
         return int(self.traverse_many(np.asarray(position, np.float32).reshape(1, 3), tryLeafmasks)[0])
 
     def setTopLevels(self, levels: int):
-        """The DAG's own lookup top table (cpvs_dag_set_top_levels): -1 automatic, 0 none, 1..8."""
+        """The DAG's own lookup top table (cpvs_dag_set_top_levels): -1 automatic, 0 none, 1..9."""
         call("cpvs_dag_set_top_levels", self._h(), levels)
 
     def top_levels(self) -> int:
@@ -425,7 +425,7 @@ class CompressedShadowContainer:
         call("cpvs_container_set_filter_size", self.handle, size)
 
     def setTopLevels(self, levels: int):
-        """Levels of the lookup top table built by copyToGPU: -1 automatic, 0 none, 1..8."""
+        """Levels of the lookup top table built by copyToGPU: -1 automatic, 0 none, 1..9."""
         call("cpvs_container_set_top_levels", self.handle, levels)
 
     def top_levels(self) -> int:

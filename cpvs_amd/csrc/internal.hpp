@@ -19,7 +19,7 @@ constexpr uint32_t kMaxFilterSize = 64;  // PCF kernel width limit (64 x 64 look
 // A container's lookup top table: at most this many levels, and (chosen automatically) at most
 // kTopTableBytes, one XCD's L2, so that lookups in any order find it cached; an explicit request
 // (coherent queries gain from more levels) may take up to kTopTableMaxBytes.
-constexpr int kMaxTopLevels = 8;
+constexpr int kMaxTopLevels = 9;
 constexpr uint64_t kTopTableBytes = 4ull << 20;
 constexpr uint64_t kTopTableMaxBytes = 1ull << 30;
 

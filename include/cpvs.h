@@ -165,7 +165,7 @@ int cpvs_container_copy_to_gpu(cpvs_container *c);
 /* No counterpart in the reference (its lookup walks every level, shader/traverse.cs:89-133).
  * copy_to_gpu also builds a top table: per grid cell, the walk's state after its first K levels,
  * which the lookups read in place of walking them (same results).  levels -1 (default): K
- * automatic (the most levels within 4 MiB, one XCD's L2); 0: no table; 1..8: K = min(levels, the DAGs' inner
+ * automatic (the most levels within 4 MiB, one XCD's L2); 0: no table; 1..9: K = min(levels, the DAGs' inner
  * levels), a table of at most 1 GiB.  Takes effect at the next copy_to_gpu.  top_levels reports
  * the K of the table built (0: none). */
 int cpvs_container_set_top_levels(cpvs_container *c, int levels);
@@ -297,7 +297,7 @@ int cpvs_dagset_destroy(cpvs_dagset *s);
 int cpvs_dag_evaluate(const cpvs_dag *dag, const float *ndc, uint32_t stride, uint64_t count, uint8_t *vis,
                       void *stream);
 /* No counterpart in the reference: a lookup top table of the DAG's own, as a one-cell container
- * builds (cpvs_container_set_top_levels; levels -1 automatic, 0 none, 1..8), read by
+ * builds (cpvs_container_set_top_levels; levels -1 automatic, 0 none, 1..9), read by
  * cpvs_dag_evaluate, by DAG sets created after the call and by server lookups; results are the
  * same with any setting.  Built on the context stream, synchronously; call it while no lookup on
  * the DAG is queued.  A replaced table is kept until the DAG is destroyed (DAG sets created earlier

@@ -258,7 +258,7 @@ public:
         for (auto& v : views_) v.reset();
     }
     void setFilterSize(uint size) { check(cpvs_container_set_filter_size(c_, size), "cpvs_container_set_filter_size"); }
-    // Lookup top table levels for the next copyToGPU (-1 automatic, 0 none, 1..8); no counterpart in
+    // Lookup top table levels for the next copyToGPU (-1 automatic, 0 none, 1..9); no counterpart in
     // the reference, results are the same with any setting.
     void setTopLevels(int levels) { check(cpvs_container_set_top_levels(c_, levels), "cpvs_container_set_top_levels"); }
     int topLevels() const {

@@ -10,7 +10,7 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 SEED0 = 0x43505653
-MAX_K = 8
+MAX_K = 9
 AUTO_BYTES = 4 << 20
 
 
@@ -48,7 +48,7 @@ def _sweep(c, q, want, dl, cells, leaf, ctx=None, pcf=False, image=False):
     """Every top-table setting gives `want` (and one PCF / image answer across settings)."""
     dq = cp.DeviceArray.from_numpy(q, ctx=ctx)
     pcf0 = None
-    for req in [0, 1, 2, 3, 4, 5, 6, 7, 8, -1]:
+    for req in [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, -1]:
         if cells * 8 ** min(req, MAX_K) * 4 > (1 << 30):
             continue
         c.setTopLevels(req)
@@ -141,7 +141,7 @@ def test_top_levels_arguments(ctx):
             c.setTopLevels(bad)
     assert c.top_levels() == 0  # nothing built before copyToGPU
     c.close()
-    big = cp.CompressedShadowContainer(8)  # 512 cells x 8^8 entries: beyond the 1 GiB limit
+    big = cp.CompressedShadowContainer(8)  # 512 cells x 8^9 entries: beyond the 1 GiB limit
     with pytest.raises(cp.CpvsError):
         big.setTopLevels(MAX_K)
     big.setTopLevels(5)
@@ -162,7 +162,7 @@ def test_tuner_keeps_the_fastest_and_results(ctx):
     assert c.top_levels() == _expected_k(best, n, 1, True)
     assert np.array_equal(c.evaluate(dq).numpy(), O.evaluate(w, O.top_level_grid([w]), n, 0, q, nthreads=8))
     with pytest.raises(cp.CpvsError):
-        c.tuneTopLevels(dq, candidates=(0, 9))  # 9 levels: rejected before any timing
+        c.tuneTopLevels(dq, candidates=(0, MAX_K + 1))  # beyond the maximum: rejected before any timing
     assert c.top_levels() == _expected_k(best, n, 1, True)
     c.freeOnCPU()
     with pytest.raises(cp.CpvsError):
