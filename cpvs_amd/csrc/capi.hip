@@ -646,7 +646,7 @@ int cpvs_dag_set_top_levels(cpvs_dag* d, int levels) {
     const int lead = d->n - 1 - (d->leafmasks ? 3 : 1);
     int K = (levels < 0) ? std::min(lead, kMaxTopLevels) : std::min(levels, lead);
     if (levels < 0) {
-        while (K > 0 && (sizeof(u32) << (3 * K)) > kTopTableBytes) --K;
+        while (K > 0 && (sizeof(u64) << (3 * K)) > kTopTableBytes) --K;
         if (K < 3) K = 0;
     }
     // A root that is all lit or all shadow is answered before any walk; offsets must stay below
@@ -656,9 +656,9 @@ int cpvs_dag_set_top_levels(cpvs_dag* d, int levels) {
     GUARD_BEGIN
     std::lock_guard<std::mutex> lk(ctx->mu);
     CPVS_HIP_CHECK(hipSetDevice(ctx->device));
-    u32* nt = nullptr;
+    u64* nt = nullptr;
     if (K > 0) {
-        nt = static_cast<u32*>(ctx->buffers.take(sizeof(u32) << (3 * K)));
+        nt = static_cast<u64*>(ctx->buffers.take(sizeof(u64) << (3 * K)));
         if (!nt) {
             set_error("dag_set_top_levels: allocation of a %d-level table failed", K);
             return CPVS_ENOMEM;
@@ -690,7 +690,7 @@ int cpvs_dag_destroy(cpvs_dag* d) {
     hipSetDevice(d->ctx->device);
     if (d->d_words) d->ctx->buffers.give(d->d_words);
     if (d->d_top) d->ctx->buffers.give(d->d_top);
-    for (u32* t : d->retired_tops) d->ctx->buffers.give(t);
+    for (u64* t : d->retired_tops) d->ctx->buffers.give(t);
     delete d;
     return CPVS_OK;
 }
@@ -744,7 +744,7 @@ int cpvs_container_set_top_levels(cpvs_container* c, int levels) {
     CHECK_ARG(c, "container_set_top_levels: NULL container");
     CHECK_ARG(levels >= -1 && levels <= kMaxTopLevels, "container_set_top_levels: %d levels (-1..%d)", levels,
               kMaxTopLevels);
-    CHECK_ARG(levels < 0 || ((u64)c->dags.size() << (3 * levels)) * sizeof(u32) <= kTopTableMaxBytes,
+    CHECK_ARG(levels < 0 || ((u64)c->dags.size() << (3 * levels)) * sizeof(u64) <= kTopTableMaxBytes,
               "container_set_top_levels: %d levels over %zu cells exceed %llu bytes", levels, c->dags.size(),
               (unsigned long long)kTopTableMaxBytes);
     c->top_request = levels;
@@ -821,12 +821,12 @@ int cpvs_container_copy_to_gpu(cpvs_container* c) {
     const int lead = dl - 1 - (c->dags[0]->leafmasks ? 3 : 1);
     int K = (c->top_request < 0) ? std::min(lead, kMaxTopLevels) : std::min(c->top_request, lead);
     if (c->top_request < 0) {
-        while (K > 0 && ((u64)cells << (3 * K)) * sizeof(u32) > kTopTableBytes) --K;
+        while (K > 0 && ((u64)cells << (3 * K)) * sizeof(u64) > kTopTableBytes) --K;
         if (K < 3) K = 0;
     }
     if (K > 0 && total < 0xFFFFFFFDull) {
         const u64 entries = (u64)cells << (3 * K);
-        if (hipMalloc(&c->d_top, sizeof(u32) * entries) != hipSuccess) {
+        if (hipMalloc(&c->d_top, sizeof(u64) * entries) != hipSuccess) {
             (void)hipGetLastError();
             set_error("container_copy_to_gpu: hipMalloc of the %llu-entry top table failed",
                       (unsigned long long)entries);

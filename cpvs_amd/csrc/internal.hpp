@@ -172,9 +172,9 @@ struct cpvs_dag {
     int merge_mode = CPVS_MERGE_TABLE;        // merge path that produced the words
     uint32_t key_collisions = 0;              // 32-bit sort-key collisions resolved by scanning
     bool leafmasks = false;                   // level 2 holds leafmask nodes (lookups stop there)
-    uint32_t* d_top = nullptr;                // lookup top table (cpvs_dag_set_top_levels), or null
+    uint64_t* d_top = nullptr;                // lookup top table (cpvs_dag_set_top_levels), or null
     uint32_t top_levels = 0;
-    std::vector<uint32_t*> retired_tops;      // replaced tables, which DAG sets made earlier still name
+    std::vector<uint64_t*> retired_tops;      // replaced tables, which DAG sets made earlier still name
 };
 
 struct cpvs_container {
@@ -187,7 +187,7 @@ struct cpvs_container {
     uint32_t* d_grid = nullptr;  // length^3 entries
     int dag_levels = 0, grid_levels = 0;
     bool leafmasks = true;       // the DAGs' mode (all alike)
-    uint32_t* d_top = nullptr;   // the lookups' top table (launch_build_top), or null
+    uint64_t* d_top = nullptr;   // the lookups' top table (launch_build_top), or null
     uint32_t top_levels = 0;     // its levels (0: none)
     int top_request = -1;        // cpvs_container_set_top_levels: -1 automatic
     bool on_gpu = false;
@@ -264,15 +264,15 @@ struct Mat4 {
 };
 // leaf: the DAGs end in leafmask nodes at level 2; 0: they go down to level 0.  top (may be null):
 // the container's top table of topk levels (launch_build_top).
-int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u64* top, u32 topk,
                     const float* pos, u32 stride, u64 count, const Mat4& m, uint8_t* vis, hipStream_t s);
-int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u64* top, u32 topk,
                         const float* pos, u32 stride, u64 count, const Mat4& m, u32 f, float* vis, hipStream_t s);
-int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u64* top, u32 topk,
                           const float* pos, u32 width, u32 height, const Mat4& m, uint8_t* vis, hipStream_t s);
 // The top table of a container: cells x 8^K entries (the walk's state after K levels per cell);
 // grid null: one DAG of its own (cells 1, base 0).
-int launch_build_top(const u32* dag, u32 words, const u32* grid, u32 cells, int dl, u32 K, u32* top, hipStream_t s);
+int launch_build_top(const u32* dag, u32 words, const u32* grid, u32 cells, int dl, u32 K, u64* top, hipStream_t s);
 // One DAG of a set, walked as a one-cell container.  base is 0, or the grid sentinel
 // 0xFFFFFFFF / 0xFFFFFFFE when the root is all shadow / all lit.
 struct DagDesc {
@@ -281,7 +281,7 @@ struct DagDesc {
     u32 base;
     u32 dl;    // numLevels
     u32 leaf;  // 1: leafmask nodes at level 2; 0: the DAG goes down to level 0
-    const u32* top;  // the DAG's lookup top table (launch_build_top, one cell), or null
+    const u64* top;  // the DAG's lookup top table (launch_build_top, one cell), or null
     u32 topk;        // its levels
 };
 // The descriptor of a built DAG (createTopLevelGrid sentinels for a one-cell grid,

@@ -18,7 +18,7 @@ struct GridDag {
     int dl, gl;
     u32 words;  // DAG word count: every read is bounds-checked against it
     u32 leaf;   // 1: leafmask nodes at level 2; 0: the DAG goes down to level 0
-    const u32* __restrict__ top;  // the walk's state after the top topk levels (null: none)
+    const u64* __restrict__ top;  // the walk's state after the top topk levels (null: none)
     u32 topk;
 };
 
@@ -53,25 +53,24 @@ __device__ inline u32 quantise(float v, u32 R) {
 // without, the inner levels go down to level 0, whose children are single voxels, lit or shadow
 // (absoluteVisible, src/CompressedShadowUtil.cpp:34-40), never partial; a walk that passed level 0
 // would end visible (traverse.cs:130-132).
-// walk_from continues a walk at node `off` of level level0 (walk_dag starts at the root).
-__device__ inline u32 walk_from(const u32* __restrict__ dag, u32 words, u32 leaf, u32 base, u32 off, int level0,
+// walk_node continues a walk at node `off` of level level0, whose mask m (= dag[off], off < words)
+// the caller has read; walk_from reads it first (walk_dag starts at the root).
+__device__ inline u32 walk_node(const u32* __restrict__ dag, u32 words, u32 leaf, u32 base, u32 off, u32 m, int level0,
                                 u32 px, u32 py, u32 pz) {
     const int last = leaf ? 3 : 0;
     for (int level = level0; level >= last; --level) {
         const u32 ci = (((px >> level) & 1u) << 1) | (((py >> level) & 1u) << 2) | (((pz >> level) & 1u) << 3);
-        if (off >= words) return 0u;
-        const u32 m = dag[off];
         const u32 v = (m >> ci) & 3u;
         if (v != 2u) return v;  // 0 shadow, 1 visible (3 never occurs)
         if (level == 0) return 1u;  // a partial voxel does not occur
         const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
         if (off + 1 + co >= words) return 0u;
         off = base + dag[off + 1 + co];
+        if (off >= words) return 0u;
+        m = dag[off];
     }
     if (!leaf) return 1u;
-    if (off >= words) return 0u;
     const u32 ci = (pz & 7u) * 2u;
-    const u32 m = dag[off];
     const u32 v = (m >> ci) & 3u;
     if (v != 2u) return v;
     const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
@@ -81,17 +80,23 @@ __device__ inline u32 walk_from(const u32* __restrict__ dag, u32 words, u32 leaf
     const u32 w = dag[idx];
     return (w >> (bit & 31u)) & 1u;
 }
+__device__ inline u32 walk_from(const u32* __restrict__ dag, u32 words, u32 leaf, u32 base, u32 off, int level0,
+                                u32 px, u32 py, u32 pz) {
+    if (off >= words) return 0u;
+    return walk_node(dag, words, leaf, base, off, dag[off], level0, px, py, pz);
+}
 __device__ inline u32 walk_dag(const u32* __restrict__ dag, u32 words, int dl, u32 leaf, u32 base, u32 px, u32 py,
                                u32 pz) {
     return walk_from(dag, words, leaf, base, base, dl - 2, px, py, pz);
 }
 
 // A container's top table: for every grid cell and every cell of its DAG's level dl-2-K (K = topk),
-// the result of walking the K levels above it: the node offset there, or the value a walk that
-// stopped earlier returns (kTopShadow 0, kTopVisible 1, kTopThree 3 for a child state 3, which no
-// valid DAG holds).  Entry (cell, t) with t = z * 4^K + y * 2^K + x over the K path bits below
-// the cell's.  A lookup reads one entry instead of walking K levels: the same result by
-// construction.  Valid offsets lie below kTopThree (the container checks its word count).
+// the result of walking the K levels above it: the node offset there (low half) with that node's
+// mask (high half), or the value a walk that stopped earlier returns (kTopShadow 0, kTopVisible 1,
+// kTopThree 3 for a child state 3, which no valid DAG holds; high half 0).  Entry (cell, t) with
+// t = z * 4^K + y * 2^K + x over the K path bits below the cell's.  A lookup reads one 8-byte entry
+// instead of walking K levels and reading the next node's mask: the same result by construction.
+// Valid offsets lie below kTopThree (the container checks its word count).
 constexpr u32 kTopShadow = 0xFFFFFFFFu, kTopVisible = 0xFFFFFFFEu, kTopThree = 0xFFFFFFFDu;
 
 // One thread per entry of level K-1 (the parent): the walk over the parent's K-1 path bits, then
@@ -99,7 +104,7 @@ constexpr u32 kTopShadow = 0xFFFFFFFFu, kTopVisible = 0xFFFFFFFEu, kTopThree = 0
 // partial child).  Each child's value is the one a per-entry walk of all K bits gives, checks
 // included.
 __global__ __launch_bounds__(256) void k_build_top(const u32* __restrict__ dag, u32 words, const u32* __restrict__ grid,
-                                                   u64 parents, u32 K, u32* __restrict__ top) {
+                                                   u64 parents, u32 K, u64* __restrict__ top) {
     const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= parents) return;
     const u32 P = K - 1, pm = (1u << P) - 1u;
@@ -137,30 +142,36 @@ __global__ __launch_bounds__(256) void k_build_top(const u32* __restrict__ dag, 
     }
     const bool inside = node && e < words;
     const u32 m = inside ? dag[e] : 0u;
-    u32* out = top + ((size_t)cell << (3 * K));
+    u64* out = top + ((size_t)cell << (3 * K));
+    u32 r[8];
 #pragma unroll
     for (u32 c = 0; c < 8; ++c) {
         const u32 cx = c & 1u, cy = (c >> 1) & 1u, cz = c >> 2;
-        const u32 idx = (((2 * pz + cz) << (2 * K)) | ((2 * py + cy) << K)) | (2 * px + cx);
-        u32 r = node ? kTopShadow : e;  // a parent past the words: shadow, as walk_from's first check
+        r[c] = node ? kTopShadow : e;  // a parent past the words: shadow, as walk_from's first check
         if (inside) {
             const u32 ci = (cx << 1) | (cy << 2) | (cz << 3);
             const u32 v = (m >> ci) & 3u;
             if (v != 2u) {
-                r = (v == 0u) ? kTopShadow : (v == 1u) ? kTopVisible : kTopThree;
+                r[c] = (v == 0u) ? kTopShadow : (v == 1u) ? kTopVisible : kTopThree;
             } else {
                 const u32 co = (u32)__builtin_popcount(m & 0xAAAAu & ((1u << ci) - 1u));
                 if (e + 1 + co < words) {
                     const u32 nx = base + dag[e + 1 + co];
-                    r = (nx < words) ? nx : kTopShadow;  // walk_from's first check on it
+                    r[c] = (nx < words) ? nx : kTopShadow;  // walk_from's first check on it
                 }
             }
         }
-        out[idx] = r;
+    }
+#pragma unroll
+    for (u32 c = 0; c < 8; ++c) {
+        const u32 cx = c & 1u, cy = (c >> 1) & 1u, cz = c >> 2;
+        const u32 idx = (((2 * pz + cz) << (2 * K)) | ((2 * py + cy) << K)) | (2 * px + cx);
+        const u32 cm = (r[c] < kTopThree) ? dag[r[c]] : 0u;  // the child node's mask (r[c] < words)
+        out[idx] = ((u64)cm << 32) | r[c];
     }
 }
 
-int launch_build_top(const u32* dag, u32 words, const u32* grid, u32 cells, int dl, u32 K, u32* top, hipStream_t s) {
+int launch_build_top(const u32* dag, u32 words, const u32* grid, u32 cells, int dl, u32 K, u64* top, hipStream_t s) {
     (void)dl;
     if (K == 0) return CPVS_OK;
     const u64 parents = (u64)cells << (3 * (K - 1));
@@ -182,9 +193,10 @@ __device__ inline u32 walk(const GridDag& g, u32 px, u32 py, u32 pz) {
     if (g.topk) {
         const u32 K = g.topk, s = sh - K, msk = (1u << K) - 1u;
         const u32 t = (((pz >> s) & msk) << (2 * K)) | (((py >> s) & msk) << K) | ((px >> s) & msk);
-        const u32 e = g.top[((size_t)cell << (3 * K)) + t];
+        const u64 te = g.top[((size_t)cell << (3 * K)) + t];
+        const u32 e = (u32)te;
         if (e >= kTopThree) return (e == kTopShadow) ? 0u : (e == kTopVisible) ? 1u : 3u;
-        return walk_from(g.dag, g.words, g.leaf, base, e, g.dl - 2 - (int)K, px, py, pz);
+        return walk_node(g.dag, g.words, g.leaf, base, e, (u32)(te >> 32), g.dl - 2 - (int)K, px, py, pz);
     }
     return walk_dag(g.dag, g.words, g.dl, g.leaf, base, px, py, pz);
 }
@@ -244,7 +256,7 @@ __global__ __launch_bounds__(kLookupThreads) void k_evaluate_pcf(GridDag g, Mat4
     vis[i] = (float)lit / (float)(f * f);
 }
 
-int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+int launch_evaluate_pcf(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u64* top, u32 topk,
                         const float* pos, u32 stride, u64 count, const Mat4& m, u32 f, float* vis, hipStream_t s) {
     if (count == 0) return CPVS_OK;
     GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u, top, top ? topk : 0u};
@@ -294,7 +306,7 @@ __global__ __launch_bounds__(256) void k_evaluate_image(GridDag g, Mat4 M, const
     vis[i] = (uint8_t)lookup_point(g, M, q.x, q.y, q.z);
 }
 
-int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u64* top, u32 topk,
                     const float* pos, u32 stride, u64 count, const Mat4& m, uint8_t* vis, hipStream_t s) {
     if (count == 0) return CPVS_OK;
     GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u, top, top ? topk : 0u};
@@ -313,7 +325,7 @@ int launch_evaluate(const u32* dag, u32 words, const u32* grid, int dl, int gl, 
     return CPVS_OK;
 }
 
-int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u32* top, u32 topk,
+int launch_evaluate_image(const u32* dag, u32 words, const u32* grid, int dl, int gl, int leaf, const u64* top, u32 topk,
                           const float* pos, u32 width, u32 height, const Mat4& m, uint8_t* vis, hipStream_t s) {
     if (width == 0 || height == 0) return CPVS_OK;
     GridDag g{dag, grid, dl, gl, words, leaf ? 1u : 0u, top, top ? topk : 0u};
@@ -403,9 +415,10 @@ __device__ inline uint8_t dagset_lookup(const DagDesc& d, const float* __restric
     if (d.topk) {  // the DAG's top table: the first topk levels in one read (as a one-cell container's)
         const u32 K = d.topk, s = d.dl - 1 - K, msk = (1u << K) - 1u;
         const u32 t = (((pz >> s) & msk) << (2 * K)) | (((py >> s) & msk) << K) | ((px >> s) & msk);
-        const u32 e = d.top[t];
+        const u64 te = d.top[t];
+        const u32 e = (u32)te;
         if (e >= kTopThree) return (e == kTopShadow) ? 0 : (e == kTopVisible) ? 1 : 3;
-        return (uint8_t)walk_from(d.words, d.nwords, d.leaf, 0u, e, (int)d.dl - 2 - (int)K, px, py, pz);
+        return (uint8_t)walk_node(d.words, d.nwords, d.leaf, 0u, e, (u32)(te >> 32), (int)d.dl - 2 - (int)K, px, py, pz);
     }
     return (uint8_t)walk_dag(d.words, d.nwords, (int)d.dl, d.leaf, 0u, px, py, pz);
 }

@@ -22,7 +22,7 @@ def _expected_k(req, dl, cells, leaf):
     lead = dl - 1 - (3 if leaf else 1)
     if req < 0:
         k = min(lead, MAX_K)
-        while k > 0 and cells * 8 ** k * 4 > AUTO_BYTES:
+        while k > 0 and cells * 8 ** k * 8 > AUTO_BYTES:
             k -= 1
         if k < 3:
             k = 0
@@ -49,7 +49,7 @@ def _sweep(c, q, want, dl, cells, leaf, ctx=None, pcf=False, image=False):
     dq = cp.DeviceArray.from_numpy(q, ctx=ctx)
     pcf0 = None
     for req in [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, -1]:
-        if cells * 8 ** min(req, MAX_K) * 4 > (1 << 30):
+        if cells * 8 ** min(req, MAX_K) * 8 > (1 << 30):
             continue
         c.setTopLevels(req)
         c.copyToGPU()
